@@ -1,0 +1,302 @@
+/* mjpbr.h — C ABI of libmjpbr_hip.so, the MI355X (gfx950) renderer behind the
+ * mujoco_warp_pbr_amd Python package.
+ *
+ * Every pointer in the structs below is a DEVICE pointer (HBM) when the struct is passed to an
+ * mjpbr_* entry point of libmjpbr_hip.so. The CPU oracle (oracle/, test infrastructure only)
+ * reads the same structs with host pointers.
+ *
+ * Conventions (mirroring the reference's Python API, mujoco_warp_pbr/_src/types.py:126-132):
+ *   - "batched" model arrays carry a leading size `*_n` (1 or nworld) and are indexed
+ *     `worldid % *_n`; per-frame data arrays (geom_xpos, cam_xmat, light_xpos, ...) are indexed
+ *     by worldid directly.
+ *   - matrices are row-major 3x3 (MuJoCo xmat: columns are the frame axes in world).
+ *   - every entry point returns 0 on success or a negative MJPBR_E* code; mjpbr_last_error()
+ *     returns the message of the last failure on the calling thread. Nothing allocates, frees
+ *     or synchronises; work is enqueued on `stream` (a hipStream_t, NULL = default stream).
+ */
+#ifndef MJPBR_H_
+#define MJPBR_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MJPBR_ABI_VERSION 1
+
+#define MJPBR_OK 0
+#define MJPBR_EINVAL (-1)  /* bad argument (shape, size, enum) */
+#define MJPBR_ELAUNCH (-2) /* HIP launch or runtime error */
+
+/* Rendering cameras are few; their layout is passed by value. */
+#define MJPBR_MAX_CAMS 16
+/* Texture roles (MuJoCo mjtTextureRole). */
+#define MJPBR_NTEXROLE 10
+/* Mip levels per texture / environment map at most. */
+#define MJPBR_MAX_LEVELS 16
+/* Triangles per mesh-BVH leaf (padding slots have face id -1). */
+#define MJPBR_BLAS_LEAF 4
+
+/* MuJoCo geom types (mjtGeom). */
+enum { MJPBR_GEOM_PLANE = 0, MJPBR_GEOM_HFIELD = 1, MJPBR_GEOM_SPHERE = 2, MJPBR_GEOM_CAPSULE = 3,
+       MJPBR_GEOM_ELLIPSOID = 4, MJPBR_GEOM_CYLINDER = 5, MJPBR_GEOM_BOX = 6, MJPBR_GEOM_MESH = 7,
+       MJPBR_GEOM_SDF = 8 };
+/* MuJoCo light types (mjtLightType). */
+enum { MJPBR_LIGHT_SPOT = 0, MJPBR_LIGHT_DIRECTIONAL = 1, MJPBR_LIGHT_POINT = 2, MJPBR_LIGHT_IMAGE = 3 };
+/* Texture storage formats. */
+enum { MJPBR_TEX_RGBA8 = 0, MJPBR_TEX_RGBA16F = 1 };
+
+/* Scene geometry and acceleration structures (reference: trace.py:77-113 `Geometry`). */
+typedef struct MjpbrGeometry {
+  int32_t nworld;
+  int32_t ngeom;
+  int32_t nmesh;
+  int32_t pad0;
+  const int32_t* geom_type;    /* [ngeom] */
+  const int32_t* geom_dataid;  /* [n, ngeom] mesh id of mesh geoms */
+  const int32_t* geom_matid;   /* [n, ngeom] */
+  const float* geom_size;      /* [n, ngeom, 3] */
+  const float* geom_rgba;      /* [n, ngeom, 4] */
+  const float* mat_rgba;       /* [n, nmat, 4] */
+  const float* geom_xpos;      /* [nworld, ngeom, 3] */
+  const float* geom_xmat;      /* [nworld, ngeom, 9] */
+  int32_t geom_dataid_n, geom_matid_n, geom_size_n, geom_rgba_n, mat_rgba_n, nmat;
+  /* Per-world scene BVH: an implicit perfect binary tree of 2^tlas_depth leaves (heap order,
+   * children of inner node i are 2i+1 and 2i+2). Inner node i stores the boxes of its two
+   * children: tlas_box[w][i] = {lo0.xyz, hi0.xyz, lo1.xyz, hi1.xyz}. Leaves hold one geom id
+   * (or -1). Build or refit with mjpbr_scene_bvh. */
+  int32_t tlas_depth;
+  int32_t nbvh;                /* number of geoms in the BVH (enabled geom groups) */
+  float* tlas_box;             /* [nworld, 2^depth - 1, 12] */
+  int32_t* tlas_leaf;          /* [nworld, 2^depth] */
+  const int32_t* bvh_geom;     /* [nbvh] enabled geom ids */
+  /* Mesh BVHs (shared by all worlds), same implicit layout, MJPBR_BLAS_LEAF triangles a leaf.
+   * blas_tri[slot] = {v0.xyz, face id (int bits)}, {e1 = v1 - v0}, {e2 = v2 - v0} in the mesh
+   * frame. */
+  const int32_t* blas_depth;   /* [nmesh] */
+  const int32_t* blas_node_adr;/* [nmesh] first inner node */
+  const int32_t* blas_tri_adr; /* [nmesh] first triangle slot */
+  const float* blas_box;       /* [sum inner nodes, 12] */
+  const float* blas_tri;       /* [slots, 12] */
+  const float* mesh_aabb;      /* [nmesh, 6] mesh-frame bounds {lo.xyz, hi.xyz} */
+} MjpbrGeometry;
+
+/* Materials, mesh attributes and textures (reference: material.py:224-249 `Materials`). */
+typedef struct MjpbrMaterials {
+  const float* geom_metallic;    /* [n, ngeom] */
+  const float* geom_roughness;   /* [n, ngeom] */
+  const float* geom_reflectance; /* [n, ngeom] dielectric reflectance, F0 = 0.16 r^2 */
+  const float* mat_reflectance;  /* [n, nmat] MuJoCo mirror reflectance */
+  const float* mat_emission;     /* [n, nmat] */
+  const int32_t* mat_texid;      /* [n, nmat, 10] */
+  const int32_t* mat_texuniform; /* [n, nmat] */
+  const float* mat_texrepeat;    /* [n, nmat, 2] */
+  int32_t geom_metallic_n, geom_roughness_n, geom_reflectance_n, mat_reflectance_n;
+  int32_t mat_emission_n, mat_texid_n, mat_texuniform_n, mat_texrepeat_n;
+  /* Mesh attributes, mesh frame. */
+  const int32_t* mesh_vertadr;      /* [nmesh] */
+  const int32_t* mesh_faceadr;      /* [nmesh] */
+  const int32_t* mesh_facenum;      /* [nmesh] */
+  const int32_t* mesh_normaladr;    /* [nmesh] */
+  const int32_t* mesh_texcoordadr;  /* [nmesh], -1: no texture coordinates */
+  const float* mesh_vert;           /* [nvert, 3] */
+  const float* mesh_normal;         /* [nnormal, 3] */
+  const int32_t* mesh_face;         /* [nface, 3] */
+  const int32_t* mesh_facenormal;   /* [nface, 3] */
+  const int32_t* mesh_facetexcoord; /* [nface, 3] */
+  const float* mesh_texcoord;       /* [ntexcoord, 2] */
+  /* Textures: mip chains in one byte pool. Level l of texture t starts at byte
+   * tex_level_adr[t * MJPBR_MAX_LEVELS + l]; tex_desc[t] = {width, height, nlevels, format}. */
+  int32_t ntex;
+  int32_t pad1;
+  const int64_t* tex_level_adr;
+  const int32_t* tex_desc;
+  const uint8_t* tex_data;
+} MjpbrMaterials;
+
+/* Lights (reference: lights.py:40-60 `Lights`). */
+typedef struct MjpbrLights {
+  int32_t nlight;
+  int32_t light_type_n, light_castshadow_n, light_active_n, light_attenuation_n, light_cutoff_n;
+  int32_t light_exponent_n, light_diffuse_n, light_specular_n, light_intensity_n, light_range_n;
+  int32_t light_bulbradius_n;
+  const int32_t* light_type;       /* [n, nlight] */
+  const int32_t* light_castshadow; /* [n, nlight] */
+  const int32_t* light_active;     /* [n, nlight] */
+  const float* light_attenuation;  /* [n, nlight, 3] */
+  const float* light_cutoff;       /* [n, nlight] degrees */
+  const float* light_exponent;     /* [n, nlight] */
+  const float* light_diffuse;      /* [n, nlight, 3] */
+  const float* light_specular;     /* [n, nlight, 3] */
+  const float* light_xpos;         /* [nworld, nlight, 3] */
+  const float* light_xdir;         /* [nworld, nlight, 3] */
+  const float* light_intensity;    /* [n, nlight] lux / candela */
+  const float* light_range;        /* [n, nlight] */
+  const float* light_bulbradius;   /* [n, nlight] */
+} MjpbrLights;
+
+/* Cameras (reference: kernel_inputs.py:151-169 `Cameras`). Rendering camera i (of nrender)
+ * is model camera cam_id_map[i] with resolution cam_res[i]. */
+typedef struct MjpbrCameras {
+  int32_t nrender;
+  int32_t ncam;
+  int32_t cam_fovy_n;
+  int32_t cam_intrinsic_n;
+  int32_t total_rays;               /* sum of the rendering cameras' pixels */
+  int32_t pad;
+  int32_t cam_res[MJPBR_MAX_CAMS][2];
+  int32_t cam_id_map[MJPBR_MAX_CAMS];
+  const int32_t* cam_projection;    /* [ncam] 0 perspective, 1 orthographic */
+  const float* cam_fovy;            /* [n, ncam] degrees (orthographic: extent) */
+  const float* cam_sensorsize;      /* [ncam, 2] */
+  const float* cam_intrinsic;       /* [n, ncam, 4] fx, fy, cx, cy (length units) */
+  const float* cam_xpos;            /* [nworld, ncam, 3] */
+  const float* cam_xmat;            /* [nworld, ncam, 9] */
+} MjpbrCameras;
+
+/* Environment lighting (reference: kernel_inputs.py:172-187 `Environment`). */
+typedef struct MjpbrEnvironment {
+  const float* exposure;         /* [n] */
+  const float* ambient;          /* [n, 3] constant ambient radiance */
+  int32_t exposure_n, ambient_n;
+  float background[3];           /* linear background colour (before 1 / exposure) */
+  int32_t dfg_size;
+  const float* dfg_lut;          /* [dfg_size (roughness), dfg_size (NoV), 2] */
+  int32_t nenv;
+  int32_t env_id_n, env_intensity_n, env_yaw_n;
+  /* Radiance maps: RGBA16F mip chains, level l of map e at half offset
+   * env_level_adr[e * MJPBR_MAX_LEVELS + l]; env_desc[e] = {width, height, nlevels, 0}. */
+  const int64_t* env_level_adr;
+  const int32_t* env_desc;
+  const uint16_t* env_radiance;
+  /* GGX-prefiltered stacks: [nenv, pf_levels, pf_height, pf_width, 4] f16. */
+  int32_t pf_levels, pf_height, pf_width, pad2;
+  const uint16_t* env_prefiltered;
+  const float* env_sh;           /* [nenv, 9, 3] */
+  const int32_t* env_id;         /* [n] */
+  const float* env_intensity;    /* [n] */
+  const float* env_yaw;          /* [n] radians */
+  /* Importance sampling tables (path tracer): prob [nenv, h, w], marginal [nenv, h + 1],
+   * conditional [nenv, h, w + 1]. */
+  int32_t cdf_h, cdf_w;
+  const float* env_prob;
+  const float* env_marginal;
+  const float* env_conditional;
+} MjpbrEnvironment;
+
+/* Render settings (reference: kernel_inputs.py:55-116 `Settings`). */
+typedef struct MjpbrSettings {
+  int32_t render_order;     /* 0 ROW, 1 TILED, 2 WORLD */
+  int32_t samples_per_axis; /* real-time tier N (N x N samples a pixel) */
+  float znear, zfar;
+  int32_t tone_mapper;      /* 0 PBR neutral, 1 ACES, 2 clamp */
+  int32_t render_hdr;
+  int32_t any_rgb;
+  int32_t light_model;      /* 0 physical, 1 MuJoCo */
+  int32_t has_spot;
+  int32_t use_shadows;
+  int32_t shadow_samples;
+  float shadow_light_fraction;
+  int32_t headlight_active;
+  float headlight_diffuse[3];
+  float headlight_specular[3];
+  float headlight_intensity;
+  int32_t has_env;
+  int32_t env_background;
+  float inv_texel_angle;
+  int32_t textures, base_color_map, normal_map, orm_map, emissive_map, opacity_map;
+  int32_t specular_aa;
+  int32_t vertex_normals;
+  int32_t has_reflection;
+  int32_t has_transparency;
+  int32_t max_layers;
+  int32_t denoise;
+  int32_t cull_backfaces;
+  int32_t render_guides;
+  /* path tracer */
+  int32_t path_samples, max_depth, rr_depth;
+  int32_t pad3;
+} MjpbrSettings;
+
+/* Output buffers (reference: kernel_inputs.py:190-204 `Outputs`). Per-world flat images:
+ * camera c's pixels start at *_adr[c] within a world row of *_stride elements. */
+typedef struct MjpbrOutputs {
+  int32_t render_rgb[MJPBR_MAX_CAMS];
+  int32_t render_depth[MJPBR_MAX_CAMS];
+  int32_t render_seg[MJPBR_MAX_CAMS];
+  int32_t rgb_adr[MJPBR_MAX_CAMS];
+  int32_t depth_adr[MJPBR_MAX_CAMS];
+  int32_t seg_adr[MJPBR_MAX_CAMS];
+  int32_t rgb_stride, depth_stride, seg_stride, hdr_stride, guide_stride, aov_stride;
+  uint32_t* rgb;     /* [nworld, rgb_stride] packed sRGB8 R | G<<8 | B<<16 | 255<<24 */
+  float* depth;      /* [nworld, depth_stride] */
+  int32_t* seg;      /* [nworld, seg_stride, 2] (id, object type) */
+  float* hdr;        /* [nworld, hdr_stride, 3] linear radiance (optional) */
+  float* albedo;     /* [nworld, guide_stride, 3] first-hit base colour (optional) */
+  float* normal;     /* [nworld, guide_stride, 3] first-hit world normal (optional) */
+  /* path tracer denoiser features, [nworld, aov_stride] (reference denoise.py:43-51) */
+  uint16_t* aov_albedo;   /* f16 x3 */
+  uint16_t* aov_normal;   /* f16 x3 */
+  float* aov_depth;
+  float* aov_emission;    /* x3 */
+  float* aov_variance;
+} MjpbrOutputs;
+
+/* One frame's inputs. */
+typedef struct MjpbrScene {
+  MjpbrGeometry geo;
+  MjpbrMaterials mats;
+  MjpbrLights lit;
+  MjpbrCameras cams;
+  MjpbrEnvironment env;
+} MjpbrScene;
+
+/* À-trous parameters (reference: denoise.py:204-248, types.py:303-306). */
+typedef struct MjpbrAtrousParams {
+  int32_t nworld;
+  int32_t nrender;
+  int32_t cam_res[MJPBR_MAX_CAMS][2];
+  int32_t rgb_adr[MJPBR_MAX_CAMS];
+  int32_t stride;          /* pixels a world row */
+  int32_t iterations;
+  float sigma_color, sigma_normal, sigma_depth, sigma_albedo;
+  int32_t tone_mapper;
+  int32_t render_hdr;
+  const float* exposure;
+  int32_t exposure_n;
+  int32_t pad;
+} MjpbrAtrousParams;
+
+/* ---------------------------------------------------------------------------------------- */
+/* Library */
+int mjpbr_abi_version(void);
+const char* mjpbr_last_error(void);
+/* sizeof of every ABI struct, in declaration order, so bindings can check their mirrors. */
+int mjpbr_struct_sizes(int64_t* sizes, int n);
+
+/* Per-world scene BVH over the enabled geoms: build (leaf assignment by median split + boxes)
+ * or refit (boxes only, leaf assignment kept). Replaces `mujoco_warp.refit_bvh` and the BVH of
+ * `mujoco_warp.create_render_context` (reference io.py:511-528, README.md:50-52). */
+int mjpbr_scene_bvh(const MjpbrGeometry* geo, int refit, void* stream);
+
+/* Real-time tier megakernel: one thread per (world, pixel), all samples of a pixel.
+ * Replaces `render` with tier REALTIME (reference render.py:430-460, megakernel :400-427). */
+int mjpbr_rt_render(const MjpbrScene* scene, const MjpbrSettings* s, MjpbrOutputs* out, int32_t seed,
+                    void* stream);
+
+/* Context precompute (reference ibl.py:62-104, envmap.py:171-221). */
+int mjpbr_dfg_lut(float* lut, int size, int nsample, void* stream);
+int mjpbr_env_prefilter(const uint16_t* radiance, const int64_t* level_adr, const int32_t* desc,
+                        uint16_t* stack, int levels, int height, int width, int nsample, void* stream);
+
+/* get_hdr / get_albedo / get_normal extraction of one camera into (n, h, w, 3)
+ * (reference output.py:55-67). */
+int mjpbr_extract_rgb_image(const float* src, int src_stride, int adr, int width, int height, int nout,
+                            float* dst, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MJPBR_H_ */

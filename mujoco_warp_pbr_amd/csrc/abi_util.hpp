@@ -1,0 +1,54 @@
+// abi_util.hpp — host-side helpers of the C ABI: thread-local error messages, launch checks,
+// tile maps of the render kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include "../../include/mjpbr.h"
+#include "camera.hpp"
+
+namespace mjpbr {
+
+int set_error(int code, const char* fmt, ...);
+
+inline int check_launch(const char* what) {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return set_error(MJPBR_ELAUNCH, "%s: %s", what, hipGetErrorString(e));
+  return MJPBR_OK;
+}
+
+inline size_t host_stage_bytes(const MjpbrGeometry* g) {
+  int nleaf = 1 << g->tlas_depth;
+  return (size_t)((nleaf - 1) * 12 + g->ngeom * kGeomRec + nleaf) * 4;
+}
+
+inline int make_tile_map(const MjpbrScene* sc, TileMap* tm, int* nlaunch) {
+  const MjpbrCameras& c = sc->cams;
+  if (c.nrender < 1 || c.nrender > MJPBR_MAX_CAMS)
+    return set_error(MJPBR_EINVAL, "nrender must be in [1, %d], got %d", MJPBR_MAX_CAMS, c.nrender);
+  if (sc->geo.tlas_depth < 1 || sc->geo.tlas_depth > 12)
+    return set_error(MJPBR_EINVAL, "scene BVH depth must be in [1, 12], got %d", sc->geo.tlas_depth);
+  tm->nrender = c.nrender;
+  int base = 0, rays = 0;
+  for (int i = 0; i < c.nrender; ++i) {
+    int w = c.cam_res[i][0], h = c.cam_res[i][1];
+    if (w < 1 || h < 1) return set_error(MJPBR_EINVAL, "camera %d has resolution %dx%d", i, w, h);
+    tm->tile_base[i] = base;
+    tm->tiles_x[i] = (w + kTile - 1) / kTile;
+    tm->ray_offset[i] = rays;
+    base += tm->tiles_x[i] * ((h + kTile - 1) / kTile);
+    rays += w * h;
+  }
+  tm->tile_base[c.nrender] = base;
+  if (rays != c.total_rays) return set_error(MJPBR_EINVAL, "total_rays %d != sum of camera pixels %d", c.total_rays, rays);
+  tm->tiles_per_world = base;
+  int64_t nb = (int64_t)base * sc->geo.nworld;
+  if (nb > (1ll << 30)) return set_error(MJPBR_EINVAL, "too many tiles (%lld)", (long long)nb);
+  tm->nblocks = (int)nb;
+  int chunk = (tm->nblocks + 7) / 8;
+  tm->xcd_chunk = chunk;
+  *nlaunch = chunk * 8;
+  return MJPBR_OK;
+}
+
+}  // namespace mjpbr

@@ -1,0 +1,261 @@
+// rt.hip — real-time tier megakernel (reference render.py:67-460).
+//
+// One workgroup = a 16 x 16 pixel tile of one world (a wave = an 8 x 8 tile); the world's scene
+// BVH and geom frames are staged in LDS first. Each thread renders all N x N sub-pixel samples
+// of its pixel: primary ray, split-sum IBL + direct lights with ray-traced (soft) shadows +
+// emission, one mirror bounce, translucent layers, tone mapping and sRGB encoding.
+#pragma clang fp contract(off)
+#include "abi_util.hpp"
+#include "stage.hpp"
+
+namespace mjpbr {
+
+constexpr float kMinThroughput = 1.0f / 255.0f;  // render.py:62
+
+struct RtCtx {
+  const MjpbrScene& sc;
+  const MjpbrSettings& s;
+  const SceneView& sv;
+  int w;
+  int seed;
+  float exposure;
+};
+
+// render.py:67-75
+MJ_DEV V2 shadow_rand2(int seed, int stream, int nlight, int li, int nsample, int k) {
+  int counter = ((stream * nlight + li) * nsample + k) * 2;
+  return V2{rand_at(seed, counter), rand_at(seed, counter + 1)};
+}
+
+// render.py:95-127
+MJ_DEV float light_visibility(const RtCtx& c, const Light& L, V3 point, V3 normal, V3 l, float dist, int stream,
+                              int nlight, int li) {
+  const MjpbrSettings& s = c.s;
+  V3 origin = point + normal * kRayOffset;
+  bool cull = s.cull_backfaces != 0;
+  if (L.type == MJPBR_LIGHT_DIRECTIONAL)
+    return trace_any(c.sv, origin, l, kUnboundedShadow, cull) ? s.shadow_light_fraction : 1.0f;
+  if (s.shadow_samples <= 1)
+    return trace_any(c.sv, origin, l, dist - kShadowGap, cull) ? s.shadow_light_fraction : 1.0f;
+  V3 t1, t2;
+  orthonormal_basis(l, &t1, &t2);
+  float visible = 0.0f;
+  for (int k = 0; k < s.shadow_samples; ++k) {
+    V2 u = shadow_rand2(c.seed, stream, nlight, li, s.shadow_samples, k);
+    float r = L.bulbradius * sqrtf(u.x);
+    float sp, cp;
+    sincos_turns(u.y, &sp, &cp);
+    V3 to_target = L.pos + (t1 * cp + t2 * sp) * r - origin;
+    float d = len(to_target);
+    if (!trace_any(c.sv, origin, to_target / d, d - kShadowGap, cull)) visible += 1.0f;
+  }
+  float v = visible / (float)s.shadow_samples;
+  return v + (1.0f - v) * s.shadow_light_fraction;
+}
+
+// render.py:129-161
+MJ_DEV V3 shade_light(const RtCtx& c, const Light& L, const Lobes& lb, V3 point, V3 normal, V3 view, float nov,
+                      int stream, int nlight, int li) {
+  V3 l, ds, ss;
+  float dist;
+  eval_light(L, c.s.light_model, c.s.has_spot != 0, point, &l, &dist, &ds, &ss);
+  float nol = dot(normal, l);
+  if (nol <= 0.0f || (len_sq(ds) == 0.0f && len_sq(ss) == 0.0f)) return v3s(0.0f);
+  float vis = 1.0f;
+  if (c.s.use_shadows && L.castshadow) vis = light_visibility(c, L, point, normal, l, dist, stream, nlight, li);
+  V3 h = normalize(l + view);
+  float noh = fmaxf(dot(normal, h), 0.0f);
+  float loh = fmaxf(dot(l, h), 0.0f);
+  V3 fr = cw(specular_lobe(lb.f0, lb.alpha, noh, nov, nol, loh), lb.energy_comp);
+  V3 fd = lb.diffuse_color * kInvPi;
+  return (cw(fd, ds) + cw(fr, ss)) * (nol * vis);
+}
+
+// render.py:184-239
+MJ_DEV V3 shade_surface(const RtCtx& c, const Surface& sf, V3 point, V3 normal, V3 view, V3 head_dir, int stream) {
+  const MjpbrScene& sc = c.sc;
+  const MjpbrSettings& s = c.s;
+  const MjpbrEnvironment& env = sc.env;
+  float nov = fmaxf(dot(normal, view), 1.0e-4f);
+  Lobes lb = shading_lobes(env, sf, nov);
+  float alpha = lb.alpha;
+  V3 ambient = ld3(env.ambient + 3 * (c.w % env.ambient_n));
+  V3 irradiance = ambient, prefiltered = ambient;
+  if (s.has_env) {
+    int e;
+    float yaw, scale;
+    env_lookup(env, c.w, &e, &yaw, &scale);
+    irradiance = ambient + eval_sh9(env.env_sh + e * 27, rotate_yaw(normal, -yaw)) * scale;
+    V3 r = view - normal * (2.0f * dot(view, normal));
+    V3 dominant = normalize(-r * (1.0f - alpha * alpha) + normal * (alpha * alpha));
+    prefiltered = ambient + sample_specular(env, e, dominant, yaw, lb.perceptual_roughness) * scale;
+  }
+  float ao = sf.occlusion;
+  float spec_ao = clampf(powf(nov + ao, exp2f(-16.0f * alpha - 1.0f)) - 1.0f + ao, 0.0f, 1.0f);
+  V3 radiance = cw(irradiance, lb.diffuse_color) * ao + cw(prefiltered, cw(lb.spec_albedo, lb.energy_comp)) * spec_ao;
+  const MjpbrLights& lt = sc.lit;
+  for (int li = 0; li < lt.nlight; ++li) {
+    if (!bat(lt.light_active, lt.light_active_n, c.w, lt.nlight)[li]) continue;
+    Light L = fetch_light(lt, c.w, li);
+    radiance += shade_light(c, L, lb, point, normal, view, nov, stream, lt.nlight, li);
+  }
+  if (s.headlight_active) {
+    Light H = headlight(s, s.headlight_intensity / c.exposure, head_dir);
+    radiance += shade_light(c, H, lb, point, normal, view, nov, stream, 1, 0);
+  }
+  return radiance + sf.emissive / c.exposure;
+}
+
+// render.py:241-247
+MJ_DEV V3 background(const RtCtx& c, V3 dir, float spread) {
+  const MjpbrEnvironment& env = c.sc.env;
+  if (c.s.env_background) {
+    int e;
+    float yaw, scale;
+    env_lookup(env, c.w, &e, &yaw, &scale);
+    float lod = log2f(fmaxf(spread * c.s.inv_texel_angle, kMinTexels));
+    return sample_radiance(env, e, dir, yaw, lod) * scale;
+  }
+  return ld3(env.background) / c.exposure;
+}
+
+// render.py:249-274
+MJ_DEV V3 shade_reflection(const RtCtx& c, V3 point, V3 normal, V3 dir, float footprint, float spread, V3 head_dir,
+                           int stream) {
+  V3 r = dir - normal * (2.0f * dot(dir, normal));
+  V3 origin = point + normal * kRayOffset;
+  Hit hit = trace_closest(c.sv, origin, r, kMaxVal, c.s.cull_backfaces != 0);
+  if (hit.geom_id == -1) return background(c, r, spread);
+  V3 p = origin + r * hit.dist;
+  V3 n;
+  Surface sf = eval_surface(c.sc, c.s, c.w, hit, p, r, footprint + spread * hit.dist, &n);
+  return shade_surface(c, sf, p, n, -r, head_dir, stream);
+}
+
+// render.py:276-341
+MJ_DEV V3 trace_pixel(const RtCtx& c, V3 origin, V3 dir, float max_dist, float footprint, float spread, V3 head_dir,
+                      int stream, Hit* first, V3* first_albedo, V3* first_normal) {
+  const MjpbrSettings& s = c.s;
+  V3 radiance = v3s(0.0f);
+  float throughput = 1.0f, travelled = 0.0f;
+  first->geom_id = -1;
+  *first_albedo = v3s(0.0f);
+  *first_normal = v3s(0.0f);
+  int nlayer = s.has_transparency ? s.max_layers : 1;
+  for (int layer = 0; layer < nlayer; ++layer) {
+    Hit hit = trace_closest(c.sv, origin, dir, max_dist - travelled, s.cull_backfaces != 0);
+    if (layer == 0) *first = hit;
+    if (hit.geom_id == -1) {
+      radiance += background(c, dir, spread) * throughput;
+      break;
+    }
+    V3 point = origin + dir * hit.dist;
+    float cone = footprint + spread * (travelled + hit.dist);
+    V3 normal;
+    Surface sf = eval_surface(c.sc, s, c.w, hit, point, dir, cone, &normal);
+    if (layer == 0 && s.render_guides) {
+      *first_albedo = sf.base_color;
+      *first_normal = normal;
+    }
+    V3 shaded = shade_surface(c, sf, point, normal, -dir, head_dir, stream);
+    if (s.has_reflection) {
+      float mirror = mirror_reflectance(c.sc, c.w, hit.geom_id);
+      if (mirror > 0.0f) {
+        V3 refl = shade_reflection(c, point, normal, dir, cone, spread, head_dir, stream + 1);
+        shaded = shaded * (1.0f - mirror) + refl * mirror;
+      }
+    }
+    if (s.has_transparency) {
+      float opacity = clampf(sf.alpha, 0.0f, 1.0f);
+      radiance += shaded * (throughput * opacity);
+      throughput *= 1.0f - opacity;
+      if (throughput < kMinThroughput) break;
+      travelled += hit.dist;
+      origin = point + dir * kRayOffset;
+    } else {
+      radiance += shaded;
+      break;
+    }
+  }
+  return radiance;
+}
+
+__global__ void __launch_bounds__(256) rt_kernel(MjpbrScene sc, MjpbrSettings s, MjpbrOutputs out, int seed,
+                                                 TileMap tm) {
+  extern __shared__ float lds[];
+  int b = logical_block(tm);
+  if (b >= tm.nblocks) return;
+  int w = b / tm.tiles_per_world;
+  int t = b - w * tm.tiles_per_world;
+  int cidx = 0;
+  while (cidx + 1 < tm.nrender && t >= tm.tile_base[cidx + 1]) ++cidx;
+  int tl = t - tm.tile_base[cidx];
+  int tx = tl % tm.tiles_x[cidx], ty = tl / tm.tiles_x[cidx];
+  int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  int px = tx * kTile + (wave & 1) * 8 + (lane & 7);
+  int py = ty * kTile + (wave >> 1) * 8 + (lane >> 3);
+
+  SceneView sv = stage_world(sc, w, lds);
+
+  Pixel pix;
+  if (!setup_pixel(sc.cams, out, s, w, cidx, px, py, tm.ray_offset[cidx], &pix)) return;
+  float exposure = sc.env.exposure[w % sc.env.exposure_n];
+  RtCtx c{sc, s, sv, w, 0, exposure};
+  bool cull = s.cull_backfaces != 0;
+  if (!pix.want_rgb) {
+    V3 o, d, dc;
+    float md;
+    primary_ray(pix, s, 1, 0, 0, &o, &d, &md, &dc);
+    Hit hit = trace_closest(sv, o, d, md, cull);
+    write_geometry(out, w, pix, hit, dc);
+    return;
+  }
+  int n_aa = s.samples_per_axis;
+  float inv_aa = 1.0f / (float)n_aa;
+  float pspread, pwidth;
+  pixel_cone(pix, &pspread, &pwidth);
+  float spread = pspread * inv_aa, width = pwidth * inv_aa;
+  V3 head_dir = -V3{pix.xmat[2], pix.xmat[5], pix.xmat[8]};
+  c.seed = seed + w * sc.cams.total_rays + pix.rayid;
+  V3 display = v3s(0.0f), hdr = v3s(0.0f);
+  for (int smp = 0; smp < n_aa * n_aa; ++smp) {
+    V3 o, d, dc;
+    float md;
+    primary_ray(pix, s, n_aa, smp % n_aa, smp / n_aa, &o, &d, &md, &dc);
+    Hit first;
+    V3 alb, nrm;
+    V3 radiance = trace_pixel(c, o, d, md, width, spread, head_dir, 2 * smp, &first, &alb, &nrm);
+    if (smp == 0) {
+      write_geometry(out, w, pix, first, dc);
+      if (s.render_guides) {
+        int64_t a = (int64_t)w * out.guide_stride + out.rgb_adr[pix.camid] + pix.pixelid;
+        st3(out.albedo + 3 * a, alb);
+        st3(out.normal + 3 * a, nrm);
+      }
+    }
+    display += display_transform(radiance * exposure, s.tone_mapper);
+    hdr += radiance;
+  }
+  float inv2 = inv_aa * inv_aa;
+  int64_t adr = out.rgb_adr[pix.camid] + pix.pixelid;
+  if (s.render_hdr) st3(out.hdr + 3 * ((int64_t)w * out.hdr_stride + adr), hdr * inv2);
+  out.rgb[(int64_t)w * out.rgb_stride + adr] = encode_display(display * inv2);
+}
+
+}  // namespace mjpbr
+
+using namespace mjpbr;
+
+extern "C" int mjpbr_rt_render(const MjpbrScene* scene, const MjpbrSettings* s, MjpbrOutputs* out, int32_t seed,
+                               void* stream) {
+  TileMap tm;
+  int nlaunch;
+  int rc = make_tile_map(scene, &tm, &nlaunch);
+  if (rc) return rc;
+  if (s->samples_per_axis < 1) return set_error(MJPBR_EINVAL, "samples_per_axis must be at least 1");
+  size_t lds = host_stage_bytes(&scene->geo);
+  if (lds > 64 * 1024) return set_error(MJPBR_EINVAL, "scene too large for LDS staging (%d geoms)", scene->geo.ngeom);
+  hipLaunchKernelGGL(rt_kernel, dim3(nlaunch), dim3(256), lds, (hipStream_t)stream, *scene, *s, *out, seed,
+                     tm);
+  return check_launch("rt_kernel");
+}

@@ -1,0 +1,266 @@
+// trace.hpp — closest-hit and any-hit ray queries over the per-world scene BVH (staged in LDS)
+// and the shared mesh BVHs (HBM). Semantics follow the reference's ordered traversal,
+// traverse.py:194-323: pruning cut dist*(1+tol)+tol, geoms whose rgba alpha is 0 skipped,
+// back-face hits culled, ties within tol go to the higher geom id; any-hit returns at the first
+// hit closer than max_dist.
+//
+// Both BVHs are implicit perfect binary trees (heap order) whose inner nodes hold their two
+// children's boxes. Traversal is nearest-child-first with a one-register "trail": bit k set
+// means the far child at level k is still pending; popping recomputes the pending node from
+// the heap index and re-tests its box against the current cut, so no stack memory is used.
+#pragma once
+#include "../../include/mjpbr.h"
+#include "geom.hpp"
+
+namespace mjpbr {
+
+// Staged geom record: 16 floats.
+//   [0..2] xpos  [3] size.x  [4..12] xmat  [13] size.y  [14] size.z  [15] bits
+// bits = type | alpha_zero << 8 | (dataid + 1) << 9
+constexpr int kGeomRec = 16;
+
+struct Hit {
+  int geom_id;  // -1: miss
+  float dist;
+  V3 normal;
+  float bary_u, bary_v;  // weights of the triangle's first and second vertex (meshes)
+  int face_id;
+  int mesh_id;
+};
+
+struct SceneView {
+  const float* tlas_box;  // [ninner][12] (LDS)
+  const int* tlas_leaf;   // [nleaf] (LDS)
+  const float* geoms;     // [ngeom][16] (LDS)
+  int depth;
+  // mesh BVHs (global)
+  const float* blas_box;
+  const float* blas_tri;
+  const int* blas_depth;
+  const int* blas_node_adr;
+  const int* blas_tri_adr;
+};
+
+MJ_DEV float cut_of(float dist) { return dist * (1.0f + kRayTolRel) + kRayTolAbs; }
+
+// Visits the leaves of an implicit BVH nearest-first. leaf(slot) returns true to stop.
+template <class CutFn, class LeafFn>
+MJ_DEV void traverse(const float* boxes, int depth, V3 o, V3 rcp, CutFn cut, LeafFn leaf) {
+  const int ninner = (1 << depth) - 1;
+  int node = 0, level = 0;
+  uint32_t trail = 0;
+  for (;;) {
+    if (level < depth) {
+      const float* b = boxes + 12 * node;
+      float c = cut();
+      float ta = ray_aabb(o, rcp, b, b + 3, c);
+      float tb = ray_aabb(o, rcp, b + 6, b + 9, c);
+      if (ta >= 0.0f || tb >= 0.0f) {
+        int nearside = (tb >= 0.0f && (ta < 0.0f || tb < ta)) ? 1 : 0;
+        if (ta >= 0.0f && tb >= 0.0f) trail |= 1u << level;
+        node = 2 * node + 1 + nearside;
+        ++level;
+        continue;
+      }
+    } else {
+      if (leaf(node - ninner)) return;
+    }
+    for (;;) {
+      if (trail == 0u) return;
+      int L = 31 - __builtin_clz(trail);
+      trail &= ~(1u << L);
+      int anc1 = (node + 1) >> (level - L - 1);
+      node = (anc1 ^ 1) - 1;
+      level = L + 1;
+      const float* pb = boxes + 12 * ((node - 1) >> 1) + 6 * ((node - 1) & 1);
+      if (ray_aabb(o, rcp, pb, pb + 3, cut()) >= 0.0f) break;
+    }
+  }
+}
+
+MJ_DEV V3 safe_rcp(V3 d) { return V3{1.0f / d.x, 1.0f / d.y, 1.0f / d.z}; }
+
+// Closest triangle of mesh `mesh` (mesh-frame ray) with t < max_t; ties -> lower face id.
+MJ_DEV bool mesh_closest(const SceneView& sv, int mesh, V3 lo, V3 ld, float max_t, float* t_out,
+                         int* face_out, float* b1_out, float* b2_out, V3* e1_out, V3* e2_out) {
+  const float* boxes = sv.blas_box + 12 * (int64_t)sv.blas_node_adr[mesh];
+  const float* tris = sv.blas_tri + 12 * (int64_t)sv.blas_tri_adr[mesh];
+  int depth = sv.blas_depth[mesh];
+  float best = INFINITY;
+  int best_face = -1;
+  float bb1 = 0.0f, bb2 = 0.0f;
+  V3 be1 = v3s(0.0f), be2 = v3s(0.0f);
+  V3 rcp = safe_rcp(ld);
+  traverse(
+      boxes, depth, lo, rcp, [&]() { return best == INFINITY ? max_t : fminf(max_t, cut_of(best)); },
+      [&](int slot) {
+        const float4* tr = reinterpret_cast<const float4*>(tris) + 3 * slot * MJPBR_BLAS_LEAF;
+#pragma unroll
+        for (int k = 0; k < MJPBR_BLAS_LEAF; ++k) {
+          float4 a = tr[3 * k], b = tr[3 * k + 1], c = tr[3 * k + 2];
+          int face = __float_as_int(a.w);
+          if (face < 0) continue;
+          V3 v0 = V3{a.x, a.y, a.z}, e1 = V3{b.x, b.y, b.z}, e2 = V3{c.x, c.y, c.z};
+          float t, b1, b2;
+          if (ray_triangle(lo, ld, v0, e1, e2, max_t, &t, &b1, &b2)) {
+            if (t < best || (t == best && face < best_face)) {
+              best = t;
+              best_face = face;
+              bb1 = b1;
+              bb2 = b2;
+              be1 = e1;
+              be2 = e2;
+            }
+          }
+        }
+        return false;
+      });
+  if (best_face < 0) return false;
+  *t_out = best;
+  *face_out = best_face;
+  *b1_out = bb1;
+  *b2_out = bb2;
+  *e1_out = be1;
+  *e2_out = be2;
+  return true;
+}
+
+MJ_DEV bool mesh_any(const SceneView& sv, int mesh, V3 lo, V3 ld, float max_t) {
+  const float* boxes = sv.blas_box + 12 * (int64_t)sv.blas_node_adr[mesh];
+  const float* tris = sv.blas_tri + 12 * (int64_t)sv.blas_tri_adr[mesh];
+  int depth = sv.blas_depth[mesh];
+  bool hit = false;
+  V3 rcp = safe_rcp(ld);
+  traverse(
+      boxes, depth, lo, rcp, [&]() { return max_t; },
+      [&](int slot) {
+        const float4* tr = reinterpret_cast<const float4*>(tris) + 3 * slot * MJPBR_BLAS_LEAF;
+#pragma unroll
+        for (int k = 0; k < MJPBR_BLAS_LEAF; ++k) {
+          float4 a = tr[3 * k], b = tr[3 * k + 1], c = tr[3 * k + 2];
+          if (__float_as_int(a.w) < 0) continue;
+          float t, b1, b2;
+          if (ray_triangle(lo, ld, V3{a.x, a.y, a.z}, V3{b.x, b.y, b.z}, V3{c.x, c.y, c.z}, max_t, &t, &b1,
+                           &b2)) {
+            hit = true;
+            return true;
+          }
+        }
+        return false;
+      });
+  return hit;
+}
+
+// One geom against the ray (traverse.py:131-192). Returns the distance or -1.
+template <bool kAnyHit>
+MJ_DEV float intersect_geom(const SceneView& sv, const float* g, V3 o, V3 d, float max_t, V3* n, float* bu,
+                            float* bv, int* face, int* mesh) {
+  int bits = __float_as_int(g[15]);
+  int type = bits & 0xff;
+  V3 pos = V3{g[0], g[1], g[2]};
+  const float* mat = g + 4;
+  V3 size = V3{g[3], g[13], g[14]};
+  switch (type) {
+    case MJPBR_GEOM_PLANE:
+      return ray_plane(pos, mat, size, o, d, n);
+    case MJPBR_GEOM_SPHERE:
+      return ray_sphere(pos, size.x * size.x, o, d, n);
+    case MJPBR_GEOM_ELLIPSOID:
+      return ray_ellipsoid(pos, mat, size, o, d, n);
+    case MJPBR_GEOM_CAPSULE:
+      return ray_capsule(pos, mat, size, o, d, n);
+    case MJPBR_GEOM_CYLINDER:
+      return ray_cylinder(pos, mat, size, o, d, n);
+    case MJPBR_GEOM_BOX:
+      return ray_box(pos, mat, size, o, d, n);
+    case MJPBR_GEOM_MESH: {
+      int mid = (bits >> 9) - 1;
+      if (mid < 0) return -1.0f;
+      V3 lo = mat_tmul(mat, o - pos);
+      V3 ld = mat_tmul(mat, d);
+      if (kAnyHit) {
+        *n = v3s(0.0f);
+        return mesh_any(sv, mid, lo, ld, max_t) ? 0.0f : -1.0f;
+      }
+      float t, b1, b2;
+      int f;
+      V3 e1, e2;
+      if (!mesh_closest(sv, mid, lo, ld, max_t, &t, &f, &b1, &b2, &e1, &e2)) return -1.0f;
+      *n = normalize(mat_mul(mat, normalize(cross(e1, e2))));
+      *bu = 1.0f - b1 - b2;
+      *bv = b1;
+      *face = f;
+      *mesh = mid;
+      return t;
+    }
+    default:
+      return -1.0f;
+  }
+}
+
+MJ_DEV Hit trace_closest(const SceneView& sv, V3 o, V3 d, float max_dist, bool cull) {
+  Hit h;
+  h.geom_id = -1;
+  h.dist = max_dist;
+  h.normal = v3s(0.0f);
+  h.bary_u = 0.0f;
+  h.bary_v = 0.0f;
+  h.face_id = -1;
+  h.mesh_id = -1;
+  V3 rcp = safe_rcp(d);
+  traverse(
+      sv.tlas_box, sv.depth, o, rcp, [&]() { return cut_of(h.dist); },
+      [&](int slot) {
+        int gi = sv.tlas_leaf[slot];
+        if (gi < 0) return false;
+        const float* g = sv.geoms + kGeomRec * gi;
+        int bits = __float_as_int(g[15]);
+        if (bits & 0x100) return false;  // alpha 0
+        V3 n = v3s(0.0f);
+        float bu = 0.0f, bv = 0.0f;
+        int face = -1, mesh = -1;
+        float dd = intersect_geom<false>(sv, g, o, d, cut_of(h.dist), &n, &bu, &bv, &face, &mesh);
+        if (cull && dd >= 0.0f && dot(d, n) > 0.0f) dd = -1.0f;
+        float tol = kRayTolRel * fmaxf(h.dist, 1.0f);
+        bool closer = (dd < h.dist - tol) || (fabsf(dd - h.dist) <= tol && (h.geom_id < 0 || gi > h.geom_id));
+        if (dd >= 0.0f && closer) {
+          h.dist = dd;
+          h.normal = n;
+          h.geom_id = gi;
+          h.bary_u = bu;
+          h.bary_v = bv;
+          h.face_id = face;
+          h.mesh_id = mesh;
+        }
+        return false;
+      });
+  return h;
+}
+
+MJ_DEV bool trace_any(const SceneView& sv, V3 o, V3 d, float max_dist, bool cull) {
+  V3 rcp = safe_rcp(d);
+  bool hit = false;
+  float cut = cut_of(max_dist);
+  traverse(
+      sv.tlas_box, sv.depth, o, rcp, [&]() { return cut; },
+      [&](int slot) {
+        int gi = sv.tlas_leaf[slot];
+        if (gi < 0) return false;
+        const float* g = sv.geoms + kGeomRec * gi;
+        int bits = __float_as_int(g[15]);
+        if (bits & 0x100) return false;
+        V3 n = v3s(0.0f);
+        float bu, bv;
+        int face, mesh;
+        float dd = intersect_geom<true>(sv, g, o, d, max_dist, &n, &bu, &bv, &face, &mesh);
+        if (cull && dd >= 0.0f && dot(d, n) > 0.0f) dd = -1.0f;
+        if (dd >= 0.0f && dd < max_dist) {
+          hit = true;
+          return true;
+        }
+        return false;
+      });
+  return hit;
+}
+
+}  // namespace mjpbr

@@ -1,0 +1,389 @@
+"""create_render_context (reference io.py:331-634)."""
+
+from __future__ import annotations
+
+import math
+import os
+import warnings
+from collections.abc import Sequence
+
+import numpy as np
+import torch
+
+from mujoco_warp_pbr_amd import _abi
+from mujoco_warp_pbr_amd import bvh as bvh_lib
+from mujoco_warp_pbr_amd import envmap as envmap_lib
+from mujoco_warp_pbr_amd import material
+from mujoco_warp_pbr_amd import model as mdl
+from mujoco_warp_pbr_amd import texture
+from mujoco_warp_pbr_amd.types import LightModel
+from mujoco_warp_pbr_amd.types import PbrRenderContext
+from mujoco_warp_pbr_amd.types import RenderBase
+from mujoco_warp_pbr_amd.types import RenderOrder
+from mujoco_warp_pbr_amd.types import Tier
+from mujoco_warp_pbr_amd.types import ToneMapper
+from mujoco_warp_pbr_amd.types import UnsetMaterial
+
+DEFAULT_EV100 = 14.96578428466209
+DFG_LUT_SIZE = 128
+DFG_LUT_SAMPLES = 1024
+MAX_PATH_VERTICES = 63
+_MATERIAL_KEYS = ("metallic", "roughness")
+_TRAVERSALS = ("software", "optix", "auto")
+_MESH_BVH_FRAMES = ("auto", "mujoco")
+_MESH_BVH_CONSTRUCTORS = ("cubql", "sah", "lbvh")
+
+
+def _enum(cls, value, name):
+  try:
+    return cls(value)
+  except ValueError:
+    options = ", ".join(f"{cls.__name__}.{e.name}" for e in cls)
+    raise ValueError(f"{name} must be one of {options}, got {value!r}") from None
+
+
+def _check_at_least(name: str, value: int, low: int):
+  if value < low:
+    raise ValueError(f"{name} must be at least {low}, got {value}")
+
+
+def _check_default_material(default_material):
+  for key, value in (default_material or {}).items():
+    if key not in _MATERIAL_KEYS:
+      raise ValueError(f"default_material keys must be among {_MATERIAL_KEYS}, got {key!r}")
+    if not 0.0 <= value <= 1.0:
+      raise ValueError(f"default_material[{key!r}] must be in [0, 1], got {value}")
+
+
+def _per_camera(value, n: int, name: str, default: bool) -> np.ndarray:
+  if value is None:
+    return np.full(n, default, bool)
+  if isinstance(value, (bool, np.bool_)):
+    return np.full(n, bool(value), bool)
+  value = list(value)
+  if len(value) != n:
+    raise ValueError(f"{name} must have one entry per rendered camera ({n}), got {len(value)}")
+  return np.array(value, bool)
+
+
+def _envmap_images(envmap) -> list[np.ndarray]:
+  if envmap is None:
+    return []
+  items = envmap if isinstance(envmap, (list, tuple)) else [envmap]
+  images = []
+  for i, item in enumerate(items):
+    if isinstance(item, (str, bytes, os.PathLike)):
+      images.append(envmap_lib.load_hdr(os.fsdecode(item)))
+      continue
+    image = np.asarray(item)
+    if image.ndim != 3 or image.shape[2] not in (3, 4) or image.shape[0] < 1 or image.shape[1] < 1:
+      raise ValueError(f"envmap[{i}] must be a path or a (height, width, 3) radiance array, got shape {image.shape}")
+    images.append(image)
+  return images
+
+
+def create_render_context(
+  mjm: mdl.HostModel,
+  nworld: int = 1,
+  cam_res=None,
+  render_rgb=None,
+  render_depth=None,
+  render_seg=None,
+  enabled_geom_groups: Sequence[int] = (0, 1, 2),
+  cam_active=None,
+  background_color=(0.0, 0.0, 0.0, 1.0),
+  use_fast_math: bool = True,
+  use_shadows: bool = True,
+  shadow_light_fraction: float = 0.0,
+  enable_backface_culling: bool = True,
+  enable_vertex_normals: bool = True,
+  light_model=LightModel.PHYSICAL,
+  tone_mapper=ToneMapper.PBR_NEUTRAL,
+  ev100: float = DEFAULT_EV100,
+  unset_material=UnsetMaterial.GLTF,
+  default_material: dict | None = None,
+  use_textures: bool = True,
+  samples_per_pixel: int = 1,
+  specular_aa: bool = True,
+  shadow_samples: int = 1,
+  max_layers: int = 4,
+  enable_reflections: bool = True,
+  envmap=None,
+  envmap_intensity: float | None = None,
+  envmap_rotation: float = 0.0,
+  envmap_background: bool = True,
+  use_skybox: bool = True,
+  render_order=RenderOrder.TILED,
+  block_dim: int | None = None,
+  max_registers: int | None = None,
+  mesh_bvh_constructor: str = "cubql",
+  mesh_bvh_frame: str = "auto",
+  ordered_traversal: bool | None = None,
+  tier=Tier.REALTIME,
+  path_samples: int = 64,
+  max_depth: int = 4,
+  rr_depth: int = 3,
+  denoise: bool = False,
+  denoise_iterations: int = 3,
+  render_hdr: bool = False,
+  traversal: str = "software",
+  render_guides: bool = False,
+  camera_rays=None,
+  device="cuda",
+  precompute: bool = True,
+) -> PbrRenderContext:
+  """Creates a PBR render context on `device` (reference io.py:331-634; same arguments).
+
+  Differences from the reference: the model is a `HostModel`; `render_rgb/depth/seg=None`
+  render all three outputs; BVH construction options (`mesh_bvh_constructor`,
+  `mesh_bvh_frame`, `ordered_traversal`, `block_dim`, `max_registers`, `use_fast_math`) are
+  accepted for compatibility: this renderer always traverses nearest-first over its own BVHs.
+  `traversal="optix"` is not available (no OptiX on this hardware) and falls back to
+  "software" with a warning, as the reference does without OptiX.
+
+  `precompute=False` leaves the GPU-built tables (DFG LUT, prefiltered environment stacks)
+  zero; it exists so host-side test harnesses can create contexts on device="cpu" and fill the
+  tables themselves.
+
+  Raises:
+    ValueError: for invalid arguments, or a model with nothing to render.
+  """
+  if precompute and torch.device(device).type != "cuda":
+    raise ValueError("create_render_context builds its tables with HIP kernels: use a GPU device (or precompute=False)")
+  light_model = _enum(LightModel, light_model, "light_model")
+  tone_mapper = _enum(ToneMapper, tone_mapper, "tone_mapper")
+  unset_material = _enum(UnsetMaterial, unset_material, "unset_material")
+  render_order = _enum(RenderOrder, render_order, "render_order")
+  tier = _enum(Tier, tier, "tier")
+  for name, value in (
+    ("samples_per_pixel", samples_per_pixel),
+    ("shadow_samples", shadow_samples),
+    ("max_layers", max_layers),
+    ("path_samples", path_samples),
+  ):
+    _check_at_least(name, value, 1)
+  for name, value in (("max_depth", max_depth), ("rr_depth", rr_depth), ("denoise_iterations", denoise_iterations)):
+    _check_at_least(name, value, 0)
+  if tier == Tier.PATH and max_depth + 4 * max_layers >= MAX_PATH_VERTICES:
+    raise ValueError(
+      f"max_depth + 4 * max_layers must be below {MAX_PATH_VERTICES} for the path tracer's random "
+      f"number layout, got {max_depth} + 4 * {max_layers}"
+    )
+  if not 0.0 <= shadow_light_fraction <= 1.0:
+    raise ValueError(f"shadow_light_fraction must be in [0, 1], got {shadow_light_fraction}")
+  _check_default_material(default_material)
+  if denoise and tier != Tier.PATH:
+    raise ValueError("denoise applies to the path-traced tier (Tier.PATH)")
+  if mesh_bvh_frame not in _MESH_BVH_FRAMES:
+    raise ValueError(f'mesh_bvh_frame must be "auto" or "mujoco", got {mesh_bvh_frame!r}')
+  if mesh_bvh_constructor not in _MESH_BVH_CONSTRUCTORS:
+    raise ValueError(f"mesh_bvh_constructor must be one of {_MESH_BVH_CONSTRUCTORS}, got {mesh_bvh_constructor!r}")
+  if traversal not in _TRAVERSALS:
+    raise ValueError(f'traversal must be "software", "optix" or "auto", got {traversal!r}')
+  if block_dim is not None:
+    _check_at_least("block_dim", block_dim, 1)
+  if max_registers is not None:
+    _check_at_least("max_registers", max_registers, 0)
+  if nworld < 1:
+    raise ValueError(f"nworld must be at least 1, got {nworld}")
+  if camera_rays is not None:
+    raise ValueError("camera_rays (ray bundles) are not supported by this renderer yet")
+  if traversal == "optix":
+    warnings.warn('traversal="optix" falls back to "software": no OptiX on this device', stacklevel=2)
+  traversal = "software"
+  mjm.validate()
+
+  image_lights = np.nonzero(mjm.light_type == mdl.LIGHT_IMAGE)[0] if mjm.nlight else []
+  if len(image_lights):
+    warnings.warn(f"image lights are not supported and are ignored: {list(image_lights)}", stacklevel=2)
+  rendered = np.isin(mjm.geom_group, list(enabled_geom_groups))
+  if not np.any(rendered):
+    raise ValueError("the model has no geoms in the enabled geom groups to render")
+
+  # cameras
+  if cam_active is None:
+    cams = list(range(mjm.ncam))
+  else:
+    if len(cam_active) and isinstance(cam_active[0], (bool, np.bool_)):
+      cams = [i for i, a in enumerate(cam_active) if a]
+    else:
+      cams = [int(c) for c in cam_active]
+  if not cams:
+    raise ValueError("the model has no active camera to render")
+  if len(cams) > render_struct_max_cams():
+    raise ValueError(f"at most {render_struct_max_cams()} rendering cameras are supported, got {len(cams)}")
+  nrender = len(cams)
+  if cam_res is None:
+    res = [tuple(int(x) for x in mjm.cam_resolution[c]) for c in cams]
+  elif isinstance(cam_res, tuple) and len(cam_res) == 2 and all(isinstance(x, (int, np.integer)) for x in cam_res):
+    res = [tuple(cam_res)] * nrender
+  else:
+    res = [tuple(r) for r in cam_res]
+  if len(res) != nrender:
+    raise ValueError(f"cam_res must give one resolution per rendered camera ({nrender}), got {len(res)}")
+  for r in res:
+    if r[0] < 1 or r[1] < 1:
+      raise ValueError(f"camera resolutions must be positive, got {r}")
+  want_rgb = _per_camera(render_rgb, nrender, "render_rgb", True)
+  want_depth = _per_camera(render_depth, nrender, "render_depth", True)
+  want_seg = _per_camera(render_seg, nrender, "render_seg", True)
+  npix = [w * h for w, h in res]
+
+  def adrs(flags):
+    a, off = [], 0
+    for f, n in zip(flags, npix):
+      a.append(off if f else -1)
+      off += n if f else 0
+    return np.array(a, np.int32), max(off, 1)
+
+  rgb_adr, nrgb = adrs(want_rgb)
+  depth_adr, ndepth = adrs(want_depth)
+  seg_adr, nseg = adrs(want_seg)
+
+  dev = torch.device(device)
+  enabled = np.nonzero(rendered)[0].astype(np.int32)
+  depth = bvh_lib.scene_depth(len(enabled))
+  tlas_box, tlas_leaf = bvh_lib.alloc_scene_bvh(nworld, depth, dev)
+  meshes = bvh_lib.build_mesh_bvhs(mjm)
+
+  base = RenderBase(
+    nworld=nworld,
+    nrender=nrender,
+    cam_id_map=np.array(cams, np.int32),
+    cam_res=np.array(res, np.int32),
+    render_rgb=want_rgb,
+    render_depth=want_depth,
+    render_seg=want_seg,
+    rgb_adr=rgb_adr,
+    depth_adr=depth_adr,
+    seg_adr=seg_adr,
+    total_rays=int(sum(npix)),
+    rgb_data=torch.zeros((nworld, nrgb), dtype=torch.int32, device=dev),
+    depth_data=torch.zeros((nworld, ndepth), dtype=torch.float32, device=dev),
+    seg_data=torch.zeros((nworld, nseg, 2), dtype=torch.int32, device=dev),
+    znear=0.01,
+    zfar=0.0,
+    enabled_geom_ids=torch.as_tensor(enabled).to(dev),
+    tlas_depth=depth,
+    tlas_box=tlas_box,
+    tlas_leaf=tlas_leaf,
+    blas_depth=torch.as_tensor(meshes.depth).to(dev),
+    blas_node_adr=torch.as_tensor(meshes.node_adr).to(dev),
+    blas_tri_adr=torch.as_tensor(meshes.tri_adr).to(dev),
+    blas_box=torch.as_tensor(meshes.box).to(dev),
+    blas_tri=torch.as_tensor(meshes.tri).to(dev),
+    mesh_aabb=torch.as_tensor(meshes.aabb).to(dev),
+    enable_backface_culling=enable_backface_culling,
+    enable_vertex_normals=enable_vertex_normals,
+    use_fast_math=use_fast_math,
+    headlight_active=bool(mjm.headlight_active),
+    headlight_diffuse=tuple(float(x) for x in mjm.headlight_diffuse),
+    headlight_specular=tuple(float(x) for x in mjm.headlight_specular),
+    has_spot_lights=bool(mjm.nlight and np.any(mjm.light_type == mdl.LIGHT_SPOT)),
+  )
+
+  physical = light_model == LightModel.PHYSICAL
+  lighting = material.resolve_lighting(mjm, light_model, ev100)
+  metallic, roughness, reflectance = material.resolve_geom_materials(mjm, unset_material, default_material)
+  rendered_mats = np.unique(mjm.geom_matid[rendered & (mjm.geom_matid >= 0)])
+  features = material.SurfaceFeatures(mjm, use_textures, specular_aa and tier == Tier.REALTIME, rendered_mats)
+  used_tex = np.zeros(max(mjm.ntex, 1), dtype=bool)
+  if features.textures:
+    ids = mjm.mat_texid[rendered_mats].ravel()
+    used_tex[ids[ids >= 0]] = True
+  textures = texture.upload_textures(mjm, used_tex, dev)
+  alpha = material.effective_geom_rgba(mjm)[rendered, 3]
+  has_transparency = bool(np.any((alpha > 0.0) & (alpha < 1.0))) or features.alpha_map
+  has_reflection = enable_reflections and bool(mjm.nmat and np.any(mjm.mat_reflectance[rendered_mats] > 0.0))
+  background_linear = tuple(float(x) for x in material.srgb_to_linear(background_color[:3]))
+
+  images = _envmap_images(envmap)
+  if images:
+    env = envmap_lib.build_environments(images, dev, prefilter=precompute)
+    has_env = any_env = True
+  else:
+    env = envmap_lib.empty_environments(dev)
+    has_env = any_env = False
+  if envmap_intensity is None:
+    envmap_intensity = 1.0 / lighting.exposure
+  if has_env and physical:
+    lighting.ambient[:] = 0.0
+
+  dfg = torch.zeros((DFG_LUT_SIZE, DFG_LUT_SIZE, 2), dtype=torch.float32, device=dev)
+  if precompute:
+    _abi.call("mjpbr_dfg_lut", _abi.ptr(dfg), DFG_LUT_SIZE, DFG_LUT_SAMPLES, envmap_lib._stream(dev))
+
+  npixel = int(nrgb)
+  aov = npixel if denoise else 1
+
+  def per_world(values, dtype=torch.float32):
+    return torch.as_tensor(np.asarray(values, dtype=np.float32).reshape(1, -1)).to(dev)
+
+  def zeros(shape, dtype=torch.float32):
+    return torch.zeros(shape, dtype=dtype, device=dev)
+
+  rc = PbrRenderContext(
+    base=base,
+    light_model=int(light_model),
+    tone_mapper=int(tone_mapper),
+    use_shadows=use_shadows,
+    shadow_light_fraction=shadow_light_fraction,
+    geom_metallic=per_world(metallic),
+    geom_roughness=per_world(roughness),
+    geom_reflectance=per_world(reflectance),
+    mat_reflectance=per_world(mjm.mat_reflectance if mjm.nmat else np.zeros(1)),
+    light_intensity=per_world(lighting.light_intensity if mjm.nlight else np.zeros(1)),
+    light_range=per_world(mjm.light_range if mjm.nlight else np.zeros(1)),
+    light_bulbradius=per_world(mjm.light_bulbradius if mjm.nlight else np.zeros(1)),
+    headlight_intensity=lighting.headlight_intensity,
+    exposure=torch.tensor([lighting.exposure], dtype=torch.float32, device=dev),
+    ambient_radiance=torch.as_tensor(lighting.ambient.reshape(1, 3).astype(np.float32)).to(dev),
+    dfg_lut=dfg,
+    background_linear=background_linear,
+    has_env=has_env,
+    env_background=any_env and envmap_background,
+    env=env,
+    env_texel_angle=envmap_lib.texel_angle(env.width),
+    env_id=torch.zeros(1, dtype=torch.int32, device=dev),
+    env_intensity=torch.tensor([envmap_intensity], dtype=torch.float32, device=dev),
+    env_yaw=torch.tensor([math.radians(envmap_rotation)], dtype=torch.float32, device=dev),
+    textures=textures,
+    features=features,
+    samples_per_pixel=samples_per_pixel,
+    shadow_samples=shadow_samples,
+    has_transparency=has_transparency,
+    max_layers=max_layers,
+    has_reflection=has_reflection,
+    render_order=int(render_order),
+    block_dim=block_dim,
+    max_registers=max_registers or 0,
+    ordered_traversal=True,
+    tier=int(tier),
+    path_samples=path_samples,
+    max_depth=max_depth,
+    rr_depth=rr_depth,
+    denoise=denoise,
+    denoise_iterations=denoise_iterations,
+    aov_albedo=zeros((nworld, aov, 3), torch.int16),
+    aov_normal=zeros((nworld, aov, 3), torch.int16),
+    aov_depth=zeros((nworld, aov)),
+    aov_emission=zeros((nworld, aov, 3)),
+    aov_variance=zeros((nworld, aov)),
+    denoise_buffers=(zeros((nworld, aov, 4)), zeros((nworld, aov, 4))),
+    render_hdr=render_hdr,
+    hdr_data=zeros((nworld, npixel if render_hdr or denoise else 1, 3)),
+    render_guides=render_guides,
+    guide_albedo=zeros((nworld, npixel if render_guides else 1, 3)),
+    guide_normal=zeros((nworld, npixel if render_guides else 1, 3)),
+    traversal=traversal,
+  )
+  return rc
+
+
+def render_struct_max_cams() -> int:
+  return 16
+
+
+def build_scene_bvh(m, d, rc) -> None:
+  """Builds the per-world scene BVH (leaf assignment and boxes) from the current poses."""
+  bvh_lib.refit_bvh(m, d, rc, rebuild=True)

@@ -1,0 +1,92 @@
+"""Per-camera readback into caller-allocated tensors (reference output.py:34-152).
+
+Each function copies camera `camera_index` of every world (the first `out.shape[0]` worlds)
+into `out`, whose shape is checked; nothing synchronises with the host.
+"""
+
+from __future__ import annotations
+
+import torch
+
+from mujoco_warp_pbr_amd import _abi
+from mujoco_warp_pbr_amd.render import _stream_of
+from mujoco_warp_pbr_amd.types import PbrRenderContext
+
+
+def _camera(rc: PbrRenderContext, camera_index: int, adr_name: str, what: str):
+  base = rc.base
+  if not 0 <= camera_index < base.nrender:
+    raise ValueError(f"camera_index must be in [0, {base.nrender}), got {camera_index}")
+  adr = int(getattr(base, adr_name)[camera_index])
+  if adr < 0:
+    raise ValueError(f"camera {camera_index} does not render {what}")
+  w, h = (int(x) for x in base.cam_res[camera_index])
+  return adr, w, h
+
+
+def _check_out(out: torch.Tensor, rc: PbrRenderContext, h: int, w: int, tail: tuple, dtype, name: str):
+  n = out.shape[0] if out.dim() else 0
+  shape = (n, h, w, *tail)
+  if out.dim() != len(shape) or tuple(out.shape) != shape or n < 1 or n > rc.base.nworld:
+    raise ValueError(f"{name} must have shape (n <= {rc.base.nworld}, {h}, {w}{''.join(', ' + str(t) for t in tail)}), got {tuple(out.shape)}")
+  if out.dtype != dtype:
+    raise ValueError(f"{name} must have dtype {dtype}, got {out.dtype}")
+  return n
+
+
+def get_rgb(rc: PbrRenderContext, camera_index: int, rgb_out: torch.Tensor) -> None:
+  """Display RGB in [0, 1] of one camera, (n, h, w, 3) float32 (packed sRGB8 / 255)."""
+  adr, w, h = _camera(rc, camera_index, "rgb_adr", "rgb")
+  n = _check_out(rgb_out, rc, h, w, (3,), torch.float32, "rgb_out")
+  bits = rc.base.rgb_data[:n, adr : adr + w * h]
+  for c in range(3):
+    rgb_out[..., c] = ((bits >> (8 * c)) & 0xFF).reshape(n, h, w).to(torch.float32) / 255.0
+
+
+def get_depth(rc: PbrRenderContext, camera_index: int, depth_scale: float, depth_out: torch.Tensor) -> None:
+  """Planar depth times depth_scale of one camera, (n, h, w) float32."""
+  adr, w, h = _camera(rc, camera_index, "depth_adr", "depth")
+  n = _check_out(depth_out, rc, h, w, (), torch.float32, "depth_out")
+  depth_out.copy_(rc.base.depth_data[:n, adr : adr + w * h].reshape(n, h, w) * depth_scale)
+
+
+def get_segmentation(rc: PbrRenderContext, camera_index: int, segmentation_out: torch.Tensor) -> None:
+  """(geom id, object type) of one camera, (n, h, w, 2) int32; (-1, -1) for the background."""
+  adr, w, h = _camera(rc, camera_index, "seg_adr", "segmentation")
+  n = _check_out(segmentation_out, rc, h, w, (2,), torch.int32, "segmentation_out")
+  segmentation_out.copy_(rc.base.seg_data[:n, adr : adr + w * h].reshape(n, h, w, 2))
+
+
+def _get_rgb_image(rc: PbrRenderContext, buffer: torch.Tensor, camera_index: int, out: torch.Tensor, what: str) -> None:
+  adr, w, h = _camera(rc, camera_index, "rgb_adr", "rgb")
+  if buffer.shape[1] == 1 and w * h > 1:
+    raise ValueError(f"the context does not record {what}")
+  n = _check_out(out, rc, h, w, (3,), torch.float32, "out")
+  if not out.is_contiguous():
+    raise ValueError("out must be contiguous")
+  _abi.call(
+    "mjpbr_extract_rgb_image",
+    _abi.ptr(buffer),
+    int(buffer.shape[1]),
+    adr,
+    w,
+    h,
+    n,
+    _abi.ptr(out),
+    _stream_of(buffer),
+  )
+
+
+def get_hdr(rc: PbrRenderContext, camera_index: int, hdr_out: torch.Tensor) -> None:
+  """Linear HDR radiance (before exposure and tone mapping), (n, h, w, 3); needs render_hdr."""
+  _get_rgb_image(rc, rc.hdr_data, camera_index, hdr_out, "HDR (create it with render_hdr=True)")
+
+
+def get_albedo(rc: PbrRenderContext, camera_index: int, albedo_out: torch.Tensor) -> None:
+  """First-hit linear base colour, (n, h, w, 3); needs render_guides."""
+  _get_rgb_image(rc, rc.guide_albedo, camera_index, albedo_out, "guides (create it with render_guides=True)")
+
+
+def get_normal(rc: PbrRenderContext, camera_index: int, normal_out: torch.Tensor) -> None:
+  """First-hit world shading normal, (n, h, w, 3); needs render_guides."""
+  _get_rgb_image(rc, rc.guide_normal, camera_index, normal_out, "guides (create it with render_guides=True)")

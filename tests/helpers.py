@@ -1,0 +1,77 @@
+"""Scene / context builders and comparisons shared by the tests."""
+
+from __future__ import annotations
+
+import math
+
+import numpy as np
+import torch
+
+import mujoco_warp_pbr_amd as mj
+from mujoco_warp_pbr_amd import model as mdl
+from mujoco_warp_pbr_amd import scenes
+from mujoco_warp_pbr_amd.scenes import _Builder
+
+EV100_UNIT = math.log2(1.0 / 1.2)
+
+
+def make(mjm, nworld=1, device="cpu", cam_res=(32, 32), **kw):
+  """(m, d, rc) on `device`; on the CPU the GPU-built tables come from the oracle."""
+  import oracle
+
+  kw.setdefault("ev100", EV100_UNIT)
+  kw.setdefault("render_hdr", True)
+  m = mj.put_model(mjm, device)
+  d = mj.put_data(mjm, nworld, device)
+  cpu = torch.device(device).type != "cuda"
+  rc = mj.create_render_context(mjm, nworld=nworld, cam_res=cam_res, device=device, precompute=not cpu, **kw)
+  if cpu:
+    oracle.fill_tables(rc, kw.pop("_dfg_samples", 256), 64)
+  return m, d, rc
+
+
+def plane_and_light(light_type, pos=(0, 0, 2.0), direction=(0, 0, -1), rgba=(0.8, 0.8, 0.8, 1.0), intensity=1.0, res=8,
+                    cam_height=1.0, fovy=10.0, range_=100.0):
+  """Lambertian plane (roughness 1, reflectance 0) lit by one light, camera looking straight down."""
+  b = _Builder()
+  mat = b.material(rgba=rgba, metallic=0.0, roughness=1.0)
+  b.geom(mdl.GEOM_PLANE, (5, 5, 0.1), (0, 0, 0), matid=mat)
+  b.light(light_type, pos, direction, intensity=intensity, castshadow=0, range_=range_)
+  b.camera((0, 0, cam_height), (0, 0.0001, 0), fovy, (res, res))
+  mjm = b.build()
+  mjm.cam_xmat[0] = np.eye(3, dtype=np.float32).reshape(9)  # exactly straight down
+  mjm.mat_specular[:] = 0.0
+  return mjm
+
+
+def hdr_image(rc, cam=0):
+  w, h = (int(x) for x in rc.base.cam_res[cam])
+  adr = int(rc.base.rgb_adr[cam])
+  return rc.hdr_data[:, adr : adr + w * h].reshape(-1, h, w, 3).cpu().numpy()
+
+
+def compare_rt(gpu: dict, ref: dict, hdr_atol=1e-3, depth_rtol=1e-5):
+  """Parity of a GPU render with the oracle: seg bit-exact, depth rtol, HDR atol, RGB8 +-1."""
+  np.testing.assert_array_equal(gpu["seg"], ref["seg"])
+  dg, dr = gpu["depth"], ref["depth"]
+  np.testing.assert_allclose(dg, dr, rtol=depth_rtol, atol=1e-7)
+  np.testing.assert_allclose(gpu["hdr"], ref["hdr"], atol=hdr_atol, rtol=0)
+  a = gpu["rgb"].view(np.uint32)
+  b = ref["rgb"].view(np.uint32)
+  for sh in (0, 8, 16):
+    ca = ((a >> sh) & 0xFF).astype(np.int32)
+    cb = ((b >> sh) & 0xFF).astype(np.int32)
+    assert np.abs(ca - cb).max() <= 1
+
+
+def gpu_outputs(rc) -> dict:
+  torch.cuda.synchronize()
+  base = rc.base
+  return {
+    "rgb": base.rgb_data.cpu().numpy().view(np.uint32),
+    "depth": base.depth_data.cpu().numpy(),
+    "seg": base.seg_data.cpu().numpy(),
+    "hdr": rc.hdr_data.cpu().numpy(),
+    "albedo": rc.guide_albedo.cpu().numpy(),
+    "normal": rc.guide_normal.cpu().numpy(),
+  }
