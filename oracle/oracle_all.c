@@ -1,0 +1,5 @@
+/* One translation unit for the whole CPU oracle (test infrastructure only): the path-tracer and
+ * denoiser restatements reuse the real-time tier's scalar functions. */
+#include "rt_oracle.c"
+#include "pt_oracle.c"
+#include "atrous_oracle.c"
