@@ -285,6 +285,19 @@ int mjpbr_scene_bvh(const MjpbrGeometry* geo, int refit, void* stream);
 int mjpbr_rt_render(const MjpbrScene* scene, const MjpbrSettings* s, MjpbrOutputs* out, int32_t seed,
                     void* stream);
 
+/* Path-traced tier megakernel: one thread per (world, pixel), `path_samples` paths each
+ * (reference pathtrace.py:340-438). With s->denoise it writes the noisy HDR and the first-hit
+ * features (out->aov_*) instead of RGB. */
+int mjpbr_pt_render(const MjpbrScene* scene, const MjpbrSettings* s, MjpbrOutputs* out, int32_t seed,
+                    void* stream);
+
+/* À-trous denoiser over the path tracer's noisy HDR and first-hit features (out->hdr, aov_*):
+ * demodulate, `iterations` edge-avoiding passes, remodulate + tone map + encode into out->rgb
+ * (and out->hdr with render_hdr) (reference denoise.py:214-248). work0 / work1: two
+ * [nworld, stride, 4] f32 buffers. */
+int mjpbr_atrous(const MjpbrAtrousParams* p, const MjpbrOutputs* out, float* work0, float* work1,
+                 void* stream);
+
 /* Context precompute (reference ibl.py:62-104, envmap.py:171-221). */
 int mjpbr_dfg_lut(float* lut, int size, int nsample, void* stream);
 int mjpbr_env_prefilter(const uint16_t* radiance, const int64_t* level_adr, const int32_t* desc,

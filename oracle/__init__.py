@@ -124,6 +124,91 @@ def rt_render(m, d, rc, seed: int = 0, threads: int = 1, worlds: tuple[int, int]
   return _outputs(hrc)
 
 
+def _aov(rc) -> dict[str, np.ndarray]:
+  return {
+    "aov_albedo": rc.aov_albedo.numpy().view(np.float16).astype(np.float32).copy(),
+    "aov_normal": rc.aov_normal.numpy().view(np.float16).astype(np.float32).copy(),
+    "aov_depth": rc.aov_depth.numpy().copy(),
+    "aov_emission": rc.aov_emission.numpy().copy(),
+    "aov_variance": rc.aov_variance.numpy().copy(),
+  }
+
+
+def pt_render(m, d, rc, seed: int = 0, threads: int = 1, worlds: tuple[int, int] | None = None, per_sample: bool = False) -> dict:
+  """Path-traced tier on the CPU oracle. per_sample=True also returns every sample's radiance,
+  (nworld, total_rays, path_samples, 3) in camera-concatenated row-major pixel order."""
+  from mujoco_warp_pbr_amd.render import outputs_struct
+  from mujoco_warp_pbr_amd.render import scene_struct
+  from mujoco_warp_pbr_amd.render import settings_struct
+
+  hm, hd, hrc = to_host(m, d, rc)
+  for t in (hrc.base.rgb_data, hrc.base.depth_data, hrc.base.seg_data, hrc.hdr_data, hrc.guide_albedo, hrc.guide_normal):
+    t.zero_()
+  sc = scene_struct(hm, hd, hrc)
+  s = settings_struct(hrc)
+  out = outputs_struct(hrc)
+  lib = library()
+  vp = ctypes.c_void_p
+  lib.oracle_pt_render.argtypes = [vp, vp, vp, ctypes.c_int32, ctypes.c_int, ctypes.c_int, vp]
+  w0, w1 = worlds if worlds is not None else (0, hd.nworld)
+  samples = None
+  if per_sample:
+    samples = np.zeros((w1 - w0, hrc.base.total_rays, hrc.path_samples, 3), np.float32)
+  args = (vp(ctypes.addressof(sc)), vp(ctypes.addressof(s)), vp(ctypes.addressof(out)))
+
+  def run(a, b):
+    ptr = None if samples is None else vp(samples.ctypes.data + (a - w0) * samples[0].nbytes)
+    lib.oracle_pt_render(*args, int(seed), int(a), int(b), ptr)
+
+  if threads <= 1:
+    run(w0, w1)
+  else:
+    bounds = np.linspace(w0, w1, threads + 1).astype(int)
+    pool = [threading.Thread(target=run, args=(int(a), int(b))) for a, b in zip(bounds[:-1], bounds[1:]) if b > a]
+    for th in pool:
+      th.start()
+    for th in pool:
+      th.join()
+  result = _aov(hrc)
+  result["noisy_hdr"] = hrc.hdr_data.numpy().copy()
+  if hrc.denoise:
+    _run_atrous(hrc, out)
+  result.update(_outputs(hrc))
+  if samples is not None:
+    result["samples"] = samples
+  return result
+
+
+def _run_atrous(hrc, out) -> None:
+  from mujoco_warp_pbr_amd.denoise import atrous_params
+
+  lib = library()
+  vp = ctypes.c_void_p
+  lib.oracle_atrous.argtypes = [vp, vp, vp, vp]
+  p = atrous_params(hrc, hrc.base.nworld)
+  n = hrc.base.nworld * int(hrc.base.rgb_data.shape[1])
+  w0 = np.zeros((n, 4), np.float32)
+  w1 = np.zeros((n, 4), np.float32)
+  lib.oracle_atrous(vp(ctypes.addressof(p)), vp(ctypes.addressof(out)), w0.ctypes.data, w1.ctypes.data)
+
+
+def atrous(rc, hdr: np.ndarray, aov: dict, render_hdr: bool = True) -> dict:
+  """The oracle's a-trous filter of a context's layout on the given noisy HDR and features
+  (host arrays as returned by the renderers); returns rgb and (filtered) hdr."""
+  from mujoco_warp_pbr_amd.render import outputs_struct
+
+  (hrc,) = to_host(rc)
+  hrc.render_hdr = render_hdr
+  hrc.hdr_data.copy_(torch.from_numpy(np.ascontiguousarray(hdr, np.float32)))
+  hrc.aov_albedo.copy_(torch.from_numpy(np.ascontiguousarray(aov["aov_albedo"], np.float16).view(np.int16)))
+  hrc.aov_normal.copy_(torch.from_numpy(np.ascontiguousarray(aov["aov_normal"], np.float16).view(np.int16)))
+  for k in ("aov_depth", "aov_emission", "aov_variance"):
+    getattr(hrc, k).copy_(torch.from_numpy(np.ascontiguousarray(aov[k], np.float32)))
+  out = outputs_struct(hrc)
+  _run_atrous(hrc, out)
+  return {"rgb": hrc.base.rgb_data.numpy().view(np.uint32).copy(), "hdr": hrc.hdr_data.numpy().copy()}
+
+
 def dfg_lut(size: int = 128, nsample: int = 1024) -> np.ndarray:
   lut = np.zeros((size, size, 2), np.float32)
   library().oracle_dfg_lut(lut.ctypes.data, size, nsample)
