@@ -93,15 +93,22 @@ MJ_DEV V3 shade_surface(const RtCtx& c, const Surface& sf, V3 point, V3 normal, 
   float ao = sf.occlusion;
   float spec_ao = clampf(powf(nov + ao, exp2f(-16.0f * alpha - 1.0f)) - 1.0f + ao, 0.0f, 1.0f);
   V3 radiance = cw(irradiance, lb.diffuse_color) * ao + cw(prefiltered, cw(lb.spec_albedo, lb.energy_comp)) * spec_ao;
+  // The model's lights, then the headlight as one more (shadowless, directional) light: one
+  // inlined copy of the light code.
   const MjpbrLights& lt = sc.lit;
-  for (int li = 0; li < lt.nlight; ++li) {
-    if (!bat(lt.light_active, lt.light_active_n, c.w, lt.nlight)[li]) continue;
-    Light L = fetch_light(lt, c.w, li);
-    radiance += shade_light(c, L, lb, point, normal, view, nov, stream, lt.nlight, li);
-  }
-  if (s.headlight_active) {
-    Light H = headlight(s, s.headlight_intensity / c.exposure, head_dir);
-    radiance += shade_light(c, H, lb, point, normal, view, nov, stream, 1, 0);
+  int nl = lt.nlight + (s.headlight_active ? 1 : 0);
+  for (int li = 0; li < nl; ++li) {
+    Light L;
+    int nlight = lt.nlight, lidx = li;
+    if (li < lt.nlight) {
+      if (!bat(lt.light_active, lt.light_active_n, c.w, lt.nlight)[li]) continue;
+      L = fetch_light(lt, c.w, li);
+    } else {
+      L = headlight(s, s.headlight_intensity / c.exposure, head_dir);
+      nlight = 1;
+      lidx = 0;
+    }
+    radiance += shade_light(c, L, lb, point, normal, view, nov, stream, nlight, lidx);
   }
   return radiance + sf.emissive / c.exposure;
 }
@@ -119,59 +126,86 @@ MJ_DEV V3 background(const RtCtx& c, V3 dir, float spread) {
   return ld3(env.background) / c.exposure;
 }
 
-// render.py:249-274
-MJ_DEV V3 shade_reflection(const RtCtx& c, V3 point, V3 normal, V3 dir, float footprint, float spread, V3 head_dir,
-                           int stream) {
-  V3 r = dir - normal * (2.0f * dot(dir, normal));
-  V3 origin = point + normal * kRayOffset;
-  Hit hit = trace_closest(c.sv, origin, r, kMaxVal, c.s.cull_backfaces != 0);
-  if (hit.geom_id == -1) return background(c, r, spread);
-  V3 p = origin + r * hit.dist;
-  V3 n;
-  Surface sf = eval_surface(c.sc, c.s, c.w, hit, p, r, footprint + spread * hit.dist, &n);
-  return shade_surface(c, sf, p, n, -r, head_dir, stream);
-}
-
-// render.py:276-341
+// render.py:249-341. The camera-ray segments (translucent layers) and the mirror segment of
+// each layer share one trace / surface / shading site: a segment loop with a mode flag instead of
+// the reference's nested calls (shade_reflection), so the megakernel holds one inlined copy of
+// the traversal and shading code. Arithmetic and random streams are those of the reference:
+// the mirror segment traces from point + normal * RAY_OFFSET with max distance MJ_MAXVAL, is
+// shaded with stream + 1 and blends as shaded * (1 - mirror) + reflected * mirror.
 MJ_DEV V3 trace_pixel(const RtCtx& c, V3 origin, V3 dir, float max_dist, float footprint, float spread, V3 head_dir,
                       int stream, Hit* first, V3* first_albedo, V3* first_normal) {
   const MjpbrSettings& s = c.s;
+  const bool cull = s.cull_backfaces != 0;
   V3 radiance = v3s(0.0f);
   float throughput = 1.0f, travelled = 0.0f;
   first->geom_id = -1;
   *first_albedo = v3s(0.0f);
   *first_normal = v3s(0.0f);
-  int nlayer = s.has_transparency ? s.max_layers : 1;
-  for (int layer = 0; layer < nlayer; ++layer) {
-    Hit hit = trace_closest(c.sv, origin, dir, max_dist - travelled, s.cull_backfaces != 0);
-    if (layer == 0) *first = hit;
+  const int nlayer = s.has_transparency ? s.max_layers : 1;
+  int layer = 0;
+  bool mirror_seg = false;
+  // state of the layer whose mirror ray is being traced
+  V3 layer_shaded = v3s(0.0f), layer_point = v3s(0.0f);
+  float layer_mirror = 0.0f, layer_alpha = 1.0f, layer_dist = 0.0f, layer_cone = 0.0f;
+  V3 ro = origin, rd = dir;
+  float rmax = max_dist - travelled;
+  for (;;) {
+    Hit hit = trace_closest(c.sv, ro, rd, rmax, cull);
+    if (!mirror_seg && layer == 0) *first = hit;
+    V3 shaded;
+    V3 normal = v3s(0.0f);
+    V3 point = v3s(0.0f);
+    float cone = 0.0f;
+    float alpha = 1.0f;
     if (hit.geom_id == -1) {
-      radiance += background(c, dir, spread) * throughput;
-      break;
-    }
-    V3 point = origin + dir * hit.dist;
-    float cone = footprint + spread * (travelled + hit.dist);
-    V3 normal;
-    Surface sf = eval_surface(c.sc, s, c.w, hit, point, dir, cone, &normal);
-    if (layer == 0 && s.render_guides) {
-      *first_albedo = sf.base_color;
-      *first_normal = normal;
-    }
-    V3 shaded = shade_surface(c, sf, point, normal, -dir, head_dir, stream);
-    if (s.has_reflection) {
-      float mirror = mirror_reflectance(c.sc, c.w, hit.geom_id);
-      if (mirror > 0.0f) {
-        V3 refl = shade_reflection(c, point, normal, dir, cone, spread, head_dir, stream + 1);
-        shaded = shaded * (1.0f - mirror) + refl * mirror;
+      shaded = background(c, rd, spread);
+      if (!mirror_seg) {
+        radiance += shaded * throughput;
+        break;
       }
+    } else {
+      point = ro + rd * hit.dist;
+      cone = mirror_seg ? layer_cone + spread * hit.dist : footprint + spread * (travelled + hit.dist);
+      Surface sf = eval_surface(c.sc, s, c.w, hit, point, rd, cone, &normal);
+      alpha = sf.alpha;
+      if (!mirror_seg && layer == 0 && s.render_guides) {
+        *first_albedo = sf.base_color;
+        *first_normal = normal;
+      }
+      shaded = shade_surface(c, sf, point, normal, -rd, head_dir, stream + (mirror_seg ? 1 : 0));
+    }
+    if (!mirror_seg) {
+      float mirror = s.has_reflection ? mirror_reflectance(c.sc, c.w, hit.geom_id) : 0.0f;
+      if (mirror > 0.0f) {
+        layer_shaded = shaded;
+        layer_point = point;
+        layer_mirror = mirror;
+        layer_alpha = alpha;
+        layer_dist = hit.dist;
+        layer_cone = cone;
+        mirror_seg = true;
+        ro = point + normal * kRayOffset;
+        rd = dir - normal * (2.0f * dot(dir, normal));
+        rmax = kMaxVal;
+        continue;
+      }
+      layer_point = point;
+      layer_alpha = alpha;
+      layer_dist = hit.dist;
+    } else {
+      shaded = layer_shaded * (1.0f - layer_mirror) + shaded * layer_mirror;
+      mirror_seg = false;
     }
     if (s.has_transparency) {
-      float opacity = clampf(sf.alpha, 0.0f, 1.0f);
+      float opacity = clampf(layer_alpha, 0.0f, 1.0f);
       radiance += shaded * (throughput * opacity);
       throughput *= 1.0f - opacity;
       if (throughput < kMinThroughput) break;
-      travelled += hit.dist;
-      origin = point + dir * kRayOffset;
+      travelled += layer_dist;
+      if (++layer >= nlayer) break;
+      ro = layer_point + dir * kRayOffset;
+      rd = dir;
+      rmax = max_dist - travelled;
     } else {
       radiance += shaded;
       break;
