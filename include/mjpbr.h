@@ -308,6 +308,67 @@ int mjpbr_env_prefilter(const uint16_t* radiance, const int64_t* level_adr, cons
 int mjpbr_extract_rgb_image(const float* src, int src_stride, int adr, int width, int height, int nout,
                             float* dst, void* stream);
 
+/* ---------------------------------------------------------------- learned denoiser (U-Net)
+ * No counterpart in the reference, whose only denoiser is the a-trous filter
+ * (mujoco_warp_pbr/denoise.py:63-201). These are the building blocks of a 3-level U-Net trained
+ * on the renderer's own low/high-sample pairs. Activations are NHWC BF16 (uint16 bit patterns)
+ * with channel counts that are multiples of 32 (inputs of a convolution) or 16 (outputs);
+ * accumulation is FP32 on matrix cores. Every call is asynchronous on `stream`. */
+
+/* 3x3 convolution, zero padding, stride 1: y = act(bias + conv(x * (mask > 0), w)).
+ * w is [cout][9][cin] BF16 (forward) or the mirrored transposed layout [cin][9][cout] for a data
+ * gradient (then x is the output gradient and mask the layer's output activation: ReLU backward
+ * fused into the load). bias may be NULL; y32, if not NULL, receives the first out_f32 channels
+ * in FP32 ([pixel][out_f32]). h and w must be multiples of 16, cin of 32, cout of 16. */
+int mjpbr_unet_conv3x3(const uint16_t* x, const uint16_t* mask, const uint16_t* w, const float* bias, uint16_t* y, float* y32,
+                       int out_f32, int n, int h, int wd, int cin, int cout, int relu, void* stream);
+/* Floats of the `part` workspace of mjpbr_unet_conv3x3_wgrad with nchunk pixel chunks. */
+int64_t mjpbr_unet_wgrad_workspace(int n, int h, int wd, int cin, int cout, int nchunk);
+/* Weight gradient grad[cout][9][cin] (FP32, overwritten) of a 3x3 convolution with input x and
+ * output gradient g * (mask > 0); and, if bias_grad is not NULL, the bias gradient (bias_part
+ * holds mjpbr_unet_bias_workspace floats). h and w multiples of 8, cin of 16, cout of 16. */
+int mjpbr_unet_conv3x3_wgrad(const uint16_t* x, const uint16_t* g, const uint16_t* mask, float* part, int nchunk, float* grad,
+                             float* bias_part, float* bias_grad, int n, int h, int wd, int cin, int cout, void* stream);
+int64_t mjpbr_unet_bias_workspace(int n, int h, int wd, int cout);
+/* 2x2 average pooling (grad = 0: x [n][h][w][c] -> y [n][h/2][w/2][c]); grad = 1: the pooling's
+ * backward (x at h/2 x w/2 -> y at h x w). */
+int mjpbr_unet_avgpool2(const uint16_t* x, uint16_t* y, int n, int h, int wd, int c, int grad, void* stream);
+/* out [n][h][w][cl + cs] = concat(nearest 2x upsampling of low [n][h/2][w/2][cl], skip). */
+int mjpbr_unet_upcat(const uint16_t* low, const uint16_t* skip, uint16_t* out, int n, int h, int wd, int cl, int cs,
+                     void* stream);
+/* low_grad [n][h/2][w/2][cl] = 2x2 sums of the first cl channels of g [n][h][w][cl + cs]. */
+int mjpbr_unet_upcat_low_grad(const uint16_t* g, uint16_t* low_grad, int n, int h, int wd, int cl, int cs, void* stream);
+/* skip_grad [n][h][w][cs] = pooling backward of pool_grad [n][h/2][w/2][cs] + the last cs
+ * channels of cat_grad [n][h][w][cl + cs]. */
+int mjpbr_unet_skip_grad(const uint16_t* pool_grad, const uint16_t* cat_grad, uint16_t* skip_grad, int n, int h, int wd, int cs,
+                         int cl, void* stream);
+/* L1 loss of pred [npix][3] + the first 3 channels of base [npix][cb] (BF16; NULL for none)
+ * against target [npix][3]: loss_part[ceil(npix / 256)] partial sums of |d| and the BF16
+ * gradient sign(d) / (3 npix) into grad [npix][cp] (channels 3.. zero). */
+int mjpbr_unet_l1(const float* pred, const uint16_t* base, int cb, const float* target, int64_t npix, int cp, uint16_t* grad,
+                  float* loss_part, void* stream);
+/* Adam step `step` (1-based) on FP32 w/m/v with gradient g; if wf is not NULL, also refreshes
+ * the BF16 forward (wf) and data-gradient (wd) layouts of a [cout][9][cin] weight tensor. */
+int mjpbr_unet_adam(float* w, float* m, float* v, const float* g, int64_t count, float lr, float b1, float b2, float eps, int step,
+                    uint16_t* wf, uint16_t* wd, int cin, int cout, void* stream);
+/* BF16 forward and data-gradient layouts of FP32 weights [cout][9][cin]. */
+int mjpbr_unet_weight_layouts(const float* w, int cin, int cout, uint16_t* wf, uint16_t* wd, void* stream);
+/* Network input [n][h][w][cp] BF16 of one camera: log1p(max(hdr * exposure, 0)), albedo,
+ * normal, depth / (1 + depth), zeros. hdr is the per-world flat HDR buffer (vec3, stride
+ * hdr_stride pixels a world), albedo/normal FP16 vec3 and depth FP32 per-world AOVs (stride
+ * aov_stride), adr the camera's pixel offset; exposure per world (index world % exposure_n). */
+int mjpbr_unet_pack_input(const float* hdr, int hdr_stride, const uint16_t* albedo, const uint16_t* normal, const float* depth,
+                          int aov_stride, int adr, const float* exposure, int exposure_n, int n, int h, int wd, int cp,
+                          uint16_t* out, void* stream);
+/* Denoised HDR (expm1(max(pred + base, 0)) / exposure) and display RGB of one camera from the
+ * network output; hdr or rgb may be NULL. */
+int mjpbr_unet_unpack_output(const float* pred, const uint16_t* base, int cb, int n, int h, int wd, const float* exposure,
+                             int exposure_n, int tone_mapper, float* hdr, int hdr_stride, uint32_t* rgb, int rgb_stride, int adr,
+                             void* stream);
+/* D [16][16] FP32 = A [16][32] . B [32][16] (BF16) through the convolutions' matrix-core operand
+ * layout: a self-test of the lane mapping. */
+int mjpbr_unet_mfma_probe(const uint16_t* A, const uint16_t* B, float* D, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -49,6 +49,25 @@ FUNCTIONS = {
     [_VP, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _VP, _VP],
   ),
 }
+_I, _L, _F = ctypes.c_int, ctypes.c_int64, ctypes.c_float
+FUNCTIONS.update(
+  {
+    "mjpbr_unet_conv3x3": (_I, [_VP, _VP, _VP, _VP, _VP, _VP, _I, _I, _I, _I, _I, _I, _I, _VP]),
+    "mjpbr_unet_wgrad_workspace": (_L, [_I, _I, _I, _I, _I, _I]),
+    "mjpbr_unet_bias_workspace": (_L, [_I, _I, _I, _I]),
+    "mjpbr_unet_conv3x3_wgrad": (_I, [_VP, _VP, _VP, _VP, _I, _VP, _VP, _VP, _I, _I, _I, _I, _I, _VP]),
+    "mjpbr_unet_avgpool2": (_I, [_VP, _VP, _I, _I, _I, _I, _I, _VP]),
+    "mjpbr_unet_upcat": (_I, [_VP, _VP, _VP, _I, _I, _I, _I, _I, _VP]),
+    "mjpbr_unet_upcat_low_grad": (_I, [_VP, _VP, _I, _I, _I, _I, _I, _VP]),
+    "mjpbr_unet_skip_grad": (_I, [_VP, _VP, _VP, _I, _I, _I, _I, _I, _VP]),
+    "mjpbr_unet_l1": (_I, [_VP, _VP, _I, _VP, _L, _I, _VP, _VP, _VP]),
+    "mjpbr_unet_adam": (_I, [_VP, _VP, _VP, _VP, _L, _F, _F, _F, _F, _I, _VP, _VP, _I, _I, _VP]),
+    "mjpbr_unet_weight_layouts": (_I, [_VP, _I, _I, _VP, _VP, _VP]),
+    "mjpbr_unet_pack_input": (_I, [_VP, _I, _VP, _VP, _VP, _I, _I, _VP, _I, _I, _I, _I, _I, _VP, _VP]),
+    "mjpbr_unet_unpack_output": (_I, [_VP, _VP, _I, _I, _I, _I, _VP, _I, _I, _VP, _I, _VP, _I, _I, _VP]),
+    "mjpbr_unet_mfma_probe": (_I, [_VP, _VP, _VP, _VP]),
+  }
+)
 
 _SCALARS = {
   "int32_t": ctypes.c_int32,

@@ -1,0 +1,595 @@
+// unet.hip — learned denoiser kernels: 3x3 convolutions as implicit GEMMs on CDNA4 matrix cores
+// (v_mfma_f32_16x16x32_bf16: BF16 operands, FP32 accumulation), their data- and weight-gradient
+// passes, the elementwise layers of a 3-level U-Net (2x2 average pooling, 2x nearest upsampling,
+// channel concatenation), L1 loss and Adam.
+//
+// Activations are NHWC BF16 with the channel count a multiple of 32. Weights are kept in FP32
+// (master copy for Adam) and in two BF16 layouts: forward [Cout][9][Cin] and data-gradient
+// [Cin][9][Cout] with the taps mirrored (the transposed convolution). MFMA operand layout of
+// 16x16x32 (wave64): lane l holds A[m = l % 16][k = 8 (l / 16) .. + 8], B[k = 8 (l / 16) .. + 8][n =
+// l % 16] and receives D[m = 4 (l / 16) + i][n = l % 16], i = 0..3.
+#include "abi_util.hpp"
+
+namespace mjpbr {
+
+typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8;
+typedef __attribute__((__vector_size__(4 * sizeof(float)))) float floatx4;
+typedef uint16_t bf16_t;
+
+MJ_DEV float bf2f(bf16_t h) { return __uint_as_float((uint32_t)h << 16); }
+MJ_DEV bf16_t f2bf(float f) {
+  uint32_t u = __float_as_uint(f);
+  if ((u & 0x7fffffffu) > 0x7f800000u) return (bf16_t)((u >> 16) | 0x40);  // NaN stays NaN
+  u += 0x7fffu + ((u >> 16) & 1u);                                         // round to nearest even
+  return (bf16_t)(u >> 16);
+}
+
+MJ_DEV floatx4 mfma16(const uint4& a, const uint4& b, floatx4 c) {
+  bf16x8 va = __builtin_bit_cast(bf16x8, a);
+  bf16x8 vb = __builtin_bit_cast(bf16x8, b);
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(va, vb, c, 0, 0, 0);
+}
+
+// ------------------------------------------------------------------ forward / data-gradient conv
+// y[p][n] = act(bias[n] + sum_{tap, c} x[p + off(tap)][c] * w[n][tap][c]), zero padding.
+// With `mask`, the input is x * (mask > 0) (ReLU backward of the layer whose output is `mask`).
+// Workgroup: a 16x16 pixel block (4 waves x 4 rows of 16 pixels) x 64 output channels.
+constexpr int kConvBlock = 16;
+constexpr int kPatch = kConvBlock + 2;
+constexpr int kCinChunk = 32;
+constexpr int kNTile = 64;
+constexpr int kWRow = 9 * kCinChunk + 8;  // padded LDS row of one output channel's weights
+constexpr int kPStride = kCinChunk + 8;   // padded LDS channel stride of a patch position
+
+struct ConvArgs {
+  const bf16_t* x;
+  const bf16_t* mask;
+  const bf16_t* w;  // [Cout][9][Cin]
+  const float* bias;
+  bf16_t* y;
+  float* y32;  // optional FP32 output (first out_f32 channels)
+  int n, h, wd, cin, cout;
+  int relu;
+  int out_f32;
+};
+
+__global__ void __launch_bounds__(256) conv3x3_kernel(ConvArgs a) {
+  __shared__ __attribute__((aligned(16))) bf16_t patch[kPatch * kPatch * kPStride];
+  __shared__ __attribute__((aligned(16))) bf16_t wl[kNTile * kWRow];
+  const int bx = a.wd / kConvBlock, by = a.h / kConvBlock;
+  int blk = blockIdx.x;
+  const int img = blk / (bx * by);
+  blk -= img * bx * by;
+  const int y0 = (blk / bx) * kConvBlock, x0 = (blk % bx) * kConvBlock;
+  const int n0 = blockIdx.y * kNTile;
+  const int ntile = min(kNTile, a.cout - n0) / 16;  // 1..4 MFMA column tiles
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int lm = lane & 15, lk = lane >> 4;
+  floatx4 acc[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
+
+  for (int c0 = 0; c0 < a.cin; c0 += kCinChunk) {
+    __syncthreads();
+    // stage the input patch (18 x 18 positions x 32 channels), 16 bytes a thread-step
+    for (int e = threadIdx.x; e < kPatch * kPatch * 4; e += 256) {
+      int pos = e >> 2, part = e & 3;
+      int yy = y0 - 1 + pos / kPatch, xx = x0 - 1 + pos % kPatch;
+      uint4 v = make_uint4(0, 0, 0, 0);
+      if (yy >= 0 && yy < a.h && xx >= 0 && xx < a.wd) {
+        int64_t off = (((int64_t)img * a.h + yy) * a.wd + xx) * a.cin + c0 + part * 8;
+        v = *reinterpret_cast<const uint4*>(a.x + off);
+        if (a.mask) {
+          uint4 mk = *reinterpret_cast<const uint4*>(a.mask + (((int64_t)img * a.h + yy) * a.wd + xx) * a.cin + c0 + part * 8);
+          uint32_t* pv = reinterpret_cast<uint32_t*>(&v);
+          const uint32_t* pm = reinterpret_cast<const uint32_t*>(&mk);
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            uint32_t lo = (pm[q] & 0xffffu), hi = pm[q] >> 16;
+            // keep a value where the mask is positive (bf16 sign bit clear and not +0)
+            uint32_t keep_lo = (lo != 0 && !(lo & 0x8000u)) ? 0xffffu : 0u;
+            uint32_t keep_hi = (hi != 0 && !(hi & 0x8000u)) ? 0xffff0000u : 0u;
+            pv[q] &= (keep_lo | keep_hi);
+          }
+        }
+      }
+      *reinterpret_cast<uint4*>(patch + pos * kPStride + part * 8) = v;
+    }
+    // stage the weights of this input chunk: wl[n][tap][c]
+    for (int e = threadIdx.x; e < ntile * 16 * 9 * 4; e += 256) {
+      int n = e / 36, rem = e % 36, tap = rem >> 2, part = rem & 3;
+      uint4 v = *reinterpret_cast<const uint4*>(a.w + ((int64_t)(n0 + n) * 9 + tap) * a.cin + c0 + part * 8);
+      *reinterpret_cast<uint4*>(wl + n * kWRow + tap * kCinChunk + part * 8) = v;
+    }
+    __syncthreads();
+#pragma unroll 1
+    for (int tap = 0; tap < 9; ++tap) {
+      const int dy = tap / 3, dx = tap % 3;
+      uint4 bq[4];
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+        if (j < ntile) bq[j] = *reinterpret_cast<const uint4*>(wl + (j * 16 + lm) * kWRow + tap * kCinChunk + lk * 8);
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        int row = wave * 4 + i;
+        uint4 aq = *reinterpret_cast<const uint4*>(patch + ((row + dy) * kPatch + lm + dx) * kPStride + lk * 8);
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+          if (j < ntile) acc[i][j] = mfma16(aq, bq[j], acc[i][j]);
+      }
+    }
+  }
+  // epilogue: bias, ReLU, BF16 (and optional FP32) NHWC store
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    int row = wave * 4 + i;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      if (j >= ntile) continue;
+      int n = n0 + j * 16 + lm;
+      float b = a.bias ? a.bias[n] : 0.0f;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        int col = lk * 4 + q;  // pixel within the row
+        float v = acc[i][j][q] + b;
+        if (a.relu) v = fmaxf(v, 0.0f);
+        int64_t p = ((int64_t)img * a.h + y0 + row) * a.wd + x0 + col;
+        a.y[p * a.cout + n] = f2bf(v);
+        if (a.y32 && n < a.out_f32) a.y32[p * a.out_f32 + n] = v;
+      }
+    }
+  }
+}
+
+// ------------------------------------------------------------------ weight gradient
+// part[chunk][n][tap][c] = sum over the chunk's pixels p of x[p + off(tap)][c] * g[p][n] * (mask[p][n] > 0)
+// Workgroup: 16 input channels x 9 taps (M = 144) x up to 64 output channels, looping over 8x8
+// pixel blocks. LDS holds the blocks transposed (pixel-contiguous), so MFMA operands reduce over
+// pixels with 16-byte reads.
+constexpr int kWgBlock = 8;
+constexpr int kWgCin = 16;
+
+struct WgradArgs {
+  const bf16_t* x;
+  const bf16_t* g;
+  const bf16_t* mask;
+  float* part;  // [nchunk][cout][9][cin]
+  int n, h, wd, cin, cout;
+  int blocks_per_chunk;
+};
+
+__global__ void __launch_bounds__(256) conv3x3_wgrad_kernel(WgradArgs a) {
+  // xt[dx][c][r][k]: x at (block row r - 1, block col k - 1 + dx), r = 0..9, k = 0..7
+  __shared__ __attribute__((aligned(16))) bf16_t xt[3 * kWgCin * 10 * 8];
+  __shared__ __attribute__((aligned(16))) bf16_t gt[kNTile * 64];
+  const int c0 = blockIdx.x * kWgCin;
+  const int n0 = blockIdx.y * kNTile;
+  const int nn = min(kNTile, a.cout - n0);
+  const int ntile = nn / 16;
+  const int chunk = blockIdx.z;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int lm = lane & 15, lk = lane >> 4;
+  const int bx = a.wd / kWgBlock, by = a.h / kWgBlock;
+  const int nblocks = a.n * bx * by;
+  // tiles: t = tap * ntile + j; wave w owns t = w, w + 4, ...
+  floatx4 acc[9];
+#pragma unroll
+  for (int i = 0; i < 9; ++i) acc[i] = floatx4{0.f, 0.f, 0.f, 0.f};
+  const int ntiles = 9 * ntile;
+  int b_begin = chunk * a.blocks_per_chunk;
+  int b_end = min(nblocks, b_begin + a.blocks_per_chunk);
+  for (int blk = b_begin; blk < b_end; ++blk) {
+    int img = blk / (bx * by), rem = blk % (bx * by);
+    int y0 = (rem / bx) * kWgBlock, x0 = (rem % bx) * kWgBlock;
+    __syncthreads();
+    // x (transposed, three column shifts): 3 x 10 x 8 positions x 16 channels
+    for (int e = threadIdx.x; e < 3 * 10 * 8 * 2; e += 256) {
+      int half = e & 1, pos = e >> 1;
+      int k = pos & 7, r = (pos >> 3) % 10, dx = pos / 80;
+      int yy = y0 - 1 + r, xx = x0 - 1 + k + dx;
+      uint4 v = make_uint4(0, 0, 0, 0);
+      if (yy >= 0 && yy < a.h && xx >= 0 && xx < a.wd)
+        v = *reinterpret_cast<const uint4*>(a.x + (((int64_t)img * a.h + yy) * a.wd + xx) * a.cin + c0 + half * 8);
+      const bf16_t* pv = reinterpret_cast<const bf16_t*>(&v);
+#pragma unroll
+      for (int q = 0; q < 8; ++q) xt[((dx * kWgCin + half * 8 + q) * 10 + r) * 8 + k] = pv[q];
+    }
+    // g * mask (transposed): 64 pixels x nn channels
+    for (int e = threadIdx.x; e < 64 * (nn / 8); e += 256) {
+      int p = e % 64, part = e / 64;
+      int yy = y0 + p / 8, xx = x0 + p % 8;
+      int64_t off = (((int64_t)img * a.h + yy) * a.wd + xx) * a.cout + n0 + part * 8;
+      uint4 v = *reinterpret_cast<const uint4*>(a.g + off);
+      bf16_t* pv = reinterpret_cast<bf16_t*>(&v);
+      if (a.mask) {
+        uint4 mk = *reinterpret_cast<const uint4*>(a.mask + off);
+        const bf16_t* pm = reinterpret_cast<const bf16_t*>(&mk);
+#pragma unroll
+        for (int q = 0; q < 8; ++q)
+          if (pm[q] == 0 || (pm[q] & 0x8000u)) pv[q] = 0;
+      }
+#pragma unroll
+      for (int q = 0; q < 8; ++q) gt[(part * 8 + q) * 64 + p] = pv[q];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {  // K steps: block rows 4s .. 4s + 3, one row (8 pixels) per lane group
+      int r = 4 * s + lk;
+#pragma unroll
+      for (int i = 0; i < 9; ++i) {
+        int t = wave + 4 * i;
+        if (t >= ntiles) break;
+        int tap = t / ntile, j = t % ntile;
+        int dy = tap / 3, dx = tap % 3;
+        uint4 aq = *reinterpret_cast<const uint4*>(xt + ((dx * kWgCin + lm) * 10 + r + dy) * 8);
+        uint4 bq = *reinterpret_cast<const uint4*>(gt + (j * 16 + lm) * 64 + r * 8);
+        acc[i] = mfma16(aq, bq, acc[i]);
+      }
+    }
+  }
+  // D[m = cin][n = cout] of tile (tap, j): lane holds cin 4 lk + q, cout lm
+#pragma unroll
+  for (int i = 0; i < 9; ++i) {
+    int t = wave + 4 * i;
+    if (t >= ntiles) break;
+    int tap = t / ntile, j = t % ntile;
+    int n = n0 + j * 16 + lm;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      int c = c0 + lk * 4 + q;
+      a.part[(((int64_t)chunk * a.cout + n) * 9 + tap) * a.cin + c] = acc[i][q];
+    }
+  }
+}
+
+// Sums the chunk partials: grad[n][tap][c] (FP32) and the bias gradient from g * mask.
+__global__ void wgrad_reduce_kernel(const float* part, int nchunk, int64_t count, float* grad) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  float s = 0.0f;
+  for (int k = 0; k < nchunk; ++k) s += part[(int64_t)k * count + i];
+  grad[i] = s;
+}
+
+// db[n] = sum_p g[p][n] * (mask[p][n] > 0): one workgroup per channel group of 64, a block of rows.
+__global__ void bias_grad_kernel(const bf16_t* g, const bf16_t* mask, int64_t npix, int cout, float* part, int rows_per_block) {
+  int n = blockIdx.y * 64 + (threadIdx.x & 63);
+  int sub = threadIdx.x >> 6;  // 4 row groups
+  int64_t p0 = (int64_t)blockIdx.x * rows_per_block;
+  float s = 0.0f;
+  if (n < cout) {
+    for (int64_t p = p0 + sub; p < min(npix, p0 + rows_per_block); p += 4) {
+      bf16_t gv = g[p * cout + n];
+      if (mask) {
+        bf16_t mv = mask[p * cout + n];
+        if (mv == 0 || (mv & 0x8000u)) continue;
+      }
+      s += bf2f(gv);
+    }
+  }
+  __shared__ float red[256];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  if (sub == 0 && n < cout) part[(int64_t)blockIdx.x * cout + n] = red[threadIdx.x] + red[threadIdx.x + 64] + red[threadIdx.x + 128] + red[threadIdx.x + 192];
+}
+
+// ------------------------------------------------------------------ elementwise layers
+// 2x2 average pooling, NHWC.
+__global__ void avgpool2_kernel(const bf16_t* x, bf16_t* y, int n, int h, int w, int c, int grad) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int ho = h / 2, wo = w / 2;
+  if (!grad) {
+    int64_t total = (int64_t)n * ho * wo * c;
+    if (i >= total) return;
+    int ch = i % c;
+    int64_t p = i / c;
+    int xo = p % wo, yo = (p / wo) % ho, img = p / ((int64_t)wo * ho);
+    float s = 0.0f;
+#pragma unroll
+    for (int dy = 0; dy < 2; ++dy)
+#pragma unroll
+      for (int dx = 0; dx < 2; ++dx) s += bf2f(x[(((int64_t)img * h + 2 * yo + dy) * w + 2 * xo + dx) * c + ch]);
+    y[i] = f2bf(0.25f * s);
+  } else {  // x: gradient at the pooled resolution, y: gradient at full resolution
+    int64_t total = (int64_t)n * h * w * c;
+    if (i >= total) return;
+    int ch = i % c;
+    int64_t p = i / c;
+    int xx = p % w, yy = (p / w) % h, img = p / ((int64_t)w * h);
+    y[i] = f2bf(0.25f * bf2f(x[(((int64_t)img * ho + yy / 2) * wo + xx / 2) * c + ch]));
+  }
+}
+
+// Decoder input: concat(upsample2x(low), skip) along channels; backward splits the gradient:
+// the low-resolution part sums the 2x2 gradients, the skip part adds to `skip_grad_add`
+// (which also receives the pooling path's gradient of the same tensor).
+__global__ void upcat_kernel(const bf16_t* low, const bf16_t* skip, bf16_t* out, int n, int h, int w, int cl, int cs) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int c = cl + cs;
+  int64_t total = (int64_t)n * h * w * c;
+  if (i >= total) return;
+  int ch = i % c;
+  int64_t p = i / c;
+  int xx = p % w, yy = (p / w) % h, img = p / ((int64_t)w * h);
+  if (ch < cl)
+    out[i] = low[(((int64_t)img * (h / 2) + yy / 2) * (w / 2) + xx / 2) * cl + ch];
+  else
+    out[i] = skip[p * cs + ch - cl];
+}
+
+__global__ void upcat_low_grad_kernel(const bf16_t* g, bf16_t* low_grad, int n, int h, int w, int cl, int cs) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int ho = h / 2, wo = w / 2;
+  int64_t total = (int64_t)n * ho * wo * cl;
+  if (i >= total) return;
+  int ch = i % cl;
+  int64_t p = i / cl;
+  int xo = p % wo, yo = (p / wo) % ho, img = p / ((int64_t)wo * ho);
+  float s = 0.0f;
+#pragma unroll
+  for (int dy = 0; dy < 2; ++dy)
+#pragma unroll
+    for (int dx = 0; dx < 2; ++dx) s += bf2f(g[(((int64_t)img * h + 2 * yo + dy) * w + 2 * xo + dx) * (cl + cs) + ch]);
+  low_grad[i] = f2bf(s);
+}
+
+// skip_grad = pool_backward(pool_grad) + concat slice of cat_grad
+__global__ void skip_grad_kernel(const bf16_t* pool_grad, const bf16_t* cat_grad, bf16_t* skip_grad, int n, int h, int w, int cs,
+                                 int cl) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int64_t total = (int64_t)n * h * w * cs;
+  if (i >= total) return;
+  int ch = i % cs;
+  int64_t p = i / cs;
+  int xx = p % w, yy = (p / w) % h, img = p / ((int64_t)w * h);
+  float v = 0.25f * bf2f(pool_grad[(((int64_t)img * (h / 2) + yy / 2) * (w / 2) + xx / 2) * cs + ch]);
+  v += bf2f(cat_grad[p * (cl + cs) + cl + ch]);
+  skip_grad[i] = f2bf(v);
+}
+
+// L1 loss of the prediction (FP32 network output, first 3 channels, plus the residual base: the
+// first 3 channels of the BF16 network input with `cb` channels) against FP32 targets: loss
+// partial sums and the BF16 gradient sign(pred - target) / count in a cp-channel tensor.
+__global__ void l1_kernel(const float* pred, const bf16_t* base, int cb, const float* target, int64_t npix, int cp, bf16_t* grad,
+                          float* loss_part) {
+  int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  float l = 0.0f;
+  if (p < npix) {
+    float inv = 1.0f / (float)(npix * 3);
+    for (int c = 0; c < cp; ++c) {
+      float gv = 0.0f;
+      if (c < 3) {
+        float d = pred[p * 3 + c] + (base ? bf2f(base[p * cb + c]) : 0.0f) - target[p * 3 + c];
+        l += fabsf(d);
+        gv = d > 0.0f ? inv : (d < 0.0f ? -inv : 0.0f);
+      }
+      grad[p * cp + c] = f2bf(gv);
+    }
+  }
+  __shared__ float red[256];
+  red[threadIdx.x] = l;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) loss_part[blockIdx.x] = red[0];
+}
+
+// Adam on FP32 master weights [cout][9][cin] (+ bias), refreshing both BF16 layouts.
+__global__ void adam_kernel(float* w, float* m, float* v, const float* g, int64_t count, float lr, float b1, float b2, float eps,
+                            float c1, float c2, bf16_t* wf, bf16_t* wd, int cin, int cout) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  float gi = g[i];
+  float mi = b1 * m[i] + (1.0f - b1) * gi;
+  float vi = b2 * v[i] + (1.0f - b2) * gi * gi;
+  m[i] = mi;
+  v[i] = vi;
+  float wi = w[i] - lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+  w[i] = wi;
+  if (wf) {
+    wf[i] = f2bf(wi);
+    int c = i % cin;
+    int tap = (i / cin) % 9;
+    int n = i / ((int64_t)cin * 9);
+    wd[((int64_t)c * 9 + (8 - tap)) * cout + n] = f2bf(wi);
+  }
+}
+
+// BF16 copies of FP32 master weights (forward and mirrored-transposed layouts).
+__global__ void weight_layouts_kernel(const float* w, int64_t count, int cin, int cout, bf16_t* wf, bf16_t* wd) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  float wi = w[i];
+  wf[i] = f2bf(wi);
+  int c = i % cin;
+  int tap = (i / cin) % 9;
+  int n = i / ((int64_t)cin * 9);
+  wd[((int64_t)c * 9 + (8 - tap)) * cout + n] = f2bf(wi);
+}
+
+// Network input: [log1p(hdr * exposure), albedo, normal, depth / (1 + depth), 0...] as BF16 with
+// `cp` channels, from the path tracer's per-world flat buffers of one camera.
+__global__ void pack_input_kernel(const float* hdr, int hdr_stride, const uint16_t* albedo, const uint16_t* normal, const float* depth,
+                                  int aov_stride, int adr, const float* exposure, int exposure_n, int n, int h, int w, int cp,
+                                  bf16_t* out) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int64_t total = (int64_t)n * h * w;
+  if (i >= total) return;
+  int img = i / ((int64_t)h * w);
+  int64_t pid = i - (int64_t)img * h * w;
+  const float* hp = hdr + 3 * ((int64_t)img * hdr_stride + adr + pid);
+  int64_t fa = (int64_t)img * aov_stride + adr + pid;
+  float e = exposure[img % exposure_n];
+  float vals[10];
+  for (int c = 0; c < 3; ++c) vals[c] = log1pf(fmaxf(hp[c] * e, 0.0f));
+  for (int c = 0; c < 3; ++c) vals[3 + c] = half_to_float(albedo[3 * fa + c]);
+  for (int c = 0; c < 3; ++c) vals[6 + c] = half_to_float(normal[3 * fa + c]);
+  float d = depth[fa];
+  vals[9] = d / (1.0f + d);
+  for (int c = 0; c < cp; ++c) out[i * cp + c] = f2bf(c < 10 ? vals[c] : 0.0f);
+}
+
+// Denoised HDR from the network's log-space FP32 output (+ residual base): expm1, / exposure; and
+// the display RGB.
+__global__ void unpack_output_kernel(const float* pred, const bf16_t* base, int cb, int n, int h, int w, const float* exposure,
+                                     int exposure_n, int tone_mapper, float* hdr, int hdr_stride, uint32_t* rgb, int rgb_stride,
+                                     int adr) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int64_t total = (int64_t)n * h * w;
+  if (i >= total) return;
+  int img = i / ((int64_t)h * w);
+  int64_t pid = i - (int64_t)img * h * w;
+  float e = exposure[img % exposure_n];
+  float lv[3];
+  for (int k = 0; k < 3; ++k) lv[k] = pred[3 * i + k] + (base ? bf2f(base[i * cb + k]) : 0.0f);
+  V3 c = V3{expm1f(fmaxf(lv[0], 0.0f)), expm1f(fmaxf(lv[1], 0.0f)), expm1f(fmaxf(lv[2], 0.0f))};
+  if (hdr) st3(hdr + 3 * ((int64_t)img * hdr_stride + adr + pid), c / e);
+  if (rgb) rgb[(int64_t)img * rgb_stride + adr + pid] = encode_display(display_transform(c, tone_mapper));
+}
+
+// MFMA operand layout probe: D = A (16 x 32) . B (32 x 16) through the same lane mapping as the
+// convolutions, for tests.
+__global__ void mfma_probe_kernel(const bf16_t* A, const bf16_t* B, float* D) {
+  int l = threadIdx.x;
+  int lm = l & 15, lk = l >> 4;
+  uint4 aq = *reinterpret_cast<const uint4*>(A + lm * 32 + lk * 8);
+  bf16_t bv[8];
+  for (int i = 0; i < 8; ++i) bv[i] = B[(lk * 8 + i) * 16 + lm];
+  uint4 bq = *reinterpret_cast<const uint4*>(bv);
+  floatx4 acc = mfma16(aq, bq, floatx4{0.f, 0.f, 0.f, 0.f});
+  for (int q = 0; q < 4; ++q) D[(lk * 4 + q) * 16 + lm] = acc[q];
+}
+
+}  // namespace mjpbr
+
+using namespace mjpbr;
+
+namespace {
+inline unsigned nblk(int64_t n, int t) { return (unsigned)((n + t - 1) / t); }
+}  // namespace
+
+extern "C" int mjpbr_unet_conv3x3(const uint16_t* x, const uint16_t* mask, const uint16_t* w, const float* bias, uint16_t* y,
+                                  float* y32, int out_f32, int n, int h, int wd, int cin, int cout, int relu, void* stream) {
+  if (h % kConvBlock || wd % kConvBlock || cin % kCinChunk || cout % 16 || n < 1)
+    return set_error(MJPBR_EINVAL, "unet_conv3x3: need h, w multiples of 16, cin of 32, cout of 16 (got %dx%d, %d -> %d)", h, wd,
+                     cin, cout);
+  ConvArgs a{x, mask, w, bias, y, y32, n, h, wd, cin, cout, relu, out_f32};
+  dim3 grid((unsigned)(n * (h / kConvBlock) * (wd / kConvBlock)), (unsigned)((cout + kNTile - 1) / kNTile));
+  hipLaunchKernelGGL(conv3x3_kernel, grid, dim3(256), 0, (hipStream_t)stream, a);
+  return check_launch("conv3x3_kernel");
+}
+
+extern "C" int64_t mjpbr_unet_wgrad_workspace(int n, int h, int wd, int cin, int cout, int nchunk) {
+  (void)n; (void)h; (void)wd;
+  return (int64_t)nchunk * cout * 9 * cin;
+}
+
+extern "C" int mjpbr_unet_conv3x3_wgrad(const uint16_t* x, const uint16_t* g, const uint16_t* mask, float* part, int nchunk,
+                                        float* grad, float* bias_part, float* bias_grad, int n, int h, int wd, int cin, int cout,
+                                        void* stream) {
+  if (h % kWgBlock || wd % kWgBlock || cin % kWgCin || cout % 16 || nchunk < 1)
+    return set_error(MJPBR_EINVAL, "unet_conv3x3_wgrad: bad shape %dx%d, %d -> %d", h, wd, cin, cout);
+  int nblocks = n * (h / kWgBlock) * (wd / kWgBlock);
+  int per = (nblocks + nchunk - 1) / nchunk;
+  WgradArgs a{x, g, mask, part, n, h, wd, cin, cout, per};
+  dim3 grid((unsigned)(cin / kWgCin), (unsigned)((cout + kNTile - 1) / kNTile), (unsigned)nchunk);
+  hipStream_t st = (hipStream_t)stream;
+  hipLaunchKernelGGL(conv3x3_wgrad_kernel, grid, dim3(256), 0, st, a);
+  int rc = check_launch("conv3x3_wgrad_kernel");
+  if (rc) return rc;
+  int64_t count = (int64_t)cout * 9 * cin;
+  hipLaunchKernelGGL(wgrad_reduce_kernel, dim3(nblk(count, 256)), dim3(256), 0, st, part, nchunk, count, grad);
+  rc = check_launch("wgrad_reduce_kernel");
+  if (rc || !bias_grad) return rc;
+  int64_t npix = (int64_t)n * h * wd;
+  int rows = 4096;
+  int nb = (int)((npix + rows - 1) / rows);
+  hipLaunchKernelGGL(bias_grad_kernel, dim3((unsigned)nb, (unsigned)((cout + 63) / 64)), dim3(256), 0, st, g, mask, npix, cout,
+                     bias_part, rows);
+  rc = check_launch("bias_grad_kernel");
+  if (rc) return rc;
+  // reduce nb partials of cout values each
+  hipLaunchKernelGGL(wgrad_reduce_kernel, dim3(nblk(cout, 256)), dim3(256), 0, st, bias_part, nb, (int64_t)cout, bias_grad);
+  return check_launch("wgrad_reduce_kernel");
+}
+
+extern "C" int64_t mjpbr_unet_bias_workspace(int n, int h, int wd, int cout) {
+  int64_t npix = (int64_t)n * h * wd;
+  return ((npix + 4095) / 4096) * cout;
+}
+
+extern "C" int mjpbr_unet_avgpool2(const uint16_t* x, uint16_t* y, int n, int h, int wd, int c, int grad, void* stream) {
+  int64_t total = grad ? (int64_t)n * h * wd * c : (int64_t)n * (h / 2) * (wd / 2) * c;
+  hipLaunchKernelGGL(avgpool2_kernel, dim3(nblk(total, 256)), dim3(256), 0, (hipStream_t)stream, x, y, n, h, wd, c, grad);
+  return check_launch("avgpool2_kernel");
+}
+
+extern "C" int mjpbr_unet_upcat(const uint16_t* low, const uint16_t* skip, uint16_t* out, int n, int h, int wd, int cl, int cs,
+                                void* stream) {
+  int64_t total = (int64_t)n * h * wd * (cl + cs);
+  hipLaunchKernelGGL(upcat_kernel, dim3(nblk(total, 256)), dim3(256), 0, (hipStream_t)stream, low, skip, out, n, h, wd, cl, cs);
+  return check_launch("upcat_kernel");
+}
+
+extern "C" int mjpbr_unet_upcat_low_grad(const uint16_t* g, uint16_t* low_grad, int n, int h, int wd, int cl, int cs,
+                                         void* stream) {
+  int64_t total = (int64_t)n * (h / 2) * (wd / 2) * cl;
+  hipLaunchKernelGGL(upcat_low_grad_kernel, dim3(nblk(total, 256)), dim3(256), 0, (hipStream_t)stream, g, low_grad, n, h, wd, cl, cs);
+  return check_launch("upcat_low_grad_kernel");
+}
+
+extern "C" int mjpbr_unet_skip_grad(const uint16_t* pool_grad, const uint16_t* cat_grad, uint16_t* skip_grad, int n, int h, int wd,
+                                    int cs, int cl, void* stream) {
+  int64_t total = (int64_t)n * h * wd * cs;
+  hipLaunchKernelGGL(skip_grad_kernel, dim3(nblk(total, 256)), dim3(256), 0, (hipStream_t)stream, pool_grad, cat_grad, skip_grad, n,
+                     h, wd, cs, cl);
+  return check_launch("skip_grad_kernel");
+}
+
+extern "C" int mjpbr_unet_l1(const float* pred, const uint16_t* base, int cb, const float* target, int64_t npix, int cp,
+                             uint16_t* grad, float* loss_part, void* stream) {
+  hipLaunchKernelGGL(l1_kernel, dim3(nblk(npix, 256)), dim3(256), 0, (hipStream_t)stream, pred, base, cb, target, npix, cp, grad,
+                     loss_part);
+  return check_launch("l1_kernel");
+}
+
+extern "C" int mjpbr_unet_adam(float* w, float* m, float* v, const float* g, int64_t count, float lr, float b1, float b2, float eps,
+                               int step, uint16_t* wf, uint16_t* wd, int cin, int cout, void* stream) {
+  float c1 = 1.0f - powf(b1, (float)step), c2 = 1.0f - powf(b2, (float)step);
+  hipLaunchKernelGGL(adam_kernel, dim3(nblk(count, 256)), dim3(256), 0, (hipStream_t)stream, w, m, v, g, count, lr, b1, b2, eps, c1,
+                     c2, wf, wd, cin, cout);
+  return check_launch("adam_kernel");
+}
+
+extern "C" int mjpbr_unet_weight_layouts(const float* w, int cin, int cout, uint16_t* wf, uint16_t* wd, void* stream) {
+  int64_t count = (int64_t)cout * 9 * cin;
+  hipLaunchKernelGGL(weight_layouts_kernel, dim3(nblk(count, 256)), dim3(256), 0, (hipStream_t)stream, w, count, cin, cout, wf, wd);
+  return check_launch("weight_layouts_kernel");
+}
+
+extern "C" int mjpbr_unet_pack_input(const float* hdr, int hdr_stride, const uint16_t* albedo, const uint16_t* normal,
+                                     const float* depth, int aov_stride, int adr, const float* exposure, int exposure_n, int n, int h,
+                                     int wd, int cp, uint16_t* out, void* stream) {
+  int64_t total = (int64_t)n * h * wd;
+  hipLaunchKernelGGL(pack_input_kernel, dim3(nblk(total, 256)), dim3(256), 0, (hipStream_t)stream, hdr, hdr_stride, albedo, normal,
+                     depth, aov_stride, adr, exposure, exposure_n, n, h, wd, cp, out);
+  return check_launch("pack_input_kernel");
+}
+
+extern "C" int mjpbr_unet_unpack_output(const float* pred, const uint16_t* base, int cb, int n, int h, int wd,
+                                        const float* exposure, int exposure_n, int tone_mapper, float* hdr, int hdr_stride,
+                                        uint32_t* rgb, int rgb_stride, int adr, void* stream) {
+  int64_t total = (int64_t)n * h * wd;
+  hipLaunchKernelGGL(unpack_output_kernel, dim3(nblk(total, 256)), dim3(256), 0, (hipStream_t)stream, pred, base, cb, n, h, wd, exposure,
+                     exposure_n, tone_mapper, hdr, hdr_stride, rgb, rgb_stride, adr);
+  return check_launch("unpack_output_kernel");
+}
+
+extern "C" int mjpbr_unet_mfma_probe(const uint16_t* A, const uint16_t* B, float* D, void* stream) {
+  hipLaunchKernelGGL(mfma_probe_kernel, dim3(1), dim3(64), 0, (hipStream_t)stream, A, B, D);
+  return check_launch("mfma_probe_kernel");
+}
