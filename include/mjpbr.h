@@ -266,6 +266,9 @@ typedef struct MjpbrAtrousParams {
   const float* exposure;
   int32_t exposure_n;
   int32_t pad;
+  /* NULL: filter every world; else only worlds w with world_select[w] != 0 (the others are left
+   * untouched: the learned denoiser writes them). No counterpart in the reference. */
+  const int32_t* world_select;
 } MjpbrAtrousParams;
 
 /* ---------------------------------------------------------------------------------------- */
@@ -361,10 +364,18 @@ int mjpbr_unet_pack_input(const float* hdr, int hdr_stride, const uint16_t* albe
                           int aov_stride, int adr, const float* exposure, int exposure_n, int n, int h, int wd, int cp,
                           uint16_t* out, void* stream);
 /* Denoised HDR (expm1(max(pred + base, 0)) / exposure) and display RGB of one camera from the
- * network output; hdr or rgb may be NULL. */
+ * network output; hdr or rgb may be NULL; images with skip[img] != 0 (skip may be NULL) are not
+ * written. */
 int mjpbr_unet_unpack_output(const float* pred, const uint16_t* base, int cb, int n, int h, int wd, const float* exposure,
                              int exposure_n, int tone_mapper, float* hdr, int hdr_stride, uint32_t* rgb, int rgb_stride, int adr,
-                             void* stream);
+                             const int32_t* skip, void* stream);
+/* Per-image guide statistics of a packed input x [n][h][w][cp]: stats [n][4] = {mean log
+ * radiance, background fraction, mean depth feature over hits, mean albedo over hits}. With
+ * range = {lo[4], hi[4]} (device memory, may be NULL), fallback[img] = 1 when a statistic lies
+ * outside the range the network was trained on (ORed into fallback when accumulate != 0): the
+ * caller then filters that image with the a-trous denoiser instead. */
+int mjpbr_unet_guide_stats(const uint16_t* x, int n, int h, int wd, int cp, const float* range, float* stats, int32_t* fallback,
+                           int accumulate, void* stream);
 /* D [16][16] FP32 = A [16][32] . B [32][16] (BF16) through the convolutions' matrix-core operand
  * layout: a self-test of the lane mapping. */
 int mjpbr_unet_mfma_probe(const uint16_t* A, const uint16_t* B, float* D, void* stream);

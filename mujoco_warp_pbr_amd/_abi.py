@@ -64,7 +64,8 @@ FUNCTIONS.update(
     "mjpbr_unet_adam": (_I, [_VP, _VP, _VP, _VP, _L, _F, _F, _F, _F, _I, _VP, _VP, _I, _I, _VP]),
     "mjpbr_unet_weight_layouts": (_I, [_VP, _I, _I, _VP, _VP, _VP]),
     "mjpbr_unet_pack_input": (_I, [_VP, _I, _VP, _VP, _VP, _I, _I, _VP, _I, _I, _I, _I, _I, _VP, _VP]),
-    "mjpbr_unet_unpack_output": (_I, [_VP, _VP, _I, _I, _I, _I, _VP, _I, _I, _VP, _I, _VP, _I, _I, _VP]),
+    "mjpbr_unet_unpack_output": (_I, [_VP, _VP, _I, _I, _I, _I, _VP, _I, _I, _VP, _I, _VP, _I, _I, _VP, _VP]),
+    "mjpbr_unet_guide_stats": (_I, [_VP, _I, _I, _I, _I, _VP, _VP, _VP, _I, _VP]),
     "mjpbr_unet_mfma_probe": (_I, [_VP, _VP, _VP, _VP]),
   }
 )

@@ -51,6 +51,7 @@ __global__ void demodulate_kernel(MjpbrAtrousParams p, MjpbrOutputs out, float4*
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= (int64_t)p.nworld * p.stride) return;
   int w = (int)(i / p.stride), index = (int)(i - (int64_t)w * p.stride);
+  if (p.world_select && !p.world_select[w]) return;
   int64_t a = (int64_t)w * out.aov_stride + index;
   V3 albedo = ld_half3(out.aov_albedo + 3 * a);
   V3 c = ld3(out.hdr + 3 * ((int64_t)w * out.hdr_stride + index)) * p.exposure[w % p.exposure_n] - ld3(out.aov_emission + 3 * a);
@@ -68,6 +69,7 @@ __global__ void atrous_kernel(MjpbrAtrousParams p, MjpbrOutputs out, int step, c
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= (int64_t)p.nworld * p.stride) return;
   int w = (int)(i / p.stride), index = (int)(i - (int64_t)w * p.stride);
+  if (p.world_select && !p.world_select[w]) return;
   Locate loc = locate(p, index);
   if (loc.adr < 0) return;
   const float4* srcw = src + (int64_t)w * p.stride;

@@ -214,8 +214,10 @@ MJ_DEV V3 trace_pixel(const RtCtx& c, V3 origin, V3 dir, float max_dist, float f
   return radiance;
 }
 
-__global__ void __launch_bounds__(256) rt_kernel(MjpbrScene sc, MjpbrSettings s, MjpbrOutputs out, int seed,
-                                                 TileMap tm) {
+// Occupancy over registers: the kernel is latency bound (dependent BVH and texture loads), and 8
+// waves per SIMD with spills to scratch beat 2 waves without (measured 1.9x at 1024 x 128^2).
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8)))
+rt_kernel(MjpbrScene sc, MjpbrSettings s, MjpbrOutputs out, int seed, TileMap tm) {
   extern __shared__ float lds[];
   int b = logical_block(tm);
   if (b >= tm.nblocks) return;

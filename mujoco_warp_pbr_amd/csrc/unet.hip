@@ -433,15 +433,64 @@ __global__ void pack_input_kernel(const float* hdr, int hdr_stride, const uint16
   for (int c = 0; c < cp; ++c) out[i * cp + c] = f2bf(c < 10 ? vals[c] : 0.0f);
 }
 
+// Per-image statistics of the packed input and the fallback decision. One workgroup per image:
+// stats[img] = {mean log radiance (channels 0-2), fraction of background pixels (zero normal),
+// mean depth feature over hits, mean albedo (channels 3-5) over hits}; with `range` ({lo[4],
+// hi[4]}), fallback[img] = 1 when a statistic lies outside [lo, hi] (ORed into the existing value
+// when `accumulate`).
+constexpr int kStats = 4;
+
+__global__ void __launch_bounds__(256) guide_stats_kernel(const bf16_t* x, int64_t npix, int cp, const float* range, float* stats,
+                                                          int32_t* fallback, int accumulate) {
+  const int img = blockIdx.x;
+  const bf16_t* xi = x + (int64_t)img * npix * cp;
+  float rad = 0.0f, bg = 0.0f, dep = 0.0f, alb = 0.0f;
+  for (int64_t p = threadIdx.x; p < npix; p += 256) {
+    uint4 v = *reinterpret_cast<const uint4*>(xi + p * cp);  // channels 0-7
+    uint32_t v89 = *reinterpret_cast<const uint32_t*>(xi + p * cp + 8);
+    const bf16_t* c = reinterpret_cast<const bf16_t*>(&v);
+    rad += bf2f(c[0]) + bf2f(c[1]) + bf2f(c[2]);
+    bool hit = ((c[6] | c[7] | (bf16_t)(v89 & 0xffffu)) & 0x7fffu) != 0;
+    if (hit) {
+      dep += bf2f((bf16_t)(v89 >> 16));
+      alb += bf2f(c[3]) + bf2f(c[4]) + bf2f(c[5]);
+    } else {
+      bg += 1.0f;
+    }
+  }
+  __shared__ float red[4][256];
+  red[0][threadIdx.x] = rad;
+  red[1][threadIdx.x] = bg;
+  red[2][threadIdx.x] = dep;
+  red[3][threadIdx.x] = alb;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (threadIdx.x < s)
+      for (int k = 0; k < 4; ++k) red[k][threadIdx.x] += red[k][threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    float hits = fmaxf((float)npix - red[1][0], 1.0f);
+    float st[kStats] = {red[0][0] / (3.0f * (float)npix), red[1][0] / (float)npix, red[2][0] / hits, red[3][0] / (3.0f * hits)};
+    for (int k = 0; k < kStats; ++k) stats[img * kStats + k] = st[k];
+    if (range && fallback) {
+      int out = 0;
+      for (int k = 0; k < kStats; ++k) out |= (st[k] < range[k] || st[k] > range[kStats + k]) ? 1 : 0;
+      fallback[img] = accumulate ? (fallback[img] | out) : out;
+    }
+  }
+}
+
 // Denoised HDR from the network's log-space FP32 output (+ residual base): expm1, / exposure; and
-// the display RGB.
+// the display RGB. Images with skip[img] != 0 are left untouched.
 __global__ void unpack_output_kernel(const float* pred, const bf16_t* base, int cb, int n, int h, int w, const float* exposure,
                                      int exposure_n, int tone_mapper, float* hdr, int hdr_stride, uint32_t* rgb, int rgb_stride,
-                                     int adr) {
+                                     int adr, const int32_t* skip) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   int64_t total = (int64_t)n * h * w;
   if (i >= total) return;
   int img = i / ((int64_t)h * w);
+  if (skip && skip[img]) return;
   int64_t pid = i - (int64_t)img * h * w;
   float e = exposure[img % exposure_n];
   float lv[3];
@@ -582,11 +631,19 @@ extern "C" int mjpbr_unet_pack_input(const float* hdr, int hdr_stride, const uin
 
 extern "C" int mjpbr_unet_unpack_output(const float* pred, const uint16_t* base, int cb, int n, int h, int wd,
                                         const float* exposure, int exposure_n, int tone_mapper, float* hdr, int hdr_stride,
-                                        uint32_t* rgb, int rgb_stride, int adr, void* stream) {
+                                        uint32_t* rgb, int rgb_stride, int adr, const int32_t* skip, void* stream) {
   int64_t total = (int64_t)n * h * wd;
   hipLaunchKernelGGL(unpack_output_kernel, dim3(nblk(total, 256)), dim3(256), 0, (hipStream_t)stream, pred, base, cb, n, h, wd, exposure,
-                     exposure_n, tone_mapper, hdr, hdr_stride, rgb, rgb_stride, adr);
+                     exposure_n, tone_mapper, hdr, hdr_stride, rgb, rgb_stride, adr, skip);
   return check_launch("unpack_output_kernel");
+}
+
+extern "C" int mjpbr_unet_guide_stats(const uint16_t* x, int n, int h, int wd, int cp, const float* range, float* stats,
+                                      int32_t* fallback, int accumulate, void* stream) {
+  if (cp < 10 || cp % 8 || n < 1) return set_error(MJPBR_EINVAL, "unet_guide_stats: need cp >= 10 and a multiple of 8 (got %d)", cp);
+  hipLaunchKernelGGL(guide_stats_kernel, dim3((unsigned)n), dim3(256), 0, (hipStream_t)stream, x, (int64_t)h * wd, cp, range, stats,
+                     fallback, accumulate);
+  return check_launch("guide_stats_kernel");
 }
 
 extern "C" int mjpbr_unet_mfma_probe(const uint16_t* A, const uint16_t* B, float* D, void* stream) {

@@ -131,6 +131,8 @@ def create_render_context(
   camera_rays=None,
   device="cuda",
   precompute: bool = True,
+  denoiser: str = "atrous",
+  denoiser_weights: str | None = None,
 ) -> PbrRenderContext:
   """Creates a PBR render context on `device` (reference io.py:331-634; same arguments).
 
@@ -140,6 +142,11 @@ def create_render_context(
   accepted for compatibility: this renderer always traverses nearest-first over its own BVHs.
   `traversal="optix"` is not available (no OptiX on this hardware) and falls back to
   "software" with a warning, as the reference does without OptiX.
+
+  `denoiser` ("atrous" or "learned") picks the filter `denoise=True` applies: the reference's
+  a-trous filter, or the learned U-Net (weights `denoiser_weights`, default the packaged ones)
+  with the a-trous filter as fallback for worlds outside its training range. The learned
+  denoiser needs camera sides that are multiples of 64.
 
   `precompute=False` leaves the GPU-built tables (DFG LUT, prefiltered environment stacks)
   zero; it exists so host-side test harnesses can create contexts on device="cpu" and fill the
@@ -174,6 +181,8 @@ def create_render_context(
   _check_default_material(default_material)
   if denoise and tier != Tier.PATH:
     raise ValueError("denoise applies to the path-traced tier (Tier.PATH)")
+  if denoiser not in ("atrous", "learned"):
+    raise ValueError(f'denoiser must be "atrous" or "learned", got {denoiser!r}')
   if mesh_bvh_frame not in _MESH_BVH_FRAMES:
     raise ValueError(f'mesh_bvh_frame must be "auto" or "mujoco", got {mesh_bvh_frame!r}')
   if mesh_bvh_constructor not in _MESH_BVH_CONSTRUCTORS:
@@ -224,6 +233,8 @@ def create_render_context(
   for r in res:
     if r[0] < 1 or r[1] < 1:
       raise ValueError(f"camera resolutions must be positive, got {r}")
+    if denoise and denoiser == "learned" and (r[0] % 64 or r[1] % 64):
+      raise ValueError(f"the learned denoiser needs camera sides that are multiples of 64, got {r}")
   want_rgb = _per_camera(render_rgb, nrender, "render_rgb", True)
   want_depth = _per_camera(render_depth, nrender, "render_depth", True)
   want_seg = _per_camera(render_seg, nrender, "render_seg", True)
@@ -376,6 +387,8 @@ def create_render_context(
     guide_albedo=zeros((nworld, npixel if render_guides else 1, 3)),
     guide_normal=zeros((nworld, npixel if render_guides else 1, 3)),
     traversal=traversal,
+    denoiser=denoiser,
+    denoiser_weights=denoiser_weights,
   )
   return rc
 

@@ -273,6 +273,18 @@ def check_env_id(rc: PbrRenderContext) -> None:
   rc._host_cache["env_id"] = env_id
 
 
+def render_path_samples(m, d, rc: PbrRenderContext, seed: int = 0) -> None:
+  """The path tracer alone: noisy HDR (and, with `rc.denoise`, the first-hit AOVs) into the
+  context's buffers, no filter. Training-data generation for the learned denoiser uses it."""
+  if rc.tier != Tier.PATH:
+    raise ValueError("render_path_samples needs a path-traced context (Tier.PATH)")
+  check_env_id(rc)
+  sc = scene_struct(m, d, rc)
+  s = settings_struct(rc)
+  out = outputs_struct(rc)
+  _abi.call("mjpbr_pt_render", _abi.byref(sc), _abi.byref(s), _abi.byref(out), ctypes.c_int32(int(seed)), _stream_of(d.geom_xpos))
+
+
 def render(m, d, rc: PbrRenderContext, seed: int = 0) -> None:
   """Renders the current frame into the context's output buffers (reference render.py:430-460).
 
@@ -288,8 +300,13 @@ def render(m, d, rc: PbrRenderContext, seed: int = 0) -> None:
   if rc.tier == Tier.PATH:
     _abi.call("mjpbr_pt_render", _abi.byref(sc), _abi.byref(s), _abi.byref(out), ctypes.c_int32(int(seed)), stream)
     if rc.denoise:
-      from mujoco_warp_pbr_amd import denoise
+      if rc.denoiser == "learned":
+        from mujoco_warp_pbr_amd import learned
 
-      denoise.denoise(rc, d.nworld, out, stream)
+        learned.denoise(rc, d.nworld, out, stream)
+      else:
+        from mujoco_warp_pbr_amd import denoise
+
+        denoise.denoise(rc, d.nworld, out, stream)
     return
   _abi.call("mjpbr_rt_render", _abi.byref(sc), _abi.byref(s), _abi.byref(out), ctypes.c_int32(int(seed)), stream)

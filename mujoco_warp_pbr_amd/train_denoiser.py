@@ -1,0 +1,313 @@
+"""Training and evaluation of the learned denoiser on pairs the path tracer itself renders.
+
+Data: randomized worlds of the full-feature scene generator (mirror plane, 12 primitives, 4
+convex-hull meshes, procedural textures, procedural HDR sky), each scene seed with its own
+geometry, sky and textures, and per world random materials, geom positions, camera pose, sky
+rotation / intensity and light intensities. Input: one path-traced sample per pixel with its
+first-hit albedo, normal and depth; target: a high-sample render of the same world, in log space.
+Training runs the U-Net's own matrix-core forward and backward kernels with Adam (L1 loss).
+
+Evaluation compares, on held-out scene seeds, the display image (sRGB8) of
+  - 1 sample + learned denoiser, and
+  - k samples + a-trous (5 iterations), for the k whose GPU time first reaches the learned
+    pipeline's,
+against a high-sample reference render (PSNR over RGB, 8-bit values / 255).
+
+  python -m mujoco_warp_pbr_amd.train_denoiser --scenes 32 --worlds 256 --steps 4000
+
+prints one progress line per phase and a JSON summary, and writes the weights (FP32 master copy,
+guide-statistics range, training and evaluation record) as safetensors.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import sys
+import time
+
+import numpy as np
+import torch
+
+from mujoco_warp_pbr_amd import _abi
+from mujoco_warp_pbr_amd import learned
+from mujoco_warp_pbr_amd import scenes
+
+EV100_UNIT = math.log2(1.0 / 1.2)  # exposure 1
+
+
+def log(msg: str) -> None:
+  print(f"[train] {msg}", file=sys.stderr, flush=True)
+
+
+# ----------------------------------------------------------------------------- scenes
+
+
+def scene_context(scene_seed: int, nworld: int, res: int, device, max_depth: int = 4, rr_depth: int = 3):
+  """(host model, model, data, context) of `nworld` randomized worlds of one scene seed, path
+  traced with first-hit AOVs (denoise=True)."""
+  import mujoco_warp_pbr_amd as mj
+
+  mjm = scenes.full_scene(seed=scene_seed, res=res)
+  m = mj.put_model(mjm, device)
+  d = mj.put_data(mjm, nworld, device)
+  rc = mj.create_render_context(
+    mjm,
+    nworld=nworld,
+    cam_res=(res, res),
+    envmap=scenes.procedural_sky(scene_seed, 256, 512),
+    tier=mj.Tier.PATH,
+    path_samples=1,
+    max_depth=max_depth,
+    rr_depth=rr_depth,
+    denoise=True,
+    denoise_iterations=5,
+    ev100=EV100_UNIT,
+    device=device,
+  )
+  randomize(m, d, rc, nworld, seed=1000003 * (scene_seed + 1))
+  mj.refit_bvh(m, d, rc, rebuild=True)
+  return mjm, m, d, rc
+
+
+def randomize(m, d, rc, nworld: int, seed: int) -> None:
+  """Per-world materials, geom positions (xy offsets of the non-plane geoms), camera pose, sky
+  rotation and intensity, light intensities."""
+  rng = np.random.default_rng(seed)
+  scenes.randomize_worlds(m, rc, nworld, seed=seed)
+  scenes.jitter_cameras(d, seed=seed + 1, offset=0.25, angle_deg=8.0)
+  dev = d.geom_xpos.device
+  xpos = d.geom_xpos.cpu().numpy()
+  xpos[:, 1:, :2] += rng.uniform(-0.35, 0.35, (nworld, xpos.shape[1] - 1, 2)).astype(np.float32)
+  d.geom_xpos = torch.as_tensor(xpos, device=dev)
+  rc.env_yaw = torch.as_tensor(rng.uniform(0.0, 2.0 * math.pi, nworld).astype(np.float32), device=dev)
+  rc.env_intensity = torch.as_tensor(np.exp(rng.normal(0.0, 0.6, nworld)).astype(np.float32), device=dev)
+  li = rc.light_intensity.cpu().numpy()
+  li = np.broadcast_to(li[:1], (nworld, li.shape[1])) * np.exp(rng.normal(0.0, 0.6, (nworld, li.shape[1])))
+  rc.light_intensity = torch.as_tensor(li.astype(np.float32), device=dev)
+
+
+def render_pair(m, d, rc, nworld: int, target_spp: int, seed: int):
+  """(input x BF16 (nworld, h, w, 32), log-space target (nworld * h * w, 3)) of one context."""
+  from mujoco_warp_pbr_amd.render import render_path_samples
+
+  rc.path_samples = target_spp
+  render_path_samples(m, d, rc, seed=seed + 1)
+  target = learned.log_target(rc, nworld)
+  rc.path_samples = 1
+  render_path_samples(m, d, rc, seed=seed)
+  x = learned.pack_input(rc, 0, nworld)
+  return x, target
+
+
+def generate(scene_seeds, nworld: int, res: int, target_spp: int, device):
+  """Stacked training pairs of several scene seeds (resident in HBM)."""
+  xs, ts = [], []
+  for k, sd in enumerate(scene_seeds):
+    t0 = time.perf_counter()
+    _, m, d, rc = scene_context(sd, nworld, res, device)
+    x, t = render_pair(m, d, rc, nworld, target_spp, seed=17 * sd + 5)
+    torch.cuda.synchronize()
+    xs.append(x.clone())
+    ts.append(t.view(nworld, res * res, 3).clone())
+    del rc, m, d
+    log(f"scene {k + 1}/{len(scene_seeds)} (seed {sd}): {nworld} worlds at {target_spp} spp in {time.perf_counter() - t0:.1f} s")
+  return torch.cat(xs), torch.cat(ts)
+
+
+def guide_stats(x: torch.Tensor) -> np.ndarray:
+  """Per-image guide statistics (n, 4) of packed inputs."""
+  n, h, w, c = x.shape
+  stats = torch.zeros((n, learned.NSTATS), dtype=torch.float32, device=x.device)
+  _abi.call("mjpbr_unet_guide_stats", _abi.ptr(x), n, h, w, c, 0, _abi.ptr(stats), 0, 0, torch.cuda.current_stream().cuda_stream)
+  return stats.cpu().numpy()
+
+
+# ----------------------------------------------------------------------------- training
+
+
+def train(net: learned.UNet, X: torch.Tensor, T: torch.Tensor, steps: int, batch: int, lr: float, seed: int = 0, log_every: int = 200):
+  """Adam on random batches (with random horizontal flips) of the stacked pairs; cosine
+  learning-rate decay to 5% of `lr`. Returns the loss history (every log_every steps)."""
+  n, h, w, _ = X.shape
+  gen = torch.Generator(device=X.device).manual_seed(seed)
+  T4 = T.view(n, h, w, 3)
+  history = []
+  t0 = time.perf_counter()
+  running = None
+  for step in range(steps):
+    idx = torch.randint(0, n, (batch,), device=X.device, generator=gen)
+    xb, tb = X[idx], T4[idx]
+    if step % 2 == 1:
+      xb, tb = torch.flip(xb, dims=(2,)), torch.flip(tb, dims=(2,))
+    xb = xb.contiguous()
+    tb = tb.reshape(-1, 3).contiguous()
+    net.forward(xb)
+    loss = net.backward(tb)
+    frac = step / max(steps - 1, 1)
+    net.adam_step(lr=lr * (0.05 + 0.95 * 0.5 * (1.0 + math.cos(math.pi * frac))))
+    running = loss if running is None else running + loss
+    if (step + 1) % log_every == 0 or step + 1 == steps:
+      value = float(running) / ((step % log_every) + 1)
+      running = None
+      history.append((step + 1, value))
+      log(f"step {step + 1}/{steps} loss {value:.5f} ({time.perf_counter() - t0:.1f} s)")
+  return history
+
+
+# ----------------------------------------------------------------------------- evaluation
+
+
+def psnr(a: torch.Tensor, b: torch.Tensor) -> float:
+  """PSNR in dB of two packed sRGB8 images (int32 bits R | G << 8 | B << 16), over RGB."""
+  def rgb(t):
+    t = t.to(torch.int64)
+    return torch.stack([(t >> s) & 255 for s in (0, 8, 16)], -1).to(torch.float64) / 255.0
+
+  mse = torch.mean((rgb(a) - rgb(b)) ** 2).item()
+  return 10.0 * math.log10(1.0 / max(mse, 1e-12))
+
+
+def _time_render(m, d, rc, seed: int, reps: int = 3) -> float:
+  import mujoco_warp_pbr_amd as mj
+
+  mj.render(m, d, rc, seed=seed)  # warm (first learned call loads the network)
+  st = torch.cuda.current_stream()
+  a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+  torch.cuda.synchronize()
+  a.record(st)
+  for r in range(reps):
+    mj.render(m, d, rc, seed=seed)
+  b.record(st)
+  torch.cuda.synchronize()
+  return a.elapsed_time(b) / reps
+
+
+def evaluate(scene_seeds, nworld: int, res: int, ref_spp: int, device, weights=None, atrous_spp=(1, 2, 3, 4, 6, 8)) -> dict:
+  """PSNR of the learned denoiser and of a-trous at several sample counts against a high-sample
+  reference, with the GPU time of each pipeline (path tracing + filter)."""
+  import mujoco_warp_pbr_amd as mj
+
+  rows = []
+  for sd in scene_seeds:
+    _, m, d, rc = scene_context(sd, nworld, res, device)
+    rc.denoiser_weights = str(weights) if weights else None
+    # reference: many samples, no filter
+    rc.denoise = False
+    rc.path_samples = ref_spp
+    mj.render(m, d, rc, seed=99991)
+    ref = rc.base.rgb_data.clone()
+    row = {"scene_seed": sd}
+    rc.path_samples = 1
+    mj.render(m, d, rc, seed=7)
+    row["noisy_1spp"] = psnr(rc.base.rgb_data, ref)
+    rc.denoise = True
+    rc.denoiser = "learned"
+    rc.path_samples = 1
+    row["learned_ms"] = _time_render(m, d, rc, seed=7)
+    mj.render(m, d, rc, seed=7)
+    row["learned"] = psnr(rc.base.rgb_data, ref)
+    fallback = learned.denoiser_for(rc).fallback
+    row["fallback_worlds"] = int(fallback.sum().item())
+    rc.denoiser = "atrous"
+    row["atrous"] = {}
+    for k in atrous_spp:
+      rc.path_samples = k
+      ms = _time_render(m, d, rc, seed=7)
+      mj.render(m, d, rc, seed=7)
+      row["atrous"][k] = {"ms": ms, "psnr": psnr(rc.base.rgb_data, ref)}
+    rows.append(row)
+    log(f"eval seed {sd}: noisy {row['noisy_1spp']:.2f} dB, learned {row['learned']:.2f} dB ({row['learned_ms']:.1f} ms), "
+        + ", ".join(f"atrous@{k} {v['psnr']:.2f} dB ({v['ms']:.1f} ms)" for k, v in row["atrous"].items()))
+    del rc, m, d
+  return summarize(rows)
+
+
+def summarize(rows: list[dict]) -> dict:
+  """Mean PSNRs and times over the evaluated scenes, and the equal-time comparison: a-trous at
+  the smallest sample count whose pipeline time reaches the learned pipeline's."""
+  ks = sorted(rows[0]["atrous"].keys())
+  learned_ms = float(np.mean([r["learned_ms"] for r in rows]))
+  at = {k: (float(np.mean([r["atrous"][k]["ms"] for r in rows])), float(np.mean([r["atrous"][k]["psnr"] for r in rows]))) for k in ks}
+  equal = next((k for k in ks if at[k][0] >= learned_ms), ks[-1])
+  out = {
+    "scenes": len(rows),
+    "noisy_1spp_psnr": float(np.mean([r["noisy_1spp"] for r in rows])),
+    "learned_psnr": float(np.mean([r["learned"] for r in rows])),
+    "learned_ms": learned_ms,
+    "fallback_worlds": int(sum(r["fallback_worlds"] for r in rows)),
+    "atrous": {str(k): {"ms": at[k][0], "psnr": at[k][1]} for k in ks},
+    "equal_time_atrous_spp": equal,
+    "equal_time_atrous_psnr": at[equal][1],
+    "equal_time_atrous_ms": at[equal][0],
+  }
+  out["gain_db"] = out["learned_psnr"] - out["equal_time_atrous_psnr"]
+  return out
+
+
+# ----------------------------------------------------------------------------- CLI
+
+
+def main(argv=None) -> int:
+  p = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
+  p.add_argument("--scenes", type=int, default=16, help="training scene seeds")
+  p.add_argument("--worlds", type=int, default=256, help="randomized worlds per scene seed")
+  p.add_argument("--res", type=int, default=128)
+  p.add_argument("--target-spp", type=int, default=256)
+  p.add_argument("--steps", type=int, default=3000)
+  p.add_argument("--batch", type=int, default=64)
+  p.add_argument("--lr", type=float, default=1e-3)
+  p.add_argument("--eval-scenes", type=int, default=2)
+  p.add_argument("--eval-worlds", type=int, default=128)
+  p.add_argument("--ref-spp", type=int, default=1024)
+  p.add_argument("--out", default=str(learned.DEFAULT_WEIGHTS))
+  p.add_argument("--seed", type=int, default=0)
+  args = p.parse_args(argv)
+
+  dev = torch.device("cuda:0")
+  t0 = time.perf_counter()
+  train_seeds = list(range(100, 100 + args.scenes))
+  log(f"generating {args.scenes} x {args.worlds} pairs at {args.res}^2 ({args.target_spp}-spp targets)")
+  X, T = generate(train_seeds, args.worlds, args.res, args.target_spp, dev)
+  gen_s = time.perf_counter() - t0
+  stats = guide_stats(X)
+  net = learned.UNet(dev, seed=args.seed)
+  log(f"training {net.parameter_count()} parameters for {args.steps} steps, batch {args.batch}")
+  t1 = time.perf_counter()
+  history = train(net, X, T, args.steps, args.batch, args.lr, seed=args.seed)
+  torch.cuda.synchronize()
+  train_s = time.perf_counter() - t1
+  meta = {
+    "guide_range": learned.guide_range_from_stats(stats),
+    "train": {
+      "scene_seeds": [train_seeds[0], train_seeds[-1]],
+      "worlds_per_scene": args.worlds,
+      "images": int(X.shape[0]),
+      "res": args.res,
+      "target_spp": args.target_spp,
+      "steps": args.steps,
+      "batch": args.batch,
+      "lr": args.lr,
+      "loss": history,
+      "generate_s": gen_s,
+      "train_s": train_s,
+    },
+  }
+  learned.save(net, args.out, meta)
+  log(f"saved {args.out}")
+  del X, T
+  torch.cuda.empty_cache()
+  result = {"train": meta["train"]}
+  if args.eval_scenes > 0:
+    eval_seeds = list(range(900, 900 + args.eval_scenes))
+    log(f"evaluating on scene seeds {eval_seeds} ({args.eval_worlds} worlds, {args.ref_spp}-spp reference)")
+    result["eval"] = evaluate(eval_seeds, args.eval_worlds, args.res, args.ref_spp, dev, weights=args.out)
+    meta["eval"] = result["eval"]
+    learned.save(net, args.out, meta)
+  print(json.dumps(result), flush=True)
+  return 0
+
+
+if __name__ == "__main__":
+  sys.exit(main())

@@ -174,6 +174,8 @@ class PbrRenderContext:
   guide_albedo: Any
   guide_normal: Any
   traversal: str = "software"
+  denoiser: str = "atrous"  # "atrous" (the reference's filter) or "learned" (U-Net, a-trous fallback)
+  denoiser_weights: str | None = None  # learned denoiser weights (None: the packaged ones)
   denoise_sigma_color: float = 4.0
   denoise_sigma_normal: float = 128.0
   denoise_sigma_depth: float = 0.02

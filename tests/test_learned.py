@@ -1,0 +1,156 @@
+"""Learned denoiser in the render path: input packing and guide statistics against NumPy, the
+network output written for in-range worlds, the a-trous fallback for worlds whose guide
+statistics leave the trained range (bit-identical to the plain a-trous render)."""
+
+from __future__ import annotations
+
+import math
+
+import numpy as np
+import pytest
+import torch
+
+import mujoco_warp_pbr_amd as mj
+from mujoco_warp_pbr_amd import learned
+from mujoco_warp_pbr_amd import scenes
+from mujoco_warp_pbr_amd.render import render_path_samples
+from oracle import unet as ou
+from tests.helpers import make
+
+RES = 64
+WIDE = [-1e9] * 4 + [1e9] * 4
+
+
+def test_guide_range_from_stats_widens_by_margin():
+  stats = np.array([[0.0, 0.5, 0.2, 0.1], [1.0, 0.7, 0.4, 0.3]])
+  lo_hi = learned.guide_range_from_stats(stats, margin=0.1)
+  np.testing.assert_allclose(lo_hi[:4], [-0.101, 0.479, 0.179, 0.079])
+  np.testing.assert_allclose(lo_hi[4:], [1.101, 0.721, 0.421, 0.321])
+
+
+def test_learned_denoiser_needs_64_multiples():
+  mjm = scenes.primitives_scene(0, res=32)
+  with pytest.raises(ValueError, match="multiples of 64"):
+    mj.create_render_context(mjm, cam_res=(96, 64), tier=mj.Tier.PATH, denoise=True, denoiser="learned", device="cpu", precompute=False)
+  with pytest.raises(ValueError, match="denoiser must be"):
+    mj.create_render_context(mjm, cam_res=(64, 64), tier=mj.Tier.PATH, denoise=True, denoiser="oidn", device="cpu", precompute=False)
+
+
+def _weights(path, cuda, guide_range, seed=3):
+  net = learned.UNet(cuda, seed=seed)
+  learned.save(net, path, {"guide_range": list(guide_range)})
+  return str(path)
+
+
+def _ctx(cuda, nworld, weights):
+  mjm = scenes.full_scene(seed=3, res=RES)
+  m, d, rc = make(
+    mjm, nworld, device=cuda, cam_res=(RES, RES), tier=mj.Tier.PATH, path_samples=1, denoise=True, denoiser="learned",
+    denoiser_weights=weights, envmap=scenes.procedural_sky(0, 64, 128), envmap_intensity=1.0,
+  )
+  scenes.jitter_cameras(d, seed=4, offset=0.2, angle_deg=5.0)
+  mj.refit_bvh(m, d, rc)
+  return m, d, rc
+
+
+def _np_half(t):
+  return t.cpu().numpy().view(np.float16).astype(np.float32)
+
+
+@pytest.mark.gpu
+def test_gpu_pack_input_and_guide_stats_match_numpy(cuda, tmp_path):
+  m, d, rc = _ctx(cuda, 2, _weights(tmp_path / "w.safetensors", cuda, WIDE))
+  render_path_samples(m, d, rc, seed=5)
+  x = learned.pack_input(rc, 0, 2)
+  torch.cuda.synchronize()
+  npix = RES * RES
+  hdr = rc.hdr_data.cpu().numpy()[:, :npix].astype(np.float32)
+  e = np.float32(rc.exposure.cpu().numpy()[0])
+  want = np.zeros((2, npix, 32))
+  want[..., 0:3] = np.log1p(np.maximum(hdr * e, np.float32(0)))
+  want[..., 3:6] = _np_half(rc.aov_albedo)[:, :npix]
+  want[..., 6:9] = _np_half(rc.aov_normal)[:, :npix]
+  dep = rc.aov_depth.cpu().numpy()[:, :npix].astype(np.float32)
+  want[..., 9] = dep / (np.float32(1) + dep)
+  want = ou.bf16_round(want).reshape(2, RES, RES, 32)
+  got = ou.from_bits(x.cpu().numpy())
+  # log1p of the device and of NumPy may differ by one FP32 ulp: at most one BF16 ulp after rounding
+  assert np.all(np.abs(got - want) <= np.abs(want) * 2.0**-7 + 1e-30)
+  assert np.mean(got == want) > 0.999
+  stats = torch.zeros((2, 4), dtype=torch.float32, device=cuda)
+  from mujoco_warp_pbr_amd import _abi
+
+  _abi.call("mjpbr_unet_guide_stats", _abi.ptr(x), 2, RES, RES, 32, 0, _abi.ptr(stats), 0, 0, torch.cuda.current_stream().cuda_stream)
+  hit = np.any(got[..., 6:9] != 0, axis=-1)
+  ref = np.stack(
+    [
+      got[..., 0:3].mean(axis=(1, 2, 3)),
+      1.0 - hit.mean(axis=(1, 2)),
+      np.array([got[i][..., 9][hit[i]].mean() for i in range(2)]),
+      np.array([got[i][..., 3:6][hit[i]].mean() for i in range(2)]),
+    ],
+    axis=1,
+  )
+  np.testing.assert_allclose(stats.cpu().numpy(), ref, rtol=1e-4, atol=1e-6)
+
+
+@pytest.mark.gpu
+def test_gpu_learned_render_writes_network_output(cuda, tmp_path):
+  path = _weights(tmp_path / "w.safetensors", cuda, WIDE)
+  m, d, rc = _ctx(cuda, 2, path)
+  mj.render(m, d, rc, seed=5)
+  torch.cuda.synchronize()
+  den = learned.denoiser_for(rc)
+  assert den.fallback.cpu().numpy().tolist() == [0, 0]
+  x = den.inputs[0]
+  net, _ = learned.load(path, cuda)
+  pred = net.forward(x).cpu().numpy().reshape(2, RES * RES, 3)
+  base = ou.from_bits(x.cpu().numpy()).reshape(2, RES * RES, 32)[..., :3]
+  e = float(rc.exposure.cpu().numpy()[0])
+  want = np.expm1(np.maximum(pred + base, 0.0)) / e
+  got = rc.hdr_data.cpu().numpy()[:, : RES * RES]
+  np.testing.assert_allclose(got, want, rtol=1e-5, atol=1e-6)
+
+
+@pytest.mark.gpu
+def test_gpu_out_of_range_worlds_fall_back_to_atrous(cuda, tmp_path):
+  """World 0 lit 40x brighter leaves the trained radiance range: it must come out exactly as the
+  a-trous render; world 1 (in range) exactly as the all-learned render."""
+  wide = _weights(tmp_path / "wide.safetensors", cuda, WIDE)
+  m, d, rc = _ctx(cuda, 2, wide)
+  rc.env_intensity = torch.tensor([40.0, 1.0], dtype=torch.float32, device=cuda)
+  mj.render(m, d, rc, seed=9)
+  torch.cuda.synchronize()
+  all_learned = rc.base.rgb_data.cpu().numpy().copy()
+  stats = learned.denoiser_for(rc).stats.cpu().numpy()
+  assert stats[0, 0] > stats[1, 0] + 0.05
+  rng = list(WIDE)
+  rng[0], rng[4] = float(stats[1, 0]) - 0.01, float(stats[1, 0]) + 0.01
+  rc.denoiser_weights = _weights(tmp_path / "narrow.safetensors", cuda, rng)
+  rc._host_cache.pop("learned_denoiser")
+  mj.render(m, d, rc, seed=9)
+  torch.cuda.synchronize()
+  mixed = rc.base.rgb_data.cpu().numpy().copy()
+  assert learned.denoiser_for(rc).fallback.cpu().numpy().tolist() == [1, 0]
+  rc.denoiser = "atrous"
+  mj.render(m, d, rc, seed=9)
+  torch.cuda.synchronize()
+  atrous = rc.base.rgb_data.cpu().numpy()
+  np.testing.assert_array_equal(mixed[0], atrous[0])
+  np.testing.assert_array_equal(mixed[1], all_learned[1])
+  assert not np.array_equal(mixed[1], atrous[1])
+
+
+@pytest.mark.gpu
+def test_gpu_training_pairs_and_short_training(cuda):
+  from mujoco_warp_pbr_amd import train_denoiser as td
+
+  _, m, d, rc = td.scene_context(5, 8, RES, cuda)
+  x, t = td.render_pair(m, d, rc, 8, target_spp=16, seed=3)
+  torch.cuda.synchronize()
+  assert x.shape == (8, RES, RES, 32) and t.shape == (8 * RES * RES, 3)
+  assert torch.isfinite(t).all() and (t >= 0).all()
+  net = learned.UNet(cuda, seed=1)
+  hist = td.train(net, x, t, steps=40, batch=4, lr=2e-3, log_every=20)
+  assert math.isfinite(hist[-1][1])
+  assert hist[-1][1] < hist[0][1]
