@@ -271,6 +271,23 @@ typedef struct MjpbrAtrousParams {
   const int32_t* world_select;
 } MjpbrAtrousParams;
 
+/* Persistent wavefront path tracer (no counterpart in the reference): tuning knobs of a launch. */
+typedef struct MjpbrWavefrontConfig {
+  int32_t waves_per_eu;      /* 4 or 8: register budget / occupancy variant of the kernel */
+  int32_t groups_per_cu;     /* persistent workgroups per compute unit (1..16) */
+  int32_t refill_threshold;  /* a wave refills its lanes once this many paths ended (1..64) */
+  int32_t item_limit;        /* > 0: only the first item_limit tiles (autotuner samples) */
+} MjpbrWavefrontConfig;
+
+/* What mjpbr_pt_wavefront_prepare recorded for the launches (host memory, filled by prepare). */
+typedef struct MjpbrWavefrontPlan {
+  int64_t nitems;   /* 16x16 tiles over every world of every group */
+  int64_t npixels;  /* pixels over every world of every group */
+  int32_t lds_bytes;
+  int32_t ngroup;
+  MjpbrSettings settings;
+} MjpbrWavefrontPlan;
+
 /* ---------------------------------------------------------------------------------------- */
 /* Library */
 int mjpbr_abi_version(void);
@@ -293,6 +310,20 @@ int mjpbr_rt_render(const MjpbrScene* scene, const MjpbrSettings* s, MjpbrOutput
  * features (out->aov_*) instead of RGB. */
 int mjpbr_pt_render(const MjpbrScene* scene, const MjpbrSettings* s, MjpbrOutputs* out, int32_t seed,
                     void* stream);
+
+/* Persistent wavefront path tracer over up to 8 world groups, each with its own cameras, world
+ * count and outputs (mixed resolutions in one launch); images bit-identical to mjpbr_pt_render
+ * of each group with the same seed (no counterpart in the reference, whose path tracer is the
+ * per-pixel megakernel of pathtrace.py:340-438). Workspace bytes: mjpbr_pt_wavefront_workspace
+ * (a descriptor header, the work counter, one vec3 slot per path sample). prepare validates,
+ * writes the descriptors (pointers of scenes[g] and outs[g] as they are now) into the workspace
+ * and synchronises `stream`; launch enqueues counter reset, the persistent kernel and the
+ * per-pixel resolve on `stream` without allocating or synchronising (graph-capturable). */
+int64_t mjpbr_pt_wavefront_workspace(const MjpbrScene* scenes, int ngroup, const MjpbrSettings* s);
+int mjpbr_pt_wavefront_prepare(const MjpbrScene* scenes, const MjpbrOutputs* outs, int ngroup, const MjpbrSettings* s,
+                               void* workspace, int64_t workspace_bytes, MjpbrWavefrontPlan* plan, void* stream);
+int mjpbr_pt_wavefront_launch(void* workspace, const MjpbrWavefrontPlan* plan, const MjpbrWavefrontConfig* cfg, int32_t seed,
+                              void* stream);
 
 /* À-trous denoiser over the path tracer's noisy HDR and first-hit features (out->hdr, aov_*):
  * demodulate, `iterations` edge-avoiding passes, remodulate + tone map + encode into out->rgb

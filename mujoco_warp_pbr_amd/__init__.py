@@ -31,6 +31,9 @@ from mujoco_warp_pbr_amd.types import RenderOrder
 from mujoco_warp_pbr_amd.types import Tier
 from mujoco_warp_pbr_amd.types import ToneMapper
 from mujoco_warp_pbr_amd.types import UnsetMaterial
+from mujoco_warp_pbr_amd.wavefront import WavefrontBatch
+from mujoco_warp_pbr_amd.wavefront import WavefrontConfig
+from mujoco_warp_pbr_amd.wavefront import render_batch
 
 __all__ = [
   "HostModel",
@@ -40,6 +43,8 @@ __all__ = [
   "Tier",
   "ToneMapper",
   "UnsetMaterial",
+  "WavefrontBatch",
+  "WavefrontConfig",
   "create_render_context",
   "get_albedo",
   "get_depth",
@@ -52,4 +57,5 @@ __all__ = [
   "put_model",
   "refit_bvh",
   "render",
+  "render_batch",
 ]

@@ -24,6 +24,7 @@ extern "C" int mjpbr_struct_sizes(int64_t* sizes, int n) {
       (int64_t)sizeof(MjpbrGeometry), (int64_t)sizeof(MjpbrMaterials), (int64_t)sizeof(MjpbrLights),
       (int64_t)sizeof(MjpbrCameras),  (int64_t)sizeof(MjpbrEnvironment), (int64_t)sizeof(MjpbrSettings),
       (int64_t)sizeof(MjpbrOutputs),  (int64_t)sizeof(MjpbrScene),      (int64_t)sizeof(MjpbrAtrousParams),
+      (int64_t)sizeof(MjpbrWavefrontConfig), (int64_t)sizeof(MjpbrWavefrontPlan),
   };
   int count = (int)(sizeof(all) / sizeof(all[0]));
   for (int i = 0; i < n && i < count; ++i) sizes[i] = all[i];

@@ -1,0 +1,307 @@
+// wavefront.hip — persistent path tracer over a batch of world groups with mixed camera
+// resolutions, in one launch (no counterpart in the reference, whose path tracer is the per-pixel
+// megakernel of pathtrace.py:200-438; the per-path arithmetic here is that kernel's, path.hpp).
+//
+// Work items are 16x16 pixel tiles of one world of one group (every group has its own cameras,
+// world count and outputs), handed out by a global atomic counter to a persistent grid. A
+// workgroup stages the item's world (scene BVH + geom frames) in LDS; its four waves then draw the
+// tile's 256 x spp paths from an LDS counter. Paths end after different numbers of segments
+// (misses, Russian roulette, depth limit): after each segment a wave ballots its finished lanes
+// and, once at least `refill` lanes are free, claims that many new paths with one LDS atomic and
+// hands them to the free lanes by their prefix rank (mbcnt). Lanes stay busy with live paths
+// instead of idling until the wave's longest path ends. A path's radiance goes to its sample slot;
+// a resolve pass averages each pixel's samples in sample order and encodes the display colour,
+// so the images are bit-identical to the megakernel's.
+//
+// Every launch is asynchronous and allocation-free on the caller's stream (the work counter is
+// cleared by a kernel, not a memset), so the three launches can be captured in a graph.
+#pragma clang fp contract(off)
+#include "abi_util.hpp"
+#include "path.hpp"
+
+namespace mjpbr {
+
+constexpr int kWfMaxGroups = 8;
+constexpr int kTilePixels = kTile * kTile;
+
+struct WfGroup {
+  MjpbrScene sc;
+  MjpbrOutputs out;
+  TileMap tm;
+  int64_t item_base;   // first work item of the group
+  int64_t pixel_base;  // first resolve index (worlds x total_rays before this group)
+};
+
+struct WfHeader {
+  int32_t ngroup;
+  int32_t spp;
+  int64_t nitems;
+  int64_t npixels;
+  WfGroup groups[kWfMaxGroups];
+};
+
+constexpr size_t kHeaderBytes = (sizeof(WfHeader) + 255) / 256 * 256;
+constexpr size_t kCounterBytes = 256;
+
+MJ_DEV int lanes_below(uint64_t mask) {
+  return (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+}
+
+MJ_DEV int find_group(const WfHeader* h, int64_t index, bool by_pixel) {
+  int g = 0;
+  while (g + 1 < h->ngroup && index >= (by_pixel ? h->groups[g + 1].pixel_base : h->groups[g + 1].item_base)) ++g;
+  return g;
+}
+
+__global__ void wf_reset_kernel(int* counter) {
+  if (threadIdx.x == 0) counter[0] = 0;
+}
+
+template <int kWavesPerEu>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kWavesPerEu, kWavesPerEu)))
+wf_kernel(const WfHeader* __restrict__ hdr, int* counter, float* slots, MjpbrSettings s, int seed, int refill, int64_t item_limit) {
+  extern __shared__ float lds[];
+  __shared__ int s_item;
+  __shared__ int s_pool;
+  const int lane = threadIdx.x & 63;
+  const int64_t nitems = item_limit > 0 && item_limit < hdr->nitems ? item_limit : hdr->nitems;
+  const int spp = s.path_samples;
+  const int pool = kTilePixels * spp;
+  for (;;) {
+    __syncthreads();  // every wave is done with the previous item's LDS and pool
+    if (threadIdx.x == 0) {
+      s_item = atomicAdd(counter, 1);
+      s_pool = 0;
+    }
+    __syncthreads();
+    const int item = __builtin_amdgcn_readfirstlane(s_item);
+    if (item >= nitems) break;
+    const int g = __builtin_amdgcn_readfirstlane(find_group(hdr, item, false));
+    const WfGroup& G = hdr->groups[g];
+    const MjpbrScene& sc = G.sc;
+    const int64_t local = item - G.item_base;
+    const int w = (int)(local / G.tm.tiles_per_world);
+    const int t = (int)(local - (int64_t)w * G.tm.tiles_per_world);
+    int cidx = 0;
+    while (cidx + 1 < G.tm.nrender && t >= G.tm.tile_base[cidx + 1]) ++cidx;
+    const int tl = t - G.tm.tile_base[cidx];
+    const int tx = tl % G.tm.tiles_x[cidx], ty = tl / G.tm.tiles_x[cidx];
+    SceneView sv = stage_world(sc, w, lds);
+    const float exposure = sc.env.exposure[w % sc.env.exposure_n];
+    PtCtx c{sc, s, sv, w, exposure};
+    const int total_rays = sc.cams.total_rays;
+    const int ray_offset = G.tm.ray_offset[cidx];
+    const float* xm = sc.cams.cam_xmat + ((int64_t)w * sc.cams.ncam + sc.cams.cam_id_map[cidx]) * 9;
+    const V3 head_dir = -V3{xm[2], xm[5], xm[8]};
+    const bool want_depth = G.out.render_depth[cidx] != 0, want_seg = G.out.render_seg[cidx] != 0;
+    const int64_t world_slot = ((int64_t)G.pixel_base + (int64_t)w * total_rays + ray_offset) * spp;
+
+    PathState st;
+    Primary first;
+    bool alive = false, pool_done = false;
+    int smp = 0, pseed = 0, pixelid = 0;
+    float dcz = 0.0f;
+    for (;;) {
+      const uint64_t dead = __ballot(!alive);
+      const int ndead = __popcll(dead);
+      if (!pool_done && ndead >= refill) {
+        int base = 0;
+        if (lane == 0) base = atomicAdd(&s_pool, ndead);
+        base = __shfl(base, 0);
+        if (base >= pool) {
+          pool_done = true;
+        } else if (!alive) {
+          const int pid = base + lanes_below(dead);
+          if (pid < pool) {
+            const int sample = pid / kTilePixels, lp = pid - sample * kTilePixels;
+            const int blk = lp >> 6, ln = lp & 63;
+            const int px = tx * kTile + (blk & 1) * 8 + (ln & 7), py = ty * kTile + (blk >> 1) * 8 + (ln >> 3);
+            Pixel pix;
+            if (setup_pixel(sc.cams, G.out, s, w, cidx, px, py, ray_offset, &pix)) {
+              V3 o, d, dc;
+              float md;
+              if (!pix.want_rgb) {
+                if (sample == 0) {
+                  const int centre = kSubpixel / 2;
+                  primary_ray(pix, s, kSubpixel, centre, centre, &o, &d, &md, &dc);
+                  Hit hit = trace_closest(sv, o, d, md, s.cull_backfaces != 0);
+                  write_geometry(G.out, w, pix, hit, dc);
+                }
+              } else {
+                pseed = seed + w * total_rays + pix.rayid;
+                path_primary(pix, s, pseed, sample, &o, &d, &md, &dc);
+                path_begin(&st, &first, o, d, md);
+                smp = sample;
+                pixelid = pix.pixelid;
+                dcz = dc.z;
+                alive = true;
+              }
+            }
+          }
+        }
+      }
+      if (ndead == 64 && pool_done) break;
+      if (alive && !path_step(c, &st, head_dir, pseed, smp, &first)) {
+        alive = false;
+        st3(slots + 3 * (world_slot + (int64_t)pixelid * spp + smp), st.radiance);
+        if (smp == 0) {
+          if (want_seg) {
+            int2 seg = first.hit.geom_id != -1 ? make_int2(first.hit.geom_id, 5) : make_int2(-1, -1);
+            reinterpret_cast<int2*>(G.out.seg)[(int64_t)w * G.out.seg_stride + G.out.seg_adr[cidx] + pixelid] = seg;
+          }
+          if (want_depth) {
+            float depth = first.hit.geom_id != -1 ? first.hit.dist * -dcz : 0.0f;
+            G.out.depth[(int64_t)w * G.out.depth_stride + G.out.depth_adr[cidx] + pixelid] = depth;
+          }
+          if (s.render_guides) {
+            int64_t a = (int64_t)w * G.out.guide_stride + G.out.rgb_adr[cidx] + pixelid;
+            st3(G.out.albedo + 3 * a, first.base_color);
+            st3(G.out.normal + 3 * a, first.normal);
+          }
+        }
+      }
+    }
+  }
+}
+
+// Mean of each pixel's samples in sample order, HDR and display colour (pathtrace.py:420-438).
+__global__ void wf_resolve_kernel(const WfHeader* __restrict__ hdr, const float* slots, MjpbrSettings s) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= hdr->npixels) return;
+  const WfGroup& G = hdr->groups[find_group(hdr, i, true)];
+  const int64_t local = i - G.pixel_base;
+  const int total_rays = G.sc.cams.total_rays;
+  const int w = (int)(local / total_rays);
+  const int r = (int)(local - (int64_t)w * total_rays);
+  int cidx = 0;
+  while (cidx + 1 < G.tm.nrender && r >= G.tm.ray_offset[cidx + 1]) ++cidx;
+  if (!(s.any_rgb && G.out.render_rgb[cidx])) return;
+  const int pixelid = r - G.tm.ray_offset[cidx];
+  const int spp = s.path_samples;
+  V3 sum = v3s(0.0f);
+  for (int k = 0; k < spp; ++k) sum += ld3(slots + 3 * (i * spp + k));
+  V3 h = sum / (float)spp;
+  const float exposure = G.sc.env.exposure[w % G.sc.env.exposure_n];
+  const int64_t adr = G.out.rgb_adr[cidx] + pixelid;
+  if (s.render_hdr) st3(G.out.hdr + 3 * ((int64_t)w * G.out.hdr_stride + adr), h);
+  G.out.rgb[(int64_t)w * G.out.rgb_stride + adr] = encode_display(display_transform(h * exposure, s.tone_mapper));
+}
+
+int device_cus() {
+  static int cached[64] = {0};
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
+  if (!cached[dev]) {
+    int n = 0;
+    if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n < 1) n = 256;
+    cached[dev] = n;
+  }
+  return cached[dev];
+}
+
+int64_t slot_count(const MjpbrScene* scenes, int ngroup, int spp) {
+  int64_t n = 0;
+  for (int g = 0; g < ngroup; ++g) n += (int64_t)scenes[g].geo.nworld * scenes[g].cams.total_rays;
+  return n * spp;
+}
+
+}  // namespace mjpbr
+
+using namespace mjpbr;
+
+extern "C" int64_t mjpbr_pt_wavefront_workspace(const MjpbrScene* scenes, int ngroup, const MjpbrSettings* s) {
+  if (ngroup < 1 || ngroup > kWfMaxGroups || s->path_samples < 1) return -1;
+  size_t slots = (size_t)slot_count(scenes, ngroup, s->path_samples) * 12;
+  return (int64_t)(kHeaderBytes + kCounterBytes + (slots + 255) / 256 * 256);
+}
+
+extern "C" int mjpbr_pt_wavefront_prepare(const MjpbrScene* scenes, const MjpbrOutputs* outs, int ngroup, const MjpbrSettings* s,
+                                          void* workspace, int64_t workspace_bytes, MjpbrWavefrontPlan* plan, void* stream) {
+  if (ngroup < 1 || ngroup > kWfMaxGroups) return set_error(MJPBR_EINVAL, "wavefront: 1 to %d world groups, got %d", kWfMaxGroups, ngroup);
+  if (s->path_samples < 1) return set_error(MJPBR_EINVAL, "path_samples must be at least 1");
+  if (s->max_depth < 0 || s->max_depth + 4 * s->max_layers >= kJitterDepth)
+    return set_error(MJPBR_EINVAL, "max_depth + 4 * max_layers must be below %d", kJitterDepth);
+  if (s->denoise) return set_error(MJPBR_EINVAL, "wavefront: the denoiser features are written by the per-pixel kernel only");
+  int64_t need = mjpbr_pt_wavefront_workspace(scenes, ngroup, s);
+  if (!workspace || workspace_bytes < need)
+    return set_error(MJPBR_EINVAL, "wavefront: workspace of %lld bytes, need %lld", (long long)workspace_bytes, (long long)need);
+  WfHeader* h = new WfHeader();
+  h->ngroup = ngroup;
+  h->spp = s->path_samples;
+  int64_t items = 0, pixels = 0;
+  size_t lds = 0;
+  for (int g = 0; g < ngroup; ++g) {
+    WfGroup& G = h->groups[g];
+    G.sc = scenes[g];
+    G.out = outs[g];
+    int nlaunch;
+    int rc = make_tile_map(&scenes[g], &G.tm, &nlaunch);
+    if (rc) {
+      delete h;
+      return rc;
+    }
+    size_t b = host_stage_bytes(&scenes[g].geo);
+    if (b > 64 * 1024) {
+      delete h;
+      return set_error(MJPBR_EINVAL, "wavefront: group %d scene too large for LDS staging (%d geoms)", g, scenes[g].geo.ngeom);
+    }
+    lds = b > lds ? b : lds;
+    G.item_base = items;
+    G.pixel_base = pixels;
+    items += (int64_t)G.tm.tiles_per_world * scenes[g].geo.nworld;
+    pixels += (int64_t)scenes[g].geo.nworld * scenes[g].cams.total_rays;
+  }
+  if (items >= (1ll << 31)) {
+    delete h;
+    return set_error(MJPBR_EINVAL, "wavefront: too many tiles (%lld)", (long long)items);
+  }
+  h->nitems = items;
+  h->npixels = pixels;
+  hipStream_t st = (hipStream_t)stream;
+  hipError_t e = hipMemcpyAsync(workspace, h, sizeof(WfHeader), hipMemcpyHostToDevice, st);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  delete h;
+  if (e != hipSuccess) return set_error(MJPBR_ELAUNCH, "wavefront prepare: %s", hipGetErrorString(e));
+  plan->nitems = items;
+  plan->npixels = pixels;
+  plan->lds_bytes = (int32_t)lds;
+  plan->ngroup = ngroup;
+  plan->settings = *s;
+  return MJPBR_OK;
+}
+
+extern "C" int mjpbr_pt_wavefront_launch(void* workspace, const MjpbrWavefrontPlan* plan, const MjpbrWavefrontConfig* cfg, int32_t seed,
+                                         void* stream) {
+  if (!workspace || plan->nitems < 1) return set_error(MJPBR_EINVAL, "wavefront: not prepared");
+  int refill = cfg->refill_threshold;
+  if (refill < 1 || refill > 64) return set_error(MJPBR_EINVAL, "wavefront: refill_threshold must be in [1, 64], got %d", refill);
+  if (cfg->groups_per_cu < 1 || cfg->groups_per_cu > 16)
+    return set_error(MJPBR_EINVAL, "wavefront: groups_per_cu must be in [1, 16], got %d", cfg->groups_per_cu);
+  const WfHeader* hdr = reinterpret_cast<const WfHeader*>(workspace);
+  int* counter = reinterpret_cast<int*>(reinterpret_cast<char*>(workspace) + kHeaderBytes);
+  float* slots = reinterpret_cast<float*>(reinterpret_cast<char*>(workspace) + kHeaderBytes + kCounterBytes);
+  hipStream_t st = (hipStream_t)stream;
+  hipLaunchKernelGGL(wf_reset_kernel, dim3(1), dim3(64), 0, st, counter);
+  int rc = check_launch("wf_reset_kernel");
+  if (rc) return rc;
+  int64_t items = cfg->item_limit > 0 && cfg->item_limit < plan->nitems ? cfg->item_limit : plan->nitems;
+  int64_t grid = (int64_t)device_cus() * cfg->groups_per_cu;
+  if (grid > items) grid = items;
+  const MjpbrSettings& s = plan->settings;
+  switch (cfg->waves_per_eu) {
+    case 4:
+      hipLaunchKernelGGL(wf_kernel<4>, dim3((unsigned)grid), dim3(256), plan->lds_bytes, st, hdr, counter, slots, s, seed, refill,
+                         (int64_t)cfg->item_limit);
+      break;
+    case 8:
+      hipLaunchKernelGGL(wf_kernel<8>, dim3((unsigned)grid), dim3(256), plan->lds_bytes, st, hdr, counter, slots, s, seed, refill,
+                         (int64_t)cfg->item_limit);
+      break;
+    default:
+      return set_error(MJPBR_EINVAL, "wavefront: waves_per_eu must be 4 or 8, got %d", cfg->waves_per_eu);
+  }
+  rc = check_launch("wf_kernel");
+  if (rc) return rc;
+  hipLaunchKernelGGL(wf_resolve_kernel, dim3((unsigned)((plan->npixels + 255) / 256)), dim3(256), 0, st, hdr, slots, s);
+  return check_launch("wf_resolve_kernel");
+}

@@ -154,3 +154,19 @@ def test_gpu_training_pairs_and_short_training(cuda):
   hist = td.train(net, x, t, steps=40, batch=4, lr=2e-3, log_every=20)
   assert math.isfinite(hist[-1][1])
   assert hist[-1][1] < hist[0][1]
+
+
+@pytest.mark.gpu
+def test_gpu_packaged_denoiser_beats_atrous_at_equal_time(cuda):
+  """The shipped weights on a held-out scene seed (never in training: seeds 100-131): PSNR of
+  1 sample + U-Net against a 1024-sample reference beats a-trous given at least the same GPU time
+  (its sample count raised until its pipeline is no faster) by 2 dB or more."""
+  from mujoco_warp_pbr_amd import train_denoiser as td
+
+  _, meta = learned.load(learned.DEFAULT_WEIGHTS, cuda)
+  assert meta["train"]["images"] >= 8192 and meta["train"]["target_spp"] >= 1024
+  res = td.evaluate([907], 64, 128, 1024, cuda, atrous_spp=(1, 2, 3, 4, 6))
+  assert res["fallback_worlds"] == 0
+  assert res["equal_time_atrous_ms"] >= res["learned_ms"]
+  assert res["learned_psnr"] > res["noisy_1spp_psnr"] + 5.0
+  assert res["gain_db"] >= 2.0, res

@@ -1,0 +1,142 @@
+"""Persistent wavefront path tracer: batches of world groups with mixed camera resolutions (edge
+tiles, two cameras per world) must reproduce the per-pixel megakernel bit for bit, for every
+launch shape the autotuner can pick, eagerly and as a replayed graph."""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+import torch
+
+import mujoco_warp_pbr_amd as mj
+from mujoco_warp_pbr_amd import autotune
+from mujoco_warp_pbr_amd import model as mdl
+from mujoco_warp_pbr_amd import scenes
+from mujoco_warp_pbr_amd.scenes import _Builder
+from mujoco_warp_pbr_amd.wavefront import WavefrontBatch
+from mujoco_warp_pbr_amd.wavefront import WavefrontConfig
+from tests.helpers import make
+
+PT = {"tier": mj.Tier.PATH, "path_samples": 3, "max_depth": 6, "rr_depth": 2, "render_guides": True}
+
+
+def _scene():
+  """Textured mirror plane, primitives, one convex mesh, sun + soft point light, two cameras."""
+  rng = np.random.default_rng(7)
+  b = _Builder()
+  tex = [b.texture(img, cs) for img, cs in scenes.procedural_textures(7, 64)]
+  roles = {mdl.ROLE_RGBA: tex[0], mdl.ROLE_ORM: tex[1], mdl.ROLE_NORMAL: tex[2]}
+  floor = b.material(rgba=(1, 1, 1, 1), roughness=0.6, reflectance=0.3, texid=roles, repeat=(4, 4))
+  b.geom(mdl.GEOM_PLANE, (3, 3, 0.1), (0, 0, 0), matid=floor)
+  for i, k in enumerate((mdl.GEOM_SPHERE, mdl.GEOM_BOX, mdl.GEOM_CAPSULE, mdl.GEOM_CYLINDER)):
+    mat = b.material(rgba=(*rng.uniform(0.2, 1, 3), 1), roughness=float(rng.uniform(0.1, 1)), metallic=float(i == 1))
+    b.geom(k, (0.15, 0.12, 0.1), (rng.uniform(-0.6, 0.6), rng.uniform(-0.6, 0.6), 0.2), mat=scenes.random_rotation(rng), matid=mat)
+  verts, faces = scenes.convex_hull_mesh(rng, 200, 0.2)
+  b.geom(mdl.GEOM_MESH, (0, 0, 0), (0.2, 0.3, 0.25), matid=b.material(rgba=(0.8, 0.5, 0.3, 1), roughness=0.4), dataid=b.mesh(verts, faces))
+  b.light(mdl.LIGHT_DIRECTIONAL, (0, 0, 3), (-0.4, 0.3, -0.86), bulbradius=0.05)
+  b.light(mdl.LIGHT_POINT, (0.8, -0.6, 2.0), (0, 0, -1), bulbradius=0.05)
+  b.camera((0.0, -2.5, 1.6), (0, 0, 0.2), 45.0, (32, 32))
+  b.camera((1.8, -1.2, 1.0), (0, 0, 0.2), 60.0, (32, 32))
+  return b.build()
+
+
+def _group(device, nworld, res, seed):
+  m, d, rc = make(_scene(), nworld, device=device, cam_res=res, envmap=scenes.procedural_sky(1, 32, 64), envmap_intensity=1.0, **PT)
+  if device != "cpu":
+    scenes.randomize_worlds(m, rc, nworld, seed=seed)
+    scenes.jitter_cameras(d, seed=seed + 1, offset=0.1, angle_deg=3.0)
+    mj.refit_bvh(m, d, rc)
+  return m, d, rc
+
+
+GROUPS = [(3, [(16, 16), (24, 8)]), (2, [(40, 24), (16, 16)]), (2, [(48, 48), (32, 16)])]
+
+
+def _outputs(rc):
+  torch.cuda.synchronize()
+  return {
+    "rgb": rc.base.rgb_data.cpu().numpy().copy(),
+    "hdr": rc.hdr_data.cpu().numpy().copy(),
+    "depth": rc.base.depth_data.cpu().numpy().copy(),
+    "seg": rc.base.seg_data.cpu().numpy().copy(),
+    "albedo": rc.guide_albedo.cpu().numpy().copy(),
+    "normal": rc.guide_normal.cpu().numpy().copy(),
+  }
+
+
+def _clear(rc):
+  for t in (rc.base.rgb_data, rc.hdr_data, rc.base.depth_data, rc.base.seg_data, rc.guide_albedo, rc.guide_normal):
+    t.fill_(7)
+
+
+def _assert_same(a, b, what):
+  for k in a:
+    assert np.array_equal(a[k], b[k]), f"{what}: {k} differs in {np.sum(a[k] != b[k])} entries"
+
+
+def test_batch_validation():
+  groups = [_group("cpu", 2, [(16, 16), (16, 16)], 0)]
+  m, d, rc = make(_scene(), 2, device="cpu", cam_res=[(16, 16), (16, 16)], tier=mj.Tier.REALTIME)
+  with pytest.raises(ValueError, match="path-traced"):
+    WavefrontBatch([(m, d, rc)])
+  m2, d2, rc2 = make(_scene(), 2, device="cpu", cam_res=[(16, 16), (16, 16)], envmap=scenes.procedural_sky(1, 32, 64), **{**PT, "path_samples": 5})
+  with pytest.raises(ValueError, match="settings differ"):
+    WavefrontBatch(groups + [(m2, d2, rc2)])
+  with pytest.raises(ValueError, match="runs on a GPU"):
+    WavefrontBatch(groups)
+  with pytest.raises(ValueError, match="1 to 8"):
+    WavefrontBatch(groups * 9)
+
+
+@pytest.mark.gpu
+def test_gpu_wavefront_matches_megakernel(cuda):
+  groups = [_group(cuda, n, res, 10 * i) for i, (n, res) in enumerate(GROUPS)]
+  ref = []
+  for m, d, rc in groups:
+    mj.render(m, d, rc, seed=11)
+    ref.append(_outputs(rc))
+  batch = WavefrontBatch(groups)
+  for cfg in (WavefrontConfig(), WavefrontConfig(4, 2, 1), WavefrontConfig(8, 8, 64), WavefrontConfig(8, 1, 32)):
+    for _, _, rc in groups:
+      _clear(rc)
+    batch.render(seed=11, config=cfg)
+    for i, (_, _, rc) in enumerate(groups):
+      _assert_same(_outputs(rc), ref[i], f"group {i} {cfg}")
+
+
+@pytest.mark.gpu
+def test_gpu_wavefront_graph_replays_match_eager(cuda):
+  groups = [_group(cuda, n, res, 10 * i) for i, (n, res) in enumerate(GROUPS)]
+  batch = WavefrontBatch(groups)
+  batch.render(seed=4)
+  eager = [_outputs(rc) for _, _, rc in groups]
+  graph = batch.capture(seed=4)
+  for rep in range(3):
+    for _, _, rc in groups:
+      _clear(rc)
+    torch.cuda.synchronize()
+    graph.replay()
+    for i, (_, _, rc) in enumerate(groups):
+      _assert_same(_outputs(rc), eager[i], f"replay {rep} group {i}")
+    # eager work between replays
+    mj.render(*groups[0], seed=5)
+    torch.cuda.synchronize()
+
+
+@pytest.mark.gpu
+def test_gpu_autotuner_searches_and_caches(cuda, tmp_path, monkeypatch):
+  monkeypatch.setenv("MJPBR_CACHE_DIR", str(tmp_path))
+  groups = [_group(cuda, n, res, 10 * i) for i, (n, res) in enumerate(GROUPS)]
+  batch = WavefrontBatch(groups)
+  best, table = autotune.tune(batch, seed=2)
+  assert len(table) >= 8
+  for knob, values in autotune.SPACE.items():
+    assert getattr(best, knob) in values
+  again, table2 = autotune.tune(WavefrontBatch(groups), seed=2)
+  assert again == best and table2[0][0] == "cached"
+  # the tuned shape renders the same image as the default one
+  batch.render(seed=2, config=WavefrontConfig())
+  a = [_outputs(rc) for _, _, rc in groups]
+  batch.render(seed=2, config=best)
+  for i, (_, _, rc) in enumerate(groups):
+    _assert_same(_outputs(rc), a[i], f"tuned group {i}")
