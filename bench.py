@@ -8,9 +8,16 @@ per-world jittered cameras. Outputs rgb (packed sRGB8), depth, segmentation. Inp
 resident in HBM before the timed region.
 
 Prints one JSON line: Mpixels/s, the roofline of the render kernel (algorithmic HBM bytes of
-one launch / its average duration measured with HIP events on the launch stream) and the CPU
-oracle (scalar C restatement, threads over worlds) timed on a bounded sample of the same
-workload on this host.
+one launch / its average duration measured with HIP events on the launch stream; `traffic` from
+the committed rocprofv3 counter runs) and the CPU oracle (scalar C restatement, threads over
+worlds) timed on a bounded sample of the same workload on this host. Two path-tracing sections
+follow in the same line (skip with --no-pt):
+  pt_wavefront      4096 worlds at mixed resolutions (2048 x 64^2, 1536 x 128^2, 512 x 256^2),
+                    4 spp, depth 8: the persistent wavefront tracer (autotuned, graph replay)
+                    against the per-pixel megakernel, images identical;
+  learned_denoiser  128x128 path tracing, 1 spp + learned U-Net against a-trous at the sample
+                    count of equal GPU time (timed on 1024 worlds), PSNR against a 4096-spp
+                    reference, and the U-Net's matrix-core throughput against the BF16 peak.
 """
 
 from __future__ import annotations
@@ -80,6 +87,21 @@ def algorithmic_bytes(mjm, rc, nworld: int, npix: int) -> tuple[float, float]:
   return total, total / npix
 
 
+TRAFFIC_PROFILE = ROOT / "profiles" / "rt_kernel_full_scene_1024x128_traffic.md"
+
+
+def measured_traffic(nworld: int, res: int):
+  """HBM bytes per launch of the render kernel from the committed rocprofv3 FETCH_SIZE /
+  WRITE_SIZE runs (scaled by a 1 GiB copy kernel's counters), for the workload they were taken
+  on; None for any other shape."""
+  if (nworld, res) != (1024, 128) or not TRAFFIC_PROFILE.exists():
+    return None
+  for line in TRAFFIC_PROFILE.read_text().splitlines():
+    if line.startswith("traffic_bytes_per_launch:"):
+      return int(line.split(":", 1)[1])
+  return None
+
+
 def cpu_baseline(mjm, m, d, rc, target_s: float = 12.0) -> dict:
   """The CPU oracle on a sample of the workload's worlds, one thread per available core."""
   import oracle
@@ -105,6 +127,114 @@ def cpu_baseline(mjm, m, d, rc, target_s: float = 12.0) -> dict:
   }
 
 
+C2_GROUPS = ((2048, 64), (1536, 128), (512, 256))
+
+
+def pt_wavefront_section(device) -> dict:
+  """Batched path tracing: 4096 worlds of the full-feature scene, 2048 at 64x64, 1536 at 128x128
+  and 512 at 256x256, 4 samples, depth 8, Russian roulette from 3, per-world materials, cameras,
+  sky rotation and light intensities. The persistent wavefront tracer (one launch over the three
+  groups, autotuned shape, graph replay) against the per-pixel megakernel (one launch a group);
+  images must be identical."""
+  import numpy as np
+  import torch
+
+  import mujoco_warp_pbr_amd as mj
+  from mujoco_warp_pbr_amd import autotune
+  from mujoco_warp_pbr_amd import scenes
+  from mujoco_warp_pbr_amd.wavefront import WavefrontBatch
+
+  groups = []
+  for i, (nworld, res) in enumerate(C2_GROUPS):
+    mjm = scenes.full_scene(seed=0, res=res)
+    m = mj.put_model(mjm, device)
+    d = mj.put_data(mjm, nworld, device)
+    rc = mj.create_render_context(
+      mjm, nworld=nworld, cam_res=(res, res), envmap=scenes.procedural_sky(0, 256, 512), tier=mj.Tier.PATH, path_samples=4,
+      max_depth=8, rr_depth=3, ev100=math.log2(1.0 / 1.2), device=device,
+    )
+    scenes.randomize_worlds(m, rc, nworld, seed=10 * i + 1)
+    scenes.jitter_cameras(d, seed=10 * i + 2)
+    rng = np.random.default_rng(10 * i + 3)
+    rc.env_yaw = torch.as_tensor(rng.uniform(0, 2 * math.pi, nworld).astype(np.float32), device=device)
+    li = rc.light_intensity.cpu().numpy()
+    li = np.broadcast_to(li[:1], (nworld, li.shape[1])) * np.exp(rng.normal(0.0, 0.5, (nworld, li.shape[1])))
+    rc.light_intensity = torch.as_tensor(li.astype(np.float32), device=device)
+    mj.refit_bvh(m, d, rc)
+    groups.append((m, d, rc))
+  npix = sum(rc.base.nworld * rc.base.total_rays for _, _, rc in groups)
+  paths = npix * 4
+  stream = torch.cuda.current_stream()
+
+  def timed(fn, reps=3):
+    fn()
+    torch.cuda.synchronize()
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record(stream)
+    for _ in range(reps):
+      fn()
+    b.record(stream)
+    b.synchronize()
+    return a.elapsed_time(b) / reps
+
+  mega_ms = timed(lambda: [mj.render(m, d, rc, seed=3) for m, d, rc in groups])
+  ref = [rc.base.rgb_data.clone() for _, _, rc in groups]
+  batch = WavefrontBatch(groups)
+  cfg, table = autotune.tune(batch, seed=3, log=log)
+  graph = batch.capture(seed=3, config=cfg)
+  wf_ms = timed(graph.replay)
+  identical = all(torch.equal(r, rc.base.rgb_data) for r, (_, _, rc) in zip(ref, groups))
+  log(f"pt wavefront {wf_ms:.1f} ms vs megakernel {mega_ms:.1f} ms, identical images: {identical}")
+  return {
+    "workload": "path tracing, full-feature scene, 4096 worlds: 2048 x 64^2 + 1536 x 128^2 + 512 x 256^2, 4 spp, depth 8, RR from 3",
+    "pixels": npix,
+    "paths": paths,
+    "megakernel_ms": round(mega_ms, 3),
+    "megakernel_Mpaths_s": round(paths / mega_ms / 1e3, 1),
+    "wavefront_ms": round(wf_ms, 3),
+    "wavefront_Mpaths_s": round(paths / wf_ms / 1e3, 1),
+    "wavefront_Mpixels_s": round(npix / wf_ms / 1e3, 1),
+    "speedup": round(mega_ms / wf_ms, 3),
+    "identical_to_megakernel": identical,
+    "config": {k: getattr(cfg, k) for k in ("waves_per_eu", "groups_per_cu", "refill_threshold")},
+    "autotune_table": table,
+  }
+
+
+def learned_section(device, time_worlds: int = 1024, quality_worlds: int = 128, ref_spp: int = 4096) -> dict:
+  """Path tracing at 128x128, 1 sample + learned denoiser against a-trous (5 iterations) at k
+  samples, on a held-out scene seed; GPU time of each pipeline on `time_worlds` worlds, PSNR on
+  `quality_worlds` worlds against a `ref_spp` reference. Equal time: the smallest k whose a-trous
+  pipeline is no faster than the learned one."""
+  from mujoco_warp_pbr_amd import train_denoiser as td
+
+  res = td.evaluate([900], quality_worlds, 128, ref_spp, device, atrous_spp=(1, 2, 3, 4, 6, 8), time_worlds=time_worlds)
+  log(f"learned {res['learned_psnr']:.2f} dB vs a-trous@{res['equal_time_atrous_spp']} {res['equal_time_atrous_psnr']:.2f} dB "
+      f"at {res['learned_ms']:.1f} / {res['equal_time_atrous_ms']:.1f} ms")
+  peak = 2500.0
+  return {
+    "workload": f"path tracing 128x128, full-feature scene (held-out seed), time on {time_worlds} worlds, PSNR on {quality_worlds} worlds vs {ref_spp}-spp reference",
+    "learned_psnr_db": round(res["learned_psnr"], 3),
+    "equal_time_atrous_psnr_db": round(res["equal_time_atrous_psnr"], 3),
+    "gain_db": round(res["gain_db"], 3),
+    "learned_ms": round(res["learned_ms"], 3),
+    "equal_time_atrous_spp": res["equal_time_atrous_spp"],
+    "equal_time_atrous_ms": round(res["equal_time_atrous_ms"], 3),
+    "noisy_1spp_psnr_db": round(res["noisy_1spp_psnr"], 3),
+    "atrous": res["atrous"],
+    "fallback_worlds": res["fallback_worlds"],
+    "unet_roofline": {
+      "bound": "mfma",
+      "achieved": round(res["unet_tflops"], 2),
+      "achieved_algorithmic": round(res["unet_tflops_algorithmic"], 2),
+      "peak": peak,
+      "unit": "TFLOP/s",
+      "frac": round(res["unet_tflops"] / peak, 4),
+      "ms": round(res["unet_ms"], 3),
+    },
+  }
+
+
 def main(argv=None) -> int:
   p = argparse.ArgumentParser()
   p.add_argument("--gpus", type=int, default=1)
@@ -113,7 +243,20 @@ def main(argv=None) -> int:
   p.add_argument("--nworld", type=int, default=1024)
   p.add_argument("--res", type=int, default=128)
   p.add_argument("--no-cpu-baseline", action="store_true")
+  p.add_argument("--no-pt", action="store_true", help="skip the path-tracing sections (wavefront batch, learned denoiser)")
+  p.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
+                 help="only copy BYTES between two device buffers 3 times (the reference kernel that scales HBM counters)")
   args = p.parse_args(argv)
+  if args.copy_calibration:
+    import torch
+
+    src = torch.ones(args.copy_calibration, dtype=torch.uint8, device="cuda:0")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+      dst.copy_(src)
+    torch.cuda.synchronize()
+    log(f"copied {args.copy_calibration} bytes 3 times")
+    return 0
   if args.gpus != 1:
     print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (worlds shard across GPUs with no exchange)", file=sys.stderr)
     return 2
@@ -159,6 +302,16 @@ def main(argv=None) -> int:
     log("cpu baseline (oracle)")
     baseline = cpu_baseline(mjm, m, d, rc)
 
+  pt_wavefront = learned_denoiser = None
+  if not args.no_pt:
+    del rc, m, d
+    torch.cuda.empty_cache()
+    log("path tracing: wavefront batch (4096 mixed-resolution worlds)")
+    pt_wavefront = pt_wavefront_section(device)
+    torch.cuda.empty_cache()
+    log("path tracing: learned denoiser vs a-trous at equal time")
+    learned_denoiser = learned_section(device)
+
   line = {
     "metric": METRIC,
     "value": round(value, 2),
@@ -184,10 +337,12 @@ def main(argv=None) -> int:
       "peak": HBM_PEAK_GBS,
       "unit": "GB/s",
       "frac": round(achieved / HBM_PEAK_GBS, 6),
-      "traffic": None,
+      "traffic": measured_traffic(args.nworld, args.res),
       "bytes_per_pixel": round(bpp, 3),
     },
     "cpu_baseline": baseline,
+    "pt_wavefront": pt_wavefront,
+    "learned_denoiser": learned_denoiser,
   }
   print(json.dumps(line), flush=True)
   return 0

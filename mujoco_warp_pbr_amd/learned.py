@@ -49,9 +49,16 @@ SIZE_MULTIPLE = 64
 DEFAULT_WEIGHTS = Path(__file__).resolve().parent / "weights" / "unet_denoiser.safetensors"
 
 
-def flops_per_pixel() -> float:
-  """Forward multiply-adds x 2 per full-resolution pixel (padding channels included)."""
-  return sum(2.0 * 9 * cin * cout / (4**level) for _, cin, cout, _, level in LAYERS)
+def flops_per_pixel(padded: bool = True) -> float:
+  """Forward multiply-adds x 2 per full-resolution pixel: as executed (channel padding included)
+  or algorithmic (the 10 used input and 3 used output channels)."""
+  total = 0.0
+  for name, cin, cout, _, level in LAYERS:
+    if not padded:
+      cin = USED_IN if name == "e0a" else cin
+      cout = 3 if name == "out" else cout
+    total += 2.0 * 9 * cin * cout / (4**level)
+  return total
 
 
 def _i16(shape, device):

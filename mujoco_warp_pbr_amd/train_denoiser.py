@@ -172,70 +172,109 @@ def psnr(a: torch.Tensor, b: torch.Tensor) -> float:
 def _time_render(m, d, rc, seed: int, reps: int = 3) -> float:
   import mujoco_warp_pbr_amd as mj
 
-  mj.render(m, d, rc, seed=seed)  # warm (first learned call loads the network)
+  mj.render(m, d, rc, seed=seed)  # warm (the first learned call loads the network)
   st = torch.cuda.current_stream()
   a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
   torch.cuda.synchronize()
   a.record(st)
-  for r in range(reps):
+  for _ in range(reps):
     mj.render(m, d, rc, seed=seed)
   b.record(st)
   torch.cuda.synchronize()
   return a.elapsed_time(b) / reps
 
 
-def evaluate(scene_seeds, nworld: int, res: int, ref_spp: int, device, weights=None, atrous_spp=(1, 2, 3, 4, 6, 8)) -> dict:
-  """PSNR of the learned denoiser and of a-trous at several sample counts against a high-sample
-  reference, with the GPU time of each pipeline (path tracing + filter)."""
+def timings(scene_seed: int, nworld: int, res: int, device, weights=None, atrous_spp=(1, 2, 3, 4, 6, 8)) -> dict:
+  """GPU milliseconds of each pipeline (path tracing + filter) on `nworld` worlds, and of the
+  U-Net forward pass alone."""
+  _, m, d, rc = scene_context(scene_seed, nworld, res, device)
+  rc.denoiser_weights = str(weights) if weights else None
+  out = {"worlds": nworld}
+  rc.denoise, rc.denoiser, rc.path_samples = True, "learned", 1
+  out["learned_ms"] = _time_render(m, d, rc, seed=7)
+  den = learned.denoiser_for(rc)
+  x = den.inputs[0]
+  st = torch.cuda.current_stream()
+  a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+  den.net.forward(x)
+  a.record(st)
+  for _ in range(3):
+    den.net.forward(x)
+  b.record(st)
+  b.synchronize()
+  out["unet_ms"] = a.elapsed_time(b) / 3
+  out["unet_pixels"] = int(x.shape[0] * x.shape[1] * x.shape[2])
+  rc.denoiser = "atrous"
+  out["atrous_ms"] = {}
+  for k in atrous_spp:
+    rc.path_samples = k
+    out["atrous_ms"][k] = _time_render(m, d, rc, seed=7)
+  return out
+
+
+def quality(scene_seed: int, nworld: int, res: int, ref_spp: int, device, weights=None, atrous_spp=(1, 2, 3, 4, 6, 8)) -> dict:
+  """PSNR against a `ref_spp` reference of: 1 sample unfiltered, 1 sample + learned denoiser,
+  k samples + a-trous."""
   import mujoco_warp_pbr_amd as mj
 
+  _, m, d, rc = scene_context(scene_seed, nworld, res, device)
+  rc.denoiser_weights = str(weights) if weights else None
+  rc.denoise, rc.path_samples = False, ref_spp
+  mj.render(m, d, rc, seed=99991)
+  ref = rc.base.rgb_data.clone()
+  row = {"scene_seed": scene_seed, "worlds": nworld, "ref_spp": ref_spp}
+  rc.path_samples = 1
+  mj.render(m, d, rc, seed=7)
+  row["noisy_1spp"] = psnr(rc.base.rgb_data, ref)
+  rc.denoise, rc.denoiser = True, "learned"
+  mj.render(m, d, rc, seed=7)
+  row["learned"] = psnr(rc.base.rgb_data, ref)
+  row["fallback_worlds"] = int(learned.denoiser_for(rc).fallback.sum().item())
+  rc.denoiser = "atrous"
+  row["atrous"] = {}
+  for k in atrous_spp:
+    rc.path_samples = k
+    mj.render(m, d, rc, seed=7)
+    row["atrous"][k] = psnr(rc.base.rgb_data, ref)
+  return row
+
+
+def evaluate(scene_seeds, nworld: int, res: int, ref_spp: int, device, weights=None, atrous_spp=(1, 2, 3, 4, 6, 8),
+             time_worlds: int | None = None) -> dict:
+  """Quality on `nworld` worlds per held-out scene seed and GPU times on `time_worlds` worlds
+  (default nworld) of the first seed; see summarize()."""
   rows = []
   for sd in scene_seeds:
-    _, m, d, rc = scene_context(sd, nworld, res, device)
-    rc.denoiser_weights = str(weights) if weights else None
-    # reference: many samples, no filter
-    rc.denoise = False
-    rc.path_samples = ref_spp
-    mj.render(m, d, rc, seed=99991)
-    ref = rc.base.rgb_data.clone()
-    row = {"scene_seed": sd}
-    rc.path_samples = 1
-    mj.render(m, d, rc, seed=7)
-    row["noisy_1spp"] = psnr(rc.base.rgb_data, ref)
-    rc.denoise = True
-    rc.denoiser = "learned"
-    rc.path_samples = 1
-    row["learned_ms"] = _time_render(m, d, rc, seed=7)
-    mj.render(m, d, rc, seed=7)
-    row["learned"] = psnr(rc.base.rgb_data, ref)
-    fallback = learned.denoiser_for(rc).fallback
-    row["fallback_worlds"] = int(fallback.sum().item())
-    rc.denoiser = "atrous"
-    row["atrous"] = {}
-    for k in atrous_spp:
-      rc.path_samples = k
-      ms = _time_render(m, d, rc, seed=7)
-      mj.render(m, d, rc, seed=7)
-      row["atrous"][k] = {"ms": ms, "psnr": psnr(rc.base.rgb_data, ref)}
+    row = quality(sd, nworld, res, ref_spp, device, weights, atrous_spp)
     rows.append(row)
-    log(f"eval seed {sd}: noisy {row['noisy_1spp']:.2f} dB, learned {row['learned']:.2f} dB ({row['learned_ms']:.1f} ms), "
-        + ", ".join(f"atrous@{k} {v['psnr']:.2f} dB ({v['ms']:.1f} ms)" for k, v in row["atrous"].items()))
-    del rc, m, d
-  return summarize(rows)
+    log(f"eval seed {sd}: noisy {row['noisy_1spp']:.2f} dB, learned {row['learned']:.2f} dB, "
+        + ", ".join(f"atrous@{k} {v:.2f} dB" for k, v in row["atrous"].items()))
+  t = timings(scene_seeds[0], time_worlds or nworld, res, device, weights, atrous_spp)
+  log(f"timing on {t['worlds']} worlds: learned {t['learned_ms']:.2f} ms (U-Net {t['unet_ms']:.2f} ms), "
+      + ", ".join(f"atrous@{k} {v:.2f} ms" for k, v in t["atrous_ms"].items()))
+  return summarize(rows, t)
 
 
-def summarize(rows: list[dict]) -> dict:
-  """Mean PSNRs and times over the evaluated scenes, and the equal-time comparison: a-trous at
-  the smallest sample count whose pipeline time reaches the learned pipeline's."""
+def summarize(rows: list[dict], t: dict) -> dict:
+  """Mean PSNRs over the evaluated scenes and the equal-time comparison: a-trous at the smallest
+  sample count whose pipeline time reaches the learned pipeline's."""
   ks = sorted(rows[0]["atrous"].keys())
-  learned_ms = float(np.mean([r["learned_ms"] for r in rows]))
-  at = {k: (float(np.mean([r["atrous"][k]["ms"] for r in rows])), float(np.mean([r["atrous"][k]["psnr"] for r in rows]))) for k in ks}
+  learned_ms = float(t["learned_ms"])
+  at = {k: (float(t["atrous_ms"][k]), float(np.mean([r["atrous"][k] for r in rows]))) for k in ks}
   equal = next((k for k in ks if at[k][0] >= learned_ms), ks[-1])
+  flops = learned.flops_per_pixel() * t["unet_pixels"]
+  flops_alg = learned.flops_per_pixel(padded=False) * t["unet_pixels"]
   out = {
     "scenes": len(rows),
+    "quality_worlds": rows[0]["worlds"],
+    "ref_spp": rows[0]["ref_spp"],
+    "time_worlds": t["worlds"],
     "noisy_1spp_psnr": float(np.mean([r["noisy_1spp"] for r in rows])),
     "learned_psnr": float(np.mean([r["learned"] for r in rows])),
     "learned_ms": learned_ms,
+    "unet_ms": float(t["unet_ms"]),
+    "unet_tflops": flops / (t["unet_ms"] * 1e-3) / 1e12,
+    "unet_tflops_algorithmic": flops_alg / (t["unet_ms"] * 1e-3) / 1e12,
     "fallback_worlds": int(sum(r["fallback_worlds"] for r in rows)),
     "atrous": {str(k): {"ms": at[k][0], "psnr": at[k][1]} for k in ks},
     "equal_time_atrous_spp": equal,

@@ -5,6 +5,14 @@
  * cpu_baseline leg and __graft_entry__.smoke() as the checker. It is never part of the
  * product path. Built with gcc -O2 -ffp-contract=off (no FMA contraction, no fast math) so
  * that ray/primitive arithmetic is the exact IEEE sequence written.
+ *
+ * Parity unpinned (nothing here fixes these against the reference without importing its
+ * dependencies; they are restated from MuJoCo's documented conventions and are identical in the
+ * oracle and the HIP kernels): the float sequence inside MuJoCo's ray_* primitives and the
+ * RAY_TOL_REL / RAY_TOL_ABS values; the orthographic and intrinsic branches of compute_ray; the
+ * primitive UV conventions of _texture_*; Warp's texture filtering and mip weights (a bilinear /
+ * trilinear sampler of this project's own); Warp's rand_init / randf (the PCG hash of Jarzynski
+ * and Olano, JCGT 2020); the edge behaviour of Warp's watertight mesh query.
  */
 #ifndef ORACLE_COMMON_H_
 #define ORACLE_COMMON_H_

@@ -108,9 +108,11 @@ def test_adam_first_step_is_signed_learning_rate():
 
 def test_layer_tables_agree():
   assert learned.LAYERS == ou.LAYERS
-  # 2 * 9 * sum(cin * cout / 4^level): 0.53 MFLOP per full-resolution pixel (padding included)
+  # 2 * 9 * sum(cin * cout / 4^level): 0.26 MFLOP per full-resolution pixel as executed
   expect = sum(18.0 * cin * cout / 4**lv for _, cin, cout, _, lv in ou.LAYERS)
-  assert learned.flops_per_pixel() == pytest.approx(expect)
+  assert learned.flops_per_pixel() == pytest.approx(expect) == 258048.0
+  # algorithmic: 10 real input channels of e0a, 3 real outputs of the last layer
+  assert learned.flops_per_pixel(padded=False) == pytest.approx(expect - 18.0 * (22 * 32 + 32 * 29))
 
 
 # ------------------------------------------------------------------------------ GPU kernels
