@@ -356,6 +356,11 @@ int mjpbr_extract_rgb_image(const float* src, int src_stride, int adr, int width
  * in FP32 ([pixel][out_f32]). h and w must be multiples of 16, cin of 32, cout of 16. */
 int mjpbr_unet_conv3x3(const uint16_t* x, const uint16_t* mask, const uint16_t* w, const float* bias, uint16_t* y, float* y32,
                        int out_f32, int n, int h, int wd, int cin, int cout, int relu, void* stream);
+/* The same convolution over concat(upsample2x(low), skip) without materialising it: low is
+ * [n][h/2][w/2][c_split] (nearest 2x upsampling), skip [n][h][w][cin - c_split]; c_split a
+ * multiple of 32. The U-Net decoder's input. */
+int mjpbr_unet_conv3x3_upcat(const uint16_t* low, const uint16_t* skip, int c_split, const uint16_t* w, const float* bias,
+                             uint16_t* y, int n, int h, int wd, int cin, int cout, int relu, void* stream);
 /* Floats of the `part` workspace of mjpbr_unet_conv3x3_wgrad with nchunk pixel chunks. */
 int64_t mjpbr_unet_wgrad_workspace(int n, int h, int wd, int cin, int cout, int nchunk);
 /* Weight gradient grad[cout][9][cin] (FP32, overwritten) of a 3x3 convolution with input x and

@@ -33,16 +33,29 @@ MJ_DEV floatx4 mfma16(const uint4& a, const uint4& b, floatx4 c) {
 // ------------------------------------------------------------------ forward / data-gradient conv
 // y[p][n] = act(bias[n] + sum_{tap, c} x[p + off(tap)][c] * w[n][tap][c]), zero padding.
 // With `mask`, the input is x * (mask > 0) (ReLU backward of the layer whose output is `mask`).
-// Workgroup: a 16x16 pixel block (4 waves x 4 rows of 16 pixels) x 64 output channels.
+// With `x2`, the input is concat(upsample2x(x), x2) along channels (the U-Net decoder input):
+// channels [0, c_split) are read from the half-resolution x at (y / 2, x / 2), the rest from x2,
+// so the concatenation is never materialised.
+//
+// Workgroup: 4 waves, a 16x16 pixel block (a wave owns 4 rows of 16 pixels, one MFMA row tile
+// each) x up to 64 output channels (1-4 MFMA column tiles). The K loop runs over 32-channel chunks
+// x 9 taps. The 18x18x32 input patch of the next chunk is loaded into registers while the matrix
+// cores work on the current one and written to LDS after it. LDS is sized to the column tiles in
+// use (45 KB for 32 output channels: 3 workgroups a CU). Epilogue: bias, ReLU, BF16 through LDS
+// so each lane stores 16 bytes.
 constexpr int kConvBlock = 16;
 constexpr int kPatch = kConvBlock + 2;
 constexpr int kCinChunk = 32;
 constexpr int kNTile = 64;
 constexpr int kWRow = 9 * kCinChunk + 8;  // padded LDS row of one output channel's weights
 constexpr int kPStride = kCinChunk + 8;   // padded LDS channel stride of a patch position
+constexpr int kPatchElems = kPatch * kPatch * kPStride;
+constexpr int kPatchVecs = kPatch * kPatch * (kCinChunk / 8);  // 16-byte loads of one patch
+constexpr int kPrefetch = (kPatchVecs + 255) / 256;            // per thread
 
 struct ConvArgs {
   const bf16_t* x;
+  const bf16_t* x2;
   const bf16_t* mask;
   const bf16_t* w;  // [Cout][9][Cin]
   const float* bias;
@@ -51,95 +64,146 @@ struct ConvArgs {
   int n, h, wd, cin, cout;
   int relu;
   int out_f32;
+  int c_split;
 };
 
+MJ_DEV uint4 apply_relu_mask(uint4 v, uint4 mk) {
+  uint32_t* pv = reinterpret_cast<uint32_t*>(&v);
+  const uint32_t* pm = reinterpret_cast<const uint32_t*>(&mk);
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    uint32_t lo = (pm[q] & 0xffffu), hi = pm[q] >> 16;
+    // keep a value where the mask is positive (bf16 sign bit clear and not +0)
+    uint32_t keep_lo = (lo != 0 && !(lo & 0x8000u)) ? 0xffffu : 0u;
+    uint32_t keep_hi = (hi != 0 && !(hi & 0x8000u)) ? 0xffff0000u : 0u;
+    pv[q] &= (keep_lo | keep_hi);
+  }
+  return v;
+}
+
+// 16-byte vector k (< kPatchVecs) of the input patch of block (img, y0, x0), channels c0..c0+31.
+MJ_DEV uint4 load_patch_vec(const ConvArgs& a, int img, int y0, int x0, int c0, int k) {
+  const int pos = k >> 2, part = k & 3;
+  const int yy = y0 - 1 + pos / kPatch, xx = x0 - 1 + pos % kPatch;
+  uint4 v = make_uint4(0, 0, 0, 0);
+  if (yy < 0 || yy >= a.h || xx < 0 || xx >= a.wd) return v;
+  const int c = c0 + part * 8;
+  if (a.x2) {
+    if (c < a.c_split) {
+      const int hl = a.h >> 1, wl = a.wd >> 1;
+      v = *reinterpret_cast<const uint4*>(a.x + (((int64_t)img * hl + (yy >> 1)) * wl + (xx >> 1)) * a.c_split + c);
+    } else {
+      const int c2 = a.cin - a.c_split;
+      v = *reinterpret_cast<const uint4*>(a.x2 + (((int64_t)img * a.h + yy) * a.wd + xx) * c2 + (c - a.c_split));
+    }
+    return v;
+  }
+  const int64_t off = (((int64_t)img * a.h + yy) * a.wd + xx) * a.cin + c;
+  v = *reinterpret_cast<const uint4*>(a.x + off);
+  if (a.mask) v = apply_relu_mask(v, *reinterpret_cast<const uint4*>(a.mask + off));
+  return v;
+}
+
+MJ_DEV void store_patch_vec(bf16_t* patch, int k, uint4 v) {
+  *reinterpret_cast<uint4*>(patch + (k >> 2) * kPStride + (k & 3) * 8) = v;
+}
+
+// NT: MFMA column tiles the accumulators are sized for (1, 2 or 4); the last column group of a
+// launch may use fewer (ntile).
+template <int NT>
 __global__ void __launch_bounds__(256) conv3x3_kernel(ConvArgs a) {
-  __shared__ __attribute__((aligned(16))) bf16_t patch[kPatch * kPatch * kPStride];
-  __shared__ __attribute__((aligned(16))) bf16_t wl[kNTile * kWRow];
+  extern __shared__ __attribute__((aligned(16))) bf16_t conv_lds[];
+  const int n0 = blockIdx.y * NT * 16;
+  const int nn = min(NT * 16, a.cout - n0);
+  const int ntile = nn / 16;
+  bf16_t* patch = conv_lds;
+  bf16_t* wl = conv_lds + kPatchElems;
   const int bx = a.wd / kConvBlock, by = a.h / kConvBlock;
-  int blk = blockIdx.x;
-  const int img = blk / (bx * by);
-  blk -= img * bx * by;
-  const int y0 = (blk / bx) * kConvBlock, x0 = (blk % bx) * kConvBlock;
-  const int n0 = blockIdx.y * kNTile;
-  const int ntile = min(kNTile, a.cout - n0) / 16;  // 1..4 MFMA column tiles
+  const int blk = blockIdx.x;
+  const int img = blk / (bx * by), r = blk % (bx * by);
+  const int y0 = (r / bx) * kConvBlock, x0 = (r % bx) * kConvBlock;
+  const int nchunk = a.cin / kCinChunk;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int lm = lane & 15, lk = lane >> 4;
-  floatx4 acc[4][4];
+
+  uint4 pre[kPrefetch];
+#pragma unroll
+  for (int i = 0; i < kPrefetch; ++i) {
+    const int k = threadIdx.x + 256 * i;
+    pre[i] = k < kPatchVecs ? load_patch_vec(a, img, y0, x0, 0, k) : make_uint4(0, 0, 0, 0);
+  }
+  floatx4 acc[4][NT];
 #pragma unroll
   for (int i = 0; i < 4; ++i)
 #pragma unroll
-    for (int j = 0; j < 4; ++j) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
-
-  for (int c0 = 0; c0 < a.cin; c0 += kCinChunk) {
-    __syncthreads();
-    // stage the input patch (18 x 18 positions x 32 channels), 16 bytes a thread-step
-    for (int e = threadIdx.x; e < kPatch * kPatch * 4; e += 256) {
-      int pos = e >> 2, part = e & 3;
-      int yy = y0 - 1 + pos / kPatch, xx = x0 - 1 + pos % kPatch;
-      uint4 v = make_uint4(0, 0, 0, 0);
-      if (yy >= 0 && yy < a.h && xx >= 0 && xx < a.wd) {
-        int64_t off = (((int64_t)img * a.h + yy) * a.wd + xx) * a.cin + c0 + part * 8;
-        v = *reinterpret_cast<const uint4*>(a.x + off);
-        if (a.mask) {
-          uint4 mk = *reinterpret_cast<const uint4*>(a.mask + (((int64_t)img * a.h + yy) * a.wd + xx) * a.cin + c0 + part * 8);
-          uint32_t* pv = reinterpret_cast<uint32_t*>(&v);
-          const uint32_t* pm = reinterpret_cast<const uint32_t*>(&mk);
+    for (int j = 0; j < NT; ++j) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll 1
+  for (int chunk = 0; chunk < nchunk; ++chunk) {
+    const int c0 = chunk * kCinChunk;
+    __syncthreads();  // the previous chunk's LDS readers are done
 #pragma unroll
-          for (int q = 0; q < 4; ++q) {
-            uint32_t lo = (pm[q] & 0xffffu), hi = pm[q] >> 16;
-            // keep a value where the mask is positive (bf16 sign bit clear and not +0)
-            uint32_t keep_lo = (lo != 0 && !(lo & 0x8000u)) ? 0xffffu : 0u;
-            uint32_t keep_hi = (hi != 0 && !(hi & 0x8000u)) ? 0xffff0000u : 0u;
-            pv[q] &= (keep_lo | keep_hi);
-          }
-        }
-      }
-      *reinterpret_cast<uint4*>(patch + pos * kPStride + part * 8) = v;
+    for (int i = 0; i < kPrefetch; ++i) {
+      const int k = threadIdx.x + 256 * i;
+      if (k < kPatchVecs) store_patch_vec(patch, k, pre[i]);
     }
-    // stage the weights of this input chunk: wl[n][tap][c]
     for (int e = threadIdx.x; e < ntile * 16 * 9 * 4; e += 256) {
       int n = e / 36, rem = e % 36, tap = rem >> 2, part = rem & 3;
       uint4 v = *reinterpret_cast<const uint4*>(a.w + ((int64_t)(n0 + n) * 9 + tap) * a.cin + c0 + part * 8);
       *reinterpret_cast<uint4*>(wl + n * kWRow + tap * kCinChunk + part * 8) = v;
     }
     __syncthreads();
+    if (chunk + 1 < nchunk) {  // the next chunk's patch: global loads in flight during these MFMAs
+#pragma unroll
+      for (int i = 0; i < kPrefetch; ++i) {
+        const int k = threadIdx.x + 256 * i;
+        if (k < kPatchVecs) pre[i] = load_patch_vec(a, img, y0, x0, c0 + kCinChunk, k);
+      }
+    }
 #pragma unroll 1
     for (int tap = 0; tap < 9; ++tap) {
       const int dy = tap / 3, dx = tap % 3;
-      uint4 bq[4];
+      uint4 bq[NT];
 #pragma unroll
-      for (int j = 0; j < 4; ++j)
+      for (int j = 0; j < NT; ++j)
         if (j < ntile) bq[j] = *reinterpret_cast<const uint4*>(wl + (j * 16 + lm) * kWRow + tap * kCinChunk + lk * 8);
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
-        int row = wave * 4 + i;
-        uint4 aq = *reinterpret_cast<const uint4*>(patch + ((row + dy) * kPatch + lm + dx) * kPStride + lk * 8);
+        const int row = wave * 4 + i;
+        const uint4 aq = *reinterpret_cast<const uint4*>(patch + ((row + dy) * kPatch + lm + dx) * kPStride + lk * 8);
 #pragma unroll
-        for (int j = 0; j < 4; ++j)
+        for (int j = 0; j < NT; ++j)
           if (j < ntile) acc[i][j] = mfma16(aq, bq[j], acc[i][j]);
       }
     }
   }
-  // epilogue: bias, ReLU, BF16 (and optional FP32) NHWC store
+  // epilogue: bias, ReLU; optional FP32 store; BF16 tile through LDS, 16-byte global stores
+  const int ostride = nn + 8;
+  bf16_t* otile = conv_lds;  // 256 pixels x nn channels, reuses the patch / weight space
+  __syncthreads();           // every wave finished reading the patch and the weights
 #pragma unroll
   for (int i = 0; i < 4; ++i) {
-    int row = wave * 4 + i;
+    const int row = wave * 4 + i;
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
+    for (int j = 0; j < NT; ++j) {
       if (j >= ntile) continue;
-      int n = n0 + j * 16 + lm;
-      float b = a.bias ? a.bias[n] : 0.0f;
+      const int nl = j * 16 + lm, n = n0 + nl;
+      const float b = a.bias ? a.bias[n] : 0.0f;
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
-        int col = lk * 4 + q;  // pixel within the row
+        const int col = lk * 4 + q;  // pixel within the row
         float v = acc[i][j][q] + b;
         if (a.relu) v = fmaxf(v, 0.0f);
-        int64_t p = ((int64_t)img * a.h + y0 + row) * a.wd + x0 + col;
-        a.y[p * a.cout + n] = f2bf(v);
-        if (a.y32 && n < a.out_f32) a.y32[p * a.out_f32 + n] = v;
+        otile[(row * kConvBlock + col) * ostride + nl] = f2bf(v);
+        if (a.y32 && n < a.out_f32) a.y32[(((int64_t)img * a.h + y0 + row) * a.wd + x0 + col) * a.out_f32 + n] = v;
       }
     }
+  }
+  __syncthreads();
+  const int vecs = nn / 8;  // 16-byte vectors a pixel
+  for (int e = threadIdx.x; e < kConvBlock * kConvBlock * vecs; e += 256) {
+    const int p = e / vecs, part = e - p * vecs;
+    const uint4 v = *reinterpret_cast<const uint4*>(otile + p * ostride + part * 8);
+    *reinterpret_cast<uint4*>(a.y + (((int64_t)img * a.h + y0 + p / kConvBlock) * a.wd + x0 + p % kConvBlock) * a.cout + n0 + part * 8) = v;
   }
 }
 
@@ -521,15 +585,45 @@ namespace {
 inline unsigned nblk(int64_t n, int t) { return (unsigned)((n + t - 1) / t); }
 }  // namespace
 
+namespace {
+int conv_launch(const ConvArgs& a, void* stream) {
+  const int nblocks = a.n * (a.h / kConvBlock) * (a.wd / kConvBlock);
+  const int tiles = a.cout / 16;
+  const int nt = tiles >= 3 ? 4 : tiles;  // accumulator width of the kernel instance
+  const int nn = a.cout < nt * 16 ? a.cout : nt * 16;
+  const size_t weights = (size_t)nn * kWRow;
+  const size_t tile = (size_t)kConvBlock * kConvBlock * (nn + 8);
+  size_t elems = kPatchElems + weights;
+  if (tile > elems) elems = tile;
+  dim3 grid((unsigned)nblocks, (unsigned)((a.cout + nt * 16 - 1) / (nt * 16)));
+  hipStream_t st = (hipStream_t)stream;
+  const size_t lds = elems * sizeof(bf16_t);
+  if (nt == 1)
+    hipLaunchKernelGGL(conv3x3_kernel<1>, grid, dim3(256), lds, st, a);
+  else if (nt == 2)
+    hipLaunchKernelGGL(conv3x3_kernel<2>, grid, dim3(256), lds, st, a);
+  else
+    hipLaunchKernelGGL(conv3x3_kernel<4>, grid, dim3(256), lds, st, a);
+  return check_launch("conv3x3_kernel");
+}
+}  // namespace
+
 extern "C" int mjpbr_unet_conv3x3(const uint16_t* x, const uint16_t* mask, const uint16_t* w, const float* bias, uint16_t* y,
                                   float* y32, int out_f32, int n, int h, int wd, int cin, int cout, int relu, void* stream) {
   if (h % kConvBlock || wd % kConvBlock || cin % kCinChunk || cout % 16 || n < 1)
     return set_error(MJPBR_EINVAL, "unet_conv3x3: need h, w multiples of 16, cin of 32, cout of 16 (got %dx%d, %d -> %d)", h, wd,
                      cin, cout);
-  ConvArgs a{x, mask, w, bias, y, y32, n, h, wd, cin, cout, relu, out_f32};
-  dim3 grid((unsigned)(n * (h / kConvBlock) * (wd / kConvBlock)), (unsigned)((cout + kNTile - 1) / kNTile));
-  hipLaunchKernelGGL(conv3x3_kernel, grid, dim3(256), 0, (hipStream_t)stream, a);
-  return check_launch("conv3x3_kernel");
+  ConvArgs a{x, nullptr, mask, w, bias, y, y32, n, h, wd, cin, cout, relu, out_f32, 0};
+  return conv_launch(a, stream);
+}
+
+extern "C" int mjpbr_unet_conv3x3_upcat(const uint16_t* low, const uint16_t* skip, int c_split, const uint16_t* w, const float* bias,
+                                        uint16_t* y, int n, int h, int wd, int cin, int cout, int relu, void* stream) {
+  if (h % kConvBlock || wd % kConvBlock || cin % kCinChunk || cout % 16 || n < 1 || c_split <= 0 || c_split >= cin ||
+      c_split % kCinChunk || h % 2 || wd % 2)
+    return set_error(MJPBR_EINVAL, "unet_conv3x3_upcat: bad shape %dx%d, %d (%d low) -> %d", h, wd, cin, c_split, cout);
+  ConvArgs a{low, skip, nullptr, w, bias, y, nullptr, n, h, wd, cin, cout, relu, 0, c_split};
+  return conv_launch(a, stream);
 }
 
 extern "C" int64_t mjpbr_unet_wgrad_workspace(int n, int h, int wd, int cin, int cout, int nchunk) {

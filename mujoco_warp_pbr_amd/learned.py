@@ -190,14 +190,25 @@ class UNet:
       p.cout, int(p.relu), _stream(x),
     )
 
-  def forward(self, x: torch.Tensor) -> torch.Tensor:
+  def _conv_upcat(self, name, low, skip, out, c_split):
+    p = self.params[name]
+    n, h, w = skip.shape[0], skip.shape[1], skip.shape[2]
+    _abi.call(
+      "mjpbr_unet_conv3x3_upcat", _abi.ptr(low), _abi.ptr(skip), c_split, _abi.ptr(p.wf), _abi.ptr(p.b), _abi.ptr(out), n, h, w, p.cin,
+      p.cout, int(p.relu), _stream(skip),
+    )
+
+  def forward(self, x: torch.Tensor, training: bool = False) -> torch.Tensor:
     """x: BF16 bits (n, h, w, 32). Returns the FP32 log-space residual (n * h * w, 3), a buffer
-    the next forward of the same shape overwrites."""
+    the next forward of the same shape overwrites. Inference reads the decoders' concatenated
+    inputs straight from their two sources; `training` materialises them for the weight
+    gradients of backward()."""
     n, h, w, c = x.shape
     if c != IN_CHANNELS or x.dtype != torch.int16 or not x.is_contiguous():
       raise ValueError(f"input must be contiguous int16 (BF16 bits) with {IN_CHANNELS} channels, got {x.dtype} {tuple(x.shape)}")
     bufs = self._alloc(n, h, w)
     self.x = x
+    self._trained_forward = training
     A = bufs["act"]
     st = _stream(x)
     self._conv("e0a", x, A["e0a"])
@@ -208,11 +219,17 @@ class UNet:
     _abi.call("mjpbr_unet_avgpool2", _abi.ptr(A["e1b"]), _abi.ptr(A["p1"]), n, h // 2, w // 2, 64, 0, st)
     self._conv("b0", A["p1"], A["b0"])
     self._conv("b1", A["b0"], A["b1"])
-    _abi.call("mjpbr_unet_upcat", _abi.ptr(A["b1"]), _abi.ptr(A["e1b"]), _abi.ptr(A["u1"]), n, h // 2, w // 2, 128, 64, st)
-    self._conv("d1a", A["u1"], A["d1a"])
+    if training:
+      _abi.call("mjpbr_unet_upcat", _abi.ptr(A["b1"]), _abi.ptr(A["e1b"]), _abi.ptr(A["u1"]), n, h // 2, w // 2, 128, 64, st)
+      self._conv("d1a", A["u1"], A["d1a"])
+    else:
+      self._conv_upcat("d1a", A["b1"], A["e1b"], A["d1a"], 128)
     self._conv("d1b", A["d1a"], A["d1b"])
-    _abi.call("mjpbr_unet_upcat", _abi.ptr(A["d1b"]), _abi.ptr(A["e0b"]), _abi.ptr(A["u0"]), n, h, w, 64, 32, st)
-    self._conv("d0a", A["u0"], A["d0a"])
+    if training:
+      _abi.call("mjpbr_unet_upcat", _abi.ptr(A["d1b"]), _abi.ptr(A["e0b"]), _abi.ptr(A["u0"]), n, h, w, 64, 32, st)
+      self._conv("d0a", A["u0"], A["d0a"])
+    else:
+      self._conv_upcat("d0a", A["d1b"], A["e0b"], A["d0a"], 64)
     self._conv("d0b", A["d0a"], A["d0b"])
     self._conv("out", A["d0b"], A["out"], y32=bufs["pred"], out_f32=3)
     return bufs["pred"]
@@ -243,6 +260,8 @@ class UNet:
   def backward(self, target: torch.Tensor) -> torch.Tensor:
     """L1 loss of the last forward against FP32 log-space targets (n * h * w, 3); fills every
     parameter gradient. Returns the loss (a device scalar)."""
+    if not getattr(self, "_trained_forward", False):
+      raise RuntimeError("backward() needs the activations of forward(x, training=True)")
     n, h, w = self._shape
     bufs = self._bufs[self._shape]
     A, G = bufs["act"], self._grad_bufs()

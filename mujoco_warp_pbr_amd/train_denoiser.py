@@ -143,7 +143,7 @@ def train(net: learned.UNet, X: torch.Tensor, T: torch.Tensor, steps: int, batch
       xb, tb = torch.flip(xb, dims=(2,)), torch.flip(tb, dims=(2,))
     xb = xb.contiguous()
     tb = tb.reshape(-1, 3).contiguous()
-    net.forward(xb)
+    net.forward(xb, training=True)
     loss = net.backward(tb)
     frac = step / max(steps - 1, 1)
     net.adam_step(lr=lr * (0.05 + 0.95 * 0.5 * (1.0 + math.cos(math.pi * frac))))

@@ -266,7 +266,7 @@ def test_gpu_network_forward_and_backward_match_oracle(cuda):
   x, target = _inputs(rng, n, h, w)
   net = learned.UNet(cuda)
   _upload(net, params)
-  pred = net.forward(_bits(x, cuda))
+  pred = net.forward(_bits(x, cuda), training=True)
   loss = net.backward(torch.from_numpy(target.reshape(-1, 3).astype(np.float32)).to(cuda))
   torch.cuda.synchronize()
   ref_net = ou.Net(params, bf16=True)
@@ -293,7 +293,7 @@ def test_gpu_adam_matches_oracle_and_training_reduces_loss(cuda):
   x, target = _inputs(rng, 2, 64, 64)
   xb = _bits(x, cuda)
   tt = torch.from_numpy(target.reshape(-1, 3).astype(np.float32)).to(cuda)
-  net.forward(xb)
+  net.forward(xb, training=True)
   first = float(net.backward(tt))
   g = {k: (p.gw.cpu().numpy().astype(np.float64), p.gb.cpu().numpy().astype(np.float64)) for k, p in net.params.items()}
   w0 = {k: p.w.cpu().numpy().astype(np.float64) for k, p in net.params.items()}
@@ -304,7 +304,45 @@ def test_gpu_adam_matches_oracle_and_training_reduces_loss(cuda):
     np.testing.assert_allclose(p.w.cpu().numpy(), want, rtol=1e-5, atol=1e-7)
     np.testing.assert_array_equal(p.wf.cpu().numpy(), ou.to_bits(ou.bf16_round(p.w.cpu().numpy())))
   for _ in range(100):
-    net.forward(xb)
+    net.forward(xb, training=True)
     last = float(net.backward(tt))
     net.adam_step(lr=1e-3)
   assert last < 0.7 * first, (first, last)
+
+
+@pytest.mark.gpu
+def test_gpu_fused_upcat_conv_matches_materialised_concat(cuda):
+  """The decoder convolution reading concat(upsample2x(low), skip) from its two sources equals
+  the oracle and, bit for bit, the convolution of the materialised concatenation."""
+  rng = np.random.default_rng(60)
+  n, h, w, cl, cs, cout = 2, 32, 32, 64, 32, 32
+  low = _rand_bf16(rng, (n, h // 2, w // 2, cl))
+  skip = _rand_bf16(rng, (n, h, w, cs))
+  wt = _rand_bf16(rng, (cout, 9, cl + cs), 0.1)
+  bias = rng.normal(0, 0.1, cout).astype(np.float32)
+  tl, tsk, tw = _bits(low, cuda), _bits(skip, cuda), _bits(wt, cuda)
+  tb = torch.from_numpy(bias).to(cuda)
+  fused = torch.zeros((n, h, w, cout), dtype=torch.int16, device=cuda)
+  _abi.call("mjpbr_unet_conv3x3_upcat", _abi.ptr(tl), _abi.ptr(tsk), cl, _abi.ptr(tw), _abi.ptr(tb), _abi.ptr(fused), n, h, w, cl + cs, cout, 1,
+            _stream())
+  cat = torch.zeros((n, h, w, cl + cs), dtype=torch.int16, device=cuda)
+  _abi.call("mjpbr_unet_upcat", _abi.ptr(tl), _abi.ptr(tsk), _abi.ptr(cat), n, h, w, cl, cs, _stream())
+  plain = torch.zeros_like(fused)
+  _abi.call("mjpbr_unet_conv3x3", _abi.ptr(cat), 0, _abi.ptr(tw), _abi.ptr(tb), _abi.ptr(plain), 0, 0, n, h, w, cl + cs, cout, 1, _stream())
+  torch.cuda.synchronize()
+  np.testing.assert_array_equal(fused.cpu().numpy(), plain.cpu().numpy())
+  ref = np.maximum(ou.conv3x3(ou.upcat(low, skip), wt, bias.astype(np.float64)), 0.0)
+  got = _vals(fused)
+  assert np.all(np.abs(got - ou.bf16_round(ref)) <= np.abs(ref) * 2.0**-7 + 1e-3)
+
+
+@pytest.mark.gpu
+def test_gpu_inference_forward_equals_training_forward(cuda):
+  rng = np.random.default_rng(61)
+  net = learned.UNet(cuda, seed=5)
+  x, _ = _inputs(rng, 2, 64, 128)
+  xb = _bits(x, cuda)
+  a = net.forward(xb, training=True).clone()
+  b = net.forward(xb).clone()
+  torch.cuda.synchronize()
+  assert torch.equal(a, b)
