@@ -153,6 +153,12 @@ typedef struct MjpbrCameras {
   const float* cam_intrinsic;       /* [n, ncam, 4] fx, fy, cx, cy (length units) */
   const float* cam_xpos;            /* [nworld, ncam, 3] */
   const float* cam_xmat;            /* [nworld, ncam, 9] */
+  /* Ray bundles (reference rays.py:84-143, camera.py:194-347): NULL renders the pinhole cameras;
+   * else one camera-frame ray a pixel, [total_rays, 3] each, rendering camera i's rays at its
+   * pixel offset (the sum of the earlier cameras' pixels), shared by all worlds. A zero direction
+   * marks a pixel without a ray. */
+  const float* ray_origin;
+  const float* ray_direction;
 } MjpbrCameras;
 
 /* Environment lighting (reference: kernel_inputs.py:172-187 `Environment`). */

@@ -23,6 +23,8 @@ from mujoco_warp_pbr_amd.output import get_hdr
 from mujoco_warp_pbr_amd.output import get_normal
 from mujoco_warp_pbr_amd.output import get_rgb
 from mujoco_warp_pbr_amd.output import get_segmentation
+from mujoco_warp_pbr_amd.rays import CameraRays
+from mujoco_warp_pbr_amd.rays import set_camera_rays
 from mujoco_warp_pbr_amd.render import render
 from mujoco_warp_pbr_amd.bvh import refit_bvh
 from mujoco_warp_pbr_amd.types import LightModel
@@ -36,6 +38,7 @@ from mujoco_warp_pbr_amd.wavefront import WavefrontConfig
 from mujoco_warp_pbr_amd.wavefront import render_batch
 
 __all__ = [
+  "CameraRays",
   "HostModel",
   "LightModel",
   "PbrRenderContext",
@@ -58,4 +61,5 @@ __all__ = [
   "refit_bvh",
   "render",
   "render_batch",
+  "set_camera_rays",
 ]

@@ -28,6 +28,8 @@ struct Pixel {
   float intr[4];
   float xmat[9];
   V3 xpos;
+  const float* rorg;  // ray bundle of the camera (NULL: pinhole), from its first pixel
+  const float* rdir;
 };
 
 // Reference ray id of pixel (px, py): the thread index of `camera.thread_pixel` under the render
@@ -80,9 +82,101 @@ MJ_DEV void camera_frame_ray(const Pixel& p, int subdiv, int sx, int sy, V3* dir
   *off = v3s(0.0f);
 }
 
+// ---------------------------------------------------------------- ray bundles (camera.py:194-347)
+// Bundle rays (x0, x1) and weight t of sample i of n along an image axis of nb bundle rays: sample
+// i sits at bundle coordinate (i + 0.5) nb / n - 0.5 = num / den, in integers so that t is exactly
+// 0 on a bundle pixel centre.
+MJ_DEV void bundle_axis(int i, int n, int nb, int* x0, int* x1, float* t) {
+  const int64_t num = (int64_t)(2 * i + 1) * nb - n;
+  const int64_t den = 2 * (int64_t)n;
+  int a = (int)((num + den) / den) - 1;
+  a = min(max(a, 0), max(nb - 2, 0));
+  *x0 = a;
+  *x1 = min(a + 1, nb - 1);
+  *t = (float)(num - (int64_t)a * den) / (float)den;
+}
+
+MJ_DEV V3 lerp3(V3 a, V3 b, float t) { return a + (b - a) * t; }
+
+// Camera-frame (direction, origin) through sub-pixel cell (sx, sy): bilinear between the four
+// nearest bundle rays (on the image plane z = -1 when all point forward), the nearest ray next to
+// a pixel without one. The direction is normalised, or zero.
+MJ_DEV void bundle_ray(const Pixel& p, int subdiv, int sx, int sy, V3* dir, V3* org) {
+  int x0, x1, y0, y1;
+  float tx, ty;
+  bundle_axis(p.px * subdiv + sx, p.width * subdiv, p.width, &x0, &x1, &tx);
+  bundle_axis(p.py * subdiv + sy, p.height * subdiv, p.height, &y0, &y1, &ty);
+  const int r0 = y0 * p.width, r1 = y1 * p.width;
+  V3 d00 = ld3(p.rdir + 3 * (r0 + x0)), d10 = ld3(p.rdir + 3 * (r0 + x1));
+  V3 d01 = ld3(p.rdir + 3 * (r1 + x0)), d11 = ld3(p.rdir + 3 * (r1 + x1));
+  if (len_sq(d00) == 0.0f || len_sq(d10) == 0.0f || len_sq(d01) == 0.0f || len_sq(d11) == 0.0f) {
+    const int near = (ty < 0.5f ? r0 : r1) + (tx < 0.5f ? x0 : x1);
+    V3 d = ld3(p.rdir + 3 * near);
+    *dir = len_sq(d) > 0.0f ? normalize(d) : d;
+    *org = ld3(p.rorg + 3 * near);
+    return;
+  }
+  V3 o00 = ld3(p.rorg + 3 * (r0 + x0)), o10 = ld3(p.rorg + 3 * (r0 + x1));
+  V3 o01 = ld3(p.rorg + 3 * (r1 + x0)), o11 = ld3(p.rorg + 3 * (r1 + x1));
+  if (d00.z < 0.0f && d10.z < 0.0f && d01.z < 0.0f && d11.z < 0.0f) {
+    d00 = d00 * (-1.0f / d00.z);
+    d10 = d10 * (-1.0f / d10.z);
+    d01 = d01 * (-1.0f / d01.z);
+    d11 = d11 * (-1.0f / d11.z);
+  }
+  V3 d = lerp3(lerp3(d00, d10, tx), lerp3(d01, d11, tx), ty);
+  *org = lerp3(lerp3(o00, o10, tx), lerp3(o01, o11, tx), ty);
+  *dir = len_sq(d) > 0.0f ? normalize(d) : d;
+}
+
+// (angle, distance) between bundle rays a and b, or between b and its linear extrapolation.
+MJ_DEV void bundle_step(const Pixel& p, int a, int b, bool extrapolate, float* spread, float* width) {
+  V3 da = ld3(p.rdir + 3 * a), db = ld3(p.rdir + 3 * b);
+  *spread = 0.0f;
+  *width = 0.0f;
+  if (len_sq(da) == 0.0f || len_sq(db) == 0.0f) return;
+  da = normalize(da);
+  db = normalize(db);
+  *width = len(ld3(p.rorg + 3 * b) - ld3(p.rorg + 3 * a));
+  if (extrapolate) {
+    if (da.z < 0.0f && db.z < 0.0f)
+      *spread = len(normalize(db * (-2.0f / db.z) + da * (1.0f / da.z)) - db);
+    else
+      *spread = len(normalize(db * 2.0f - da) - db);
+    return;
+  }
+  *spread = len(db - da);
+}
+
+MJ_DEV void bundle_cone(const Pixel& p, float* spread, float* width) {
+  *spread = 0.0f;
+  *width = 0.0f;
+  if (p.width < 2) return;
+  const int b = p.py * p.width + p.px;
+  if (p.px + 1 < p.width)
+    bundle_step(p, b, b + 1, false, spread, width);
+  else
+    bundle_step(p, b - 1, b, true, spread, width);
+}
+
+// Depth axis of bundle rays: depth is the distance along the ray.
+constexpr V3 kAlongRay = V3{0.0f, 0.0f, -1.0f};
+
 MJ_DEV void primary_ray(const Pixel& p, const MjpbrSettings& s, int subdiv, int sx, int sy, V3* o, V3* d,
                         float* max_dist, V3* dir_cam) {
   V3 dc, oc;
+  if (p.rdir) {
+    bundle_ray(p, subdiv, sx, sy, &dc, &oc);
+    *max_dist = s.zfar > 0.0f ? s.zfar : kMaxVal;
+    if (len_sq(dc) == 0.0f) {
+      *max_dist = 0.0f;
+      dc = kAlongRay;
+    }
+    *o = p.xpos + mat_mul(p.xmat, oc);
+    *d = mat_mul(p.xmat, dc);
+    *dir_cam = kAlongRay;
+    return;
+  }
   camera_frame_ray(p, subdiv, sx, sy, &dc, &oc);
   *o = p.xpos + mat_mul(p.xmat, oc);
   *d = mat_mul(p.xmat, dc);
@@ -91,6 +185,10 @@ MJ_DEV void primary_ray(const Pixel& p, const MjpbrSettings& s, int subdiv, int 
 }
 
 MJ_DEV void pixel_cone(const Pixel& p, float* spread, float* width) {
+  if (p.rdir) {
+    bundle_cone(p, spread, width);
+    return;
+  }
   V3 d0, o0, d1, o1;
   camera_frame_ray(p, 1, 0, 0, &d0, &o0);
   camera_frame_ray(p, 1, 1, 0, &d1, &o1);
@@ -126,6 +224,8 @@ MJ_DEV bool setup_pixel(const MjpbrCameras& cams, const MjpbrOutputs& out, const
 #pragma unroll
   for (int k = 0; k < 9; ++k) p->xmat[k] = xm[k];
   p->xpos = ld3(cams.cam_xpos + ((int64_t)w * cams.ncam + cam) * 3);
+  p->rorg = cams.ray_direction ? cams.ray_origin + 3 * (int64_t)ray_offset : nullptr;
+  p->rdir = cams.ray_direction ? cams.ray_direction + 3 * (int64_t)ray_offset : nullptr;
   return true;
 }
 

@@ -195,8 +195,6 @@ def create_render_context(
     _check_at_least("max_registers", max_registers, 0)
   if nworld < 1:
     raise ValueError(f"nworld must be at least 1, got {nworld}")
-  if camera_rays is not None:
-    raise ValueError("camera_rays (ray bundles) are not supported by this renderer yet")
   if traversal == "optix":
     warnings.warn('traversal="optix" falls back to "software": no OptiX on this device', stacklevel=2)
   traversal = "software"
@@ -390,6 +388,10 @@ def create_render_context(
     denoiser=denoiser,
     denoiser_weights=denoiser_weights,
   )
+  if camera_rays is not None:
+    from mujoco_warp_pbr_amd.rays import set_camera_rays
+
+    set_camera_rays(rc, camera_rays)
   return rc
 
 

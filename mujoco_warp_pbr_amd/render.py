@@ -148,6 +148,8 @@ def scene_struct(m, d, rc: PbrRenderContext):
   cm.cam_intrinsic = _abi.ptr(m.cam_intrinsic)
   cm.cam_xpos = _abi.ptr(_check(d.cam_xpos, torch.float32, "cam_xpos"))
   cm.cam_xmat = _abi.ptr(_check(d.cam_xmat, torch.float32, "cam_xmat"))
+  cm.ray_origin = _abi.ptr(rc.ray_origin)
+  cm.ray_direction = _abi.ptr(rc.ray_direction)
   # environment
   ev = sc.env
   env = rc.env

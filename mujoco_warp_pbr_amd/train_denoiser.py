@@ -44,12 +44,25 @@ def log(msg: str) -> None:
 # ----------------------------------------------------------------------------- scenes
 
 
-def scene_context(scene_seed: int, nworld: int, res: int, device, max_depth: int = 4, rr_depth: int = 3):
+SCENE_KINDS = ("full", "mesh")
+
+
+def make_scene(kind: str, scene_seed: int, res: int):
+  """The full-feature scene (mirror plane, 12 primitives, 4 convex hulls) or the mesh-heavy one
+  (64 instances of 16 procedural meshes, ~200k triangles)."""
+  if kind == "full":
+    return scenes.full_scene(seed=scene_seed, res=res)
+  if kind == "mesh":
+    return scenes.mesh_scene(seed=scene_seed, res=res)
+  raise ValueError(f"scene kind must be one of {SCENE_KINDS}, got {kind!r}")
+
+
+def scene_context(scene_seed: int, nworld: int, res: int, device, max_depth: int = 4, rr_depth: int = 3, kind: str = "full"):
   """(host model, model, data, context) of `nworld` randomized worlds of one scene seed, path
   traced with first-hit AOVs (denoise=True)."""
   import mujoco_warp_pbr_amd as mj
 
-  mjm = scenes.full_scene(seed=scene_seed, res=res)
+  mjm = make_scene(kind, scene_seed, res)
   m = mj.put_model(mjm, device)
   d = mj.put_data(mjm, nworld, device)
   rc = mj.create_render_context(
@@ -101,18 +114,18 @@ def render_pair(m, d, rc, nworld: int, target_spp: int, seed: int):
   return x, target
 
 
-def generate(scene_seeds, nworld: int, res: int, target_spp: int, device):
-  """Stacked training pairs of several scene seeds (resident in HBM)."""
+def generate(jobs, res: int, target_spp: int, device):
+  """Stacked training pairs (resident in HBM) of jobs [(scene kind, scene seed, worlds), ...]."""
   xs, ts = [], []
-  for k, sd in enumerate(scene_seeds):
+  for k, (kind, sd, nworld) in enumerate(jobs):
     t0 = time.perf_counter()
-    _, m, d, rc = scene_context(sd, nworld, res, device)
+    _, m, d, rc = scene_context(sd, nworld, res, device, kind=kind)
     x, t = render_pair(m, d, rc, nworld, target_spp, seed=17 * sd + 5)
     torch.cuda.synchronize()
     xs.append(x.clone())
     ts.append(t.view(nworld, res * res, 3).clone())
     del rc, m, d
-    log(f"scene {k + 1}/{len(scene_seeds)} (seed {sd}): {nworld} worlds at {target_spp} spp in {time.perf_counter() - t0:.1f} s")
+    log(f"scene {k + 1}/{len(jobs)} ({kind} seed {sd}): {nworld} worlds at {target_spp} spp in {time.perf_counter() - t0:.1f} s")
   return torch.cat(xs), torch.cat(ts)
 
 
@@ -159,14 +172,34 @@ def train(net: learned.UNet, X: torch.Tensor, T: torch.Tensor, steps: int, batch
 # ----------------------------------------------------------------------------- evaluation
 
 
+def _rgb(t: torch.Tensor) -> torch.Tensor:
+  t = t.to(torch.int64)
+  return torch.stack([(t >> s) & 255 for s in (0, 8, 16)], -1).to(torch.float32) / 255.0
+
+
 def psnr(a: torch.Tensor, b: torch.Tensor) -> float:
   """PSNR in dB of two packed sRGB8 images (int32 bits R | G << 8 | B << 16), over RGB."""
-  def rgb(t):
-    t = t.to(torch.int64)
-    return torch.stack([(t >> s) & 255 for s in (0, 8, 16)], -1).to(torch.float64) / 255.0
-
-  mse = torch.mean((rgb(a) - rgb(b)) ** 2).item()
+  mse = torch.mean((_rgb(a).double() - _rgb(b).double()) ** 2).item()
   return 10.0 * math.log10(1.0 / max(mse, 1e-12))
+
+
+def ssim(a: torch.Tensor, b: torch.Tensor, width: int, height: int) -> float:
+  """Mean SSIM (Wang et al. 2004: 11x11 Gaussian window, sigma 1.5, K1 0.01, K2 0.03, range 1)
+  over the RGB channels of packed sRGB8 images [nworld, height * width]."""
+  x = _rgb(a).reshape(-1, height, width, 3).permute(0, 3, 1, 2).reshape(-1, 1, height, width)
+  y = _rgb(b).reshape(-1, height, width, 3).permute(0, 3, 1, 2).reshape(-1, 1, height, width)
+  g = torch.exp(-((torch.arange(11, device=x.device, dtype=torch.float32) - 5.0) ** 2) / (2 * 1.5**2))
+  g = g / g.sum()
+  win = (g[:, None] * g[None, :])[None, None]
+
+  def blur(t):
+    return torch.nn.functional.conv2d(t, win)
+
+  mx, my = blur(x), blur(y)
+  sxx, syy, sxy = blur(x * x) - mx * mx, blur(y * y) - my * my, blur(x * y) - mx * my
+  c1, c2 = 0.01**2, 0.03**2
+  m = ((2 * mx * my + c1) * (2 * sxy + c2)) / ((mx * mx + my * my + c1) * (sxx + syy + c2))
+  return float(m.mean().item())
 
 
 def _time_render(m, d, rc, seed: int, reps: int = 3) -> float:
@@ -184,10 +217,10 @@ def _time_render(m, d, rc, seed: int, reps: int = 3) -> float:
   return a.elapsed_time(b) / reps
 
 
-def timings(scene_seed: int, nworld: int, res: int, device, weights=None, atrous_spp=(1, 2, 3, 4, 6, 8)) -> dict:
+def timings(scene_seed: int, nworld: int, res: int, device, weights=None, atrous_spp=(1, 2, 3, 4, 6, 8), kind: str = "full") -> dict:
   """GPU milliseconds of each pipeline (path tracing + filter) on `nworld` worlds, and of the
   U-Net forward pass alone."""
-  _, m, d, rc = scene_context(scene_seed, nworld, res, device)
+  _, m, d, rc = scene_context(scene_seed, nworld, res, device, kind=kind)
   rc.denoiser_weights = str(weights) if weights else None
   out = {"worlds": nworld}
   rc.denoise, rc.denoiser, rc.path_samples = True, "learned", 1
@@ -212,44 +245,49 @@ def timings(scene_seed: int, nworld: int, res: int, device, weights=None, atrous
   return out
 
 
-def quality(scene_seed: int, nworld: int, res: int, ref_spp: int, device, weights=None, atrous_spp=(1, 2, 3, 4, 6, 8)) -> dict:
-  """PSNR against a `ref_spp` reference of: 1 sample unfiltered, 1 sample + learned denoiser,
-  k samples + a-trous."""
+def quality(scene_seed: int, nworld: int, res: int, ref_spp: int, device, weights=None, atrous_spp=(1, 2, 3, 4, 6, 8),
+            kind: str = "full") -> dict:
+  """PSNR and SSIM against a `ref_spp` reference of: 1 sample unfiltered, 1 sample + learned
+  denoiser, k samples + a-trous."""
   import mujoco_warp_pbr_amd as mj
 
-  _, m, d, rc = scene_context(scene_seed, nworld, res, device)
+  _, m, d, rc = scene_context(scene_seed, nworld, res, device, kind=kind)
   rc.denoiser_weights = str(weights) if weights else None
   rc.denoise, rc.path_samples = False, ref_spp
   mj.render(m, d, rc, seed=99991)
   ref = rc.base.rgb_data.clone()
-  row = {"scene_seed": scene_seed, "worlds": nworld, "ref_spp": ref_spp}
+  row = {"scene_seed": scene_seed, "kind": kind, "worlds": nworld, "ref_spp": ref_spp}
+
+  def score():
+    return psnr(rc.base.rgb_data, ref), ssim(rc.base.rgb_data, ref, res, res)
+
   rc.path_samples = 1
   mj.render(m, d, rc, seed=7)
-  row["noisy_1spp"] = psnr(rc.base.rgb_data, ref)
+  row["noisy_1spp"], row["noisy_1spp_ssim"] = score()
   rc.denoise, rc.denoiser = True, "learned"
   mj.render(m, d, rc, seed=7)
-  row["learned"] = psnr(rc.base.rgb_data, ref)
+  row["learned"], row["learned_ssim"] = score()
   row["fallback_worlds"] = int(learned.denoiser_for(rc).fallback.sum().item())
   rc.denoiser = "atrous"
-  row["atrous"] = {}
+  row["atrous"], row["atrous_ssim"] = {}, {}
   for k in atrous_spp:
     rc.path_samples = k
     mj.render(m, d, rc, seed=7)
-    row["atrous"][k] = psnr(rc.base.rgb_data, ref)
+    row["atrous"][k], row["atrous_ssim"][k] = score()
   return row
 
 
 def evaluate(scene_seeds, nworld: int, res: int, ref_spp: int, device, weights=None, atrous_spp=(1, 2, 3, 4, 6, 8),
-             time_worlds: int | None = None) -> dict:
+             time_worlds: int | None = None, kind: str = "full") -> dict:
   """Quality on `nworld` worlds per held-out scene seed and GPU times on `time_worlds` worlds
   (default nworld) of the first seed; see summarize()."""
   rows = []
   for sd in scene_seeds:
-    row = quality(sd, nworld, res, ref_spp, device, weights, atrous_spp)
+    row = quality(sd, nworld, res, ref_spp, device, weights, atrous_spp, kind=kind)
     rows.append(row)
     log(f"eval seed {sd}: noisy {row['noisy_1spp']:.2f} dB, learned {row['learned']:.2f} dB, "
         + ", ".join(f"atrous@{k} {v:.2f} dB" for k, v in row["atrous"].items()))
-  t = timings(scene_seeds[0], time_worlds or nworld, res, device, weights, atrous_spp)
+  t = timings(scene_seeds[0], time_worlds or nworld, res, device, weights, atrous_spp, kind=kind)
   log(f"timing on {t['worlds']} worlds: learned {t['learned_ms']:.2f} ms (U-Net {t['unet_ms']:.2f} ms), "
       + ", ".join(f"atrous@{k} {v:.2f} ms" for k, v in t["atrous_ms"].items()))
   return summarize(rows, t)
@@ -264,21 +302,26 @@ def summarize(rows: list[dict], t: dict) -> dict:
   equal = next((k for k in ks if at[k][0] >= learned_ms), ks[-1])
   flops = learned.flops_per_pixel() * t["unet_pixels"]
   flops_alg = learned.flops_per_pixel(padded=False) * t["unet_pixels"]
+  ssim_at = {k: float(np.mean([r["atrous_ssim"][k] for r in rows])) for k in ks}
   out = {
+    "kind": rows[0]["kind"],
     "scenes": len(rows),
     "quality_worlds": rows[0]["worlds"],
     "ref_spp": rows[0]["ref_spp"],
     "time_worlds": t["worlds"],
     "noisy_1spp_psnr": float(np.mean([r["noisy_1spp"] for r in rows])),
     "learned_psnr": float(np.mean([r["learned"] for r in rows])),
+    "learned_ssim": float(np.mean([r["learned_ssim"] for r in rows])),
+    "noisy_1spp_ssim": float(np.mean([r["noisy_1spp_ssim"] for r in rows])),
     "learned_ms": learned_ms,
     "unet_ms": float(t["unet_ms"]),
     "unet_tflops": flops / (t["unet_ms"] * 1e-3) / 1e12,
     "unet_tflops_algorithmic": flops_alg / (t["unet_ms"] * 1e-3) / 1e12,
     "fallback_worlds": int(sum(r["fallback_worlds"] for r in rows)),
-    "atrous": {str(k): {"ms": at[k][0], "psnr": at[k][1]} for k in ks},
+    "atrous": {str(k): {"ms": at[k][0], "psnr": at[k][1], "ssim": ssim_at[k]} for k in ks},
     "equal_time_atrous_spp": equal,
     "equal_time_atrous_psnr": at[equal][1],
+    "equal_time_atrous_ssim": ssim_at[equal],
     "equal_time_atrous_ms": at[equal][0],
   }
   out["gain_db"] = out["learned_psnr"] - out["equal_time_atrous_psnr"]
@@ -290,8 +333,12 @@ def summarize(rows: list[dict], t: dict) -> dict:
 
 def main(argv=None) -> int:
   p = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
-  p.add_argument("--scenes", type=int, default=16, help="training scene seeds")
-  p.add_argument("--worlds", type=int, default=256, help="randomized worlds per scene seed")
+  p.add_argument("--scenes", type=int, default=16, help="training seeds of the full-feature scene")
+  p.add_argument("--worlds", type=int, default=256, help="randomized worlds per full-feature seed")
+  p.add_argument("--mesh-scenes", type=int, default=0, help="training seeds of the mesh-heavy scene")
+  p.add_argument("--mesh-worlds", type=int, default=128, help="randomized worlds per mesh-heavy seed")
+  p.add_argument("--eval-mesh-res", type=int, default=0, help="also evaluate the mesh-heavy scene at this resolution")
+  p.add_argument("--eval-mesh-time-worlds", type=int, default=1024)
   p.add_argument("--res", type=int, default=128)
   p.add_argument("--target-spp", type=int, default=256)
   p.add_argument("--steps", type=int, default=3000)
@@ -307,8 +354,12 @@ def main(argv=None) -> int:
   dev = torch.device("cuda:0")
   t0 = time.perf_counter()
   train_seeds = list(range(100, 100 + args.scenes))
-  log(f"generating {args.scenes} x {args.worlds} pairs at {args.res}^2 ({args.target_spp}-spp targets)")
-  X, T = generate(train_seeds, args.worlds, args.res, args.target_spp, dev)
+  mesh_seeds = list(range(300, 300 + args.mesh_scenes))
+  jobs = [("full", sd, args.worlds) for sd in train_seeds] + [("mesh", sd, args.mesh_worlds) for sd in mesh_seeds]
+  # interleave the kinds so an early stop still trains on both
+  jobs.sort(key=lambda j: (j[1] - (100 if j[0] == "full" else 300)) / max(args.scenes if j[0] == "full" else args.mesh_scenes, 1))
+  log(f"generating {len(jobs)} scene seeds of pairs at {args.res}^2 ({args.target_spp}-spp targets)")
+  X, T = generate(jobs, args.res, args.target_spp, dev)
   gen_s = time.perf_counter() - t0
   stats = guide_stats(X)
   net = learned.UNet(dev, seed=args.seed)
@@ -320,8 +371,10 @@ def main(argv=None) -> int:
   meta = {
     "guide_range": learned.guide_range_from_stats(stats),
     "train": {
-      "scene_seeds": [train_seeds[0], train_seeds[-1]],
+      "full_scene_seeds": [train_seeds[0], train_seeds[-1]] if train_seeds else [],
       "worlds_per_scene": args.worlds,
+      "mesh_scene_seeds": [mesh_seeds[0], mesh_seeds[-1]] if mesh_seeds else [],
+      "worlds_per_mesh_scene": args.mesh_worlds,
       "images": int(X.shape[0]),
       "res": args.res,
       "target_spp": args.target_spp,
@@ -343,6 +396,12 @@ def main(argv=None) -> int:
     log(f"evaluating on scene seeds {eval_seeds} ({args.eval_worlds} worlds, {args.ref_spp}-spp reference)")
     result["eval"] = evaluate(eval_seeds, args.eval_worlds, args.res, args.ref_spp, dev, weights=args.out)
     meta["eval"] = result["eval"]
+    learned.save(net, args.out, meta)
+  if args.eval_mesh_res > 0:
+    log(f"evaluating the mesh-heavy scene at {args.eval_mesh_res}^2")
+    result["eval_mesh"] = evaluate([950], 32, args.eval_mesh_res, args.ref_spp, dev, weights=args.out,
+                                   time_worlds=args.eval_mesh_time_worlds, kind="mesh")
+    meta["eval_mesh"] = result["eval_mesh"]
     learned.save(net, args.out, meta)
   print(json.dumps(result), flush=True)
   return 0

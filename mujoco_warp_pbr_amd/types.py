@@ -174,6 +174,8 @@ class PbrRenderContext:
   guide_albedo: Any
   guide_normal: Any
   traversal: str = "software"
+  ray_origin: Any = None  # ray bundle (set_camera_rays): camera-frame origins [total_rays, 3]
+  ray_direction: Any = None  # and directions; None renders the pinhole cameras
   denoiser: str = "atrous"  # "atrous" (the reference's filter) or "learned" (U-Net, a-trous fallback)
   denoiser_weights: str | None = None  # learned denoiser weights (None: the packaged ones)
   denoise_sigma_color: float = 4.0

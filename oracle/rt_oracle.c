@@ -987,7 +987,71 @@ typedef struct {
   int want_rgb, want_depth, want_seg, projection;
   float fovy, sensor[2], intr[4], xmat[9];
   v3 xpos;
+  const float* rorg; /* ray bundle from the camera's first pixel, NULL: pinhole */
+  const float* rdir;
 } PixO;
+
+/* Ray bundles (camera.py:194-347). */
+static void bundle_axis_o(int i, int n, int nb, int* x0, int* x1, float* t) {
+  int64_t num = (int64_t)(2 * i + 1) * nb - n, den = 2 * (int64_t)n;
+  int a = (int)((num + den) / den) - 1;
+  int hi = nb - 2 > 0 ? nb - 2 : 0;
+  if (a < 0) a = 0;
+  if (a > hi) a = hi;
+  *x0 = a;
+  *x1 = a + 1 < nb - 1 ? a + 1 : nb - 1;
+  *t = (float)(num - (int64_t)a * den) / (float)den;
+}
+
+static v3 lerp_o(v3 a, v3 b, float t) { return add3(a, mul3(sub3(b, a), t)); }
+
+static void bundle_ray_o(const PixO* p, int subdiv, int sx, int sy, v3* dir, v3* org) {
+  int x0, x1, y0, y1;
+  float tx, ty;
+  bundle_axis_o(p->px * subdiv + sx, p->width * subdiv, p->width, &x0, &x1, &tx);
+  bundle_axis_o(p->py * subdiv + sy, p->height * subdiv, p->height, &y0, &y1, &ty);
+  int r0 = y0 * p->width, r1 = y1 * p->width;
+  v3 d00 = load3(p->rdir + 3 * (r0 + x0)), d10 = load3(p->rdir + 3 * (r0 + x1));
+  v3 d01 = load3(p->rdir + 3 * (r1 + x0)), d11 = load3(p->rdir + 3 * (r1 + x1));
+  if (dot3(d00, d00) == 0.0f || dot3(d10, d10) == 0.0f || dot3(d01, d01) == 0.0f || dot3(d11, d11) == 0.0f) {
+    int near = (ty < 0.5f ? r0 : r1) + (tx < 0.5f ? x0 : x1);
+    v3 d = load3(p->rdir + 3 * near);
+    *dir = dot3(d, d) > 0.0f ? norm3(d) : d;
+    *org = load3(p->rorg + 3 * near);
+    return;
+  }
+  v3 o00 = load3(p->rorg + 3 * (r0 + x0)), o10 = load3(p->rorg + 3 * (r0 + x1));
+  v3 o01 = load3(p->rorg + 3 * (r1 + x0)), o11 = load3(p->rorg + 3 * (r1 + x1));
+  if (d00.z < 0.0f && d10.z < 0.0f && d01.z < 0.0f && d11.z < 0.0f) {
+    d00 = mul3(d00, -1.0f / d00.z);
+    d10 = mul3(d10, -1.0f / d10.z);
+    d01 = mul3(d01, -1.0f / d01.z);
+    d11 = mul3(d11, -1.0f / d11.z);
+  }
+  v3 d = lerp_o(lerp_o(d00, d10, tx), lerp_o(d01, d11, tx), ty);
+  *org = lerp_o(lerp_o(o00, o10, tx), lerp_o(o01, o11, tx), ty);
+  *dir = dot3(d, d) > 0.0f ? norm3(d) : d;
+}
+
+static void bundle_step_o(const PixO* p, int a, int b, int extrapolate, float* spread, float* width) {
+  v3 da = load3(p->rdir + 3 * a), db = load3(p->rdir + 3 * b);
+  *spread = 0.0f;
+  *width = 0.0f;
+  if (dot3(da, da) == 0.0f || dot3(db, db) == 0.0f) return;
+  da = norm3(da);
+  db = norm3(db);
+  *width = length3(sub3(load3(p->rorg + 3 * b), load3(p->rorg + 3 * a)));
+  if (extrapolate) {
+    if (da.z < 0.0f && db.z < 0.0f)
+      *spread = length3(sub3(norm3(add3(mul3(db, -2.0f / db.z), mul3(da, 1.0f / da.z))), db));
+    else
+      *spread = length3(sub3(norm3(sub3(mul3(db, 2.0f), da)), db));
+    return;
+  }
+  *spread = length3(sub3(db, da));
+}
+
+static void pixel_cone_o(const PixO* p, float* spread, float* width);
 
 static void cam_ray_o(const PixO* p, int subdiv, int sx, int sy, v3* dir, v3* off) {
   int W = p->width * subdiv, H = p->height * subdiv;
@@ -1013,8 +1077,39 @@ static void cam_ray_o(const PixO* p, int subdiv, int sx, int sy, v3* dir, v3* of
   *off = mk3(0, 0, 0);
 }
 
+static void pixel_cone_o(const PixO* p, float* spread, float* width) {
+  if (p->rdir) {
+    *spread = 0.0f;
+    *width = 0.0f;
+    if (p->width < 2) return;
+    int b = p->py * p->width + p->px;
+    if (p->px + 1 < p->width)
+      bundle_step_o(p, b, b + 1, 0, spread, width);
+    else
+      bundle_step_o(p, b - 1, b, 1, spread, width);
+    return;
+  }
+  v3 d0, o0, d1, o1;
+  cam_ray_o(p, 1, 0, 0, &d0, &o0);
+  cam_ray_o(p, 1, 1, 0, &d1, &o1);
+  *spread = length3(sub3(d1, d0));
+  *width = length3(sub3(o1, o0));
+}
+
 static void primary_o(const PixO* p, const MjpbrSettings* s, int subdiv, int sx, int sy, v3* o, v3* d, float* md, v3* dc) {
   v3 dcam, ocam;
+  if (p->rdir) {
+    bundle_ray_o(p, subdiv, sx, sy, &dcam, &ocam);
+    *md = s->zfar > 0.0f ? s->zfar : O_MAXVAL;
+    if (dot3(dcam, dcam) == 0.0f) {
+      *md = 0.0f;
+      dcam = mk3(0, 0, -1);
+    }
+    *o = add3(p->xpos, mmul(p->xmat, ocam));
+    *d = mmul(p->xmat, dcam);
+    *dc = mk3(0, 0, -1); /* depth along the ray */
+    return;
+  }
   cam_ray_o(p, subdiv, sx, sy, &dcam, &ocam);
   *o = add3(p->xpos, mmul(p->xmat, ocam));
   *d = mmul(p->xmat, dcam);
@@ -1065,6 +1160,8 @@ static int setup_pixel_o(const MjpbrScene* sc, const MjpbrOutputs* out, const Mj
   for (int k = 0; k < 4; ++k) p->intr[k] = cams->cam_intrinsic[((int64_t)(w % cams->cam_intrinsic_n) * cams->ncam + cam) * 4 + k];
   for (int k = 0; k < 9; ++k) p->xmat[k] = cams->cam_xmat[((int64_t)w * cams->ncam + cam) * 9 + k];
   p->xpos = load3(cams->cam_xpos + ((int64_t)w * cams->ncam + cam) * 3);
+  p->rorg = cams->ray_direction ? cams->ray_origin + 3 * (int64_t)ray_offset : NULL;
+  p->rdir = cams->ray_direction ? cams->ray_direction + 3 * (int64_t)ray_offset : NULL;
   return 1;
 }
 
@@ -1081,10 +1178,10 @@ static void render_pixel_o(const MjpbrScene* sc, const MjpbrSettings* s, MjpbrOu
   }
   int n_aa = s->samples_per_axis;
   float inv_aa = 1.0f / (float)n_aa;
-  v3 d0, o0, d1, o1;
-  cam_ray_o(pix, 1, 0, 0, &d0, &o0);
-  cam_ray_o(pix, 1, 1, 0, &d1, &o1);
-  float spread = length3(sub3(d1, d0)) * inv_aa, width = length3(sub3(o1, o0)) * inv_aa;
+  float spread, width;
+  pixel_cone_o(pix, &spread, &width);
+  spread *= inv_aa;
+  width *= inv_aa;
   v3 head_dir = neg3(mk3(pix->xmat[2], pix->xmat[5], pix->xmat[8]));
   c.seed = seed + w * sc->cams.total_rays + pix->rayid;
   v3 display = mk3(0, 0, 0), hdr = mk3(0, 0, 0);
