@@ -114,14 +114,15 @@ def procedural_textures(seed: int = 0, size: int = 256) -> list[tuple[np.ndarray
 
 
 def convex_hull_mesh(rng: np.random.Generator, ntri: int, radius: float):
-  """Random convex hull with about `ntri` triangles: (vertices, faces CCW outward)."""
+  """Convex hull of random points on an ellipsoid: every point is a hull vertex, so the hull has
+  2 * points - 4 = `ntri` triangles (rounded down to even). (vertices, faces CCW outward)."""
   from scipy.spatial import ConvexHull
 
   npts = max(ntri // 2 + 2, 8)
   axes = rng.uniform(0.6, 1.0, 3) * radius
   p = rng.normal(size=(npts, 3))
   p /= np.linalg.norm(p, axis=1, keepdims=True)
-  p *= axes * rng.uniform(0.92, 1.0, (npts, 1))
+  p *= axes
   hull = ConvexHull(p)
   verts = p[hull.vertices]
   remap = -np.ones(len(p), dtype=np.int64)
