@@ -17,7 +17,8 @@ follow in the same line (skip with --no-pt):
                     against the per-pixel megakernel, images identical;
   learned_denoiser  128x128 path tracing, 1 spp + learned U-Net against a-trous at the sample
                     count of equal GPU time (timed on 1024 worlds), PSNR against a 4096-spp
-                    reference, and the U-Net's matrix-core throughput against the BF16 peak.
+                    reference, and the U-Net's matrix-core throughput against the BF16 peak;
+  learned_denoiser_mesh_256  the same on the mesh-heavy scene at 256x256, 1024 worlds, with SSIM.
 """
 
 from __future__ import annotations
@@ -235,6 +236,33 @@ def learned_section(device, time_worlds: int = 1024, quality_worlds: int = 128, 
   }
 
 
+def learned_mesh_section(device, time_worlds: int = 1024, quality_worlds: int = 16, ref_spp: int = 4096) -> dict:
+  """Path tracing the mesh-heavy scene (64 instances of 16 procedural meshes, ~200k
+  triangles, textures, HDR sky) at 256x256, 1 spp + learned U-Net against a-trous (5 iterations)
+  at k spp; times on `time_worlds` worlds, PSNR / SSIM on `quality_worlds` worlds against a
+  `ref_spp` reference."""
+  from mujoco_warp_pbr_amd import train_denoiser as td
+
+  res = td.evaluate([950], quality_worlds, 256, ref_spp, device, atrous_spp=(1, 2, 3, 4), time_worlds=time_worlds, kind="mesh")
+  log(f"mesh-heavy 256^2: learned {res['learned_psnr']:.2f} dB vs a-trous@{res['equal_time_atrous_spp']} "
+      f"{res['equal_time_atrous_psnr']:.2f} dB at {res['learned_ms']:.1f} / {res['equal_time_atrous_ms']:.1f} ms")
+  return {
+    "workload": f"path tracing the mesh-heavy scene at 256x256 (held-out seed), time on {time_worlds} worlds, PSNR/SSIM on {quality_worlds} worlds vs {ref_spp}-spp reference",
+    "learned_psnr_db": round(res["learned_psnr"], 3),
+    "learned_ssim": round(res["learned_ssim"], 4),
+    "equal_time_atrous_psnr_db": round(res["equal_time_atrous_psnr"], 3),
+    "equal_time_atrous_ssim": round(res["equal_time_atrous_ssim"], 4),
+    "gain_db": round(res["gain_db"], 3),
+    "learned_ms": round(res["learned_ms"], 3),
+    "equal_time_atrous_spp": res["equal_time_atrous_spp"],
+    "equal_time_atrous_ms": round(res["equal_time_atrous_ms"], 3),
+    "atrous": res["atrous"],
+    "fallback_worlds": res["fallback_worlds"],
+    "unet_ms": round(res["unet_ms"], 3),
+    "unet_tflops": round(res["unet_tflops"], 2),
+  }
+
+
 def main(argv=None) -> int:
   p = argparse.ArgumentParser()
   p.add_argument("--gpus", type=int, default=1)
@@ -302,7 +330,7 @@ def main(argv=None) -> int:
     log("cpu baseline (oracle)")
     baseline = cpu_baseline(mjm, m, d, rc)
 
-  pt_wavefront = learned_denoiser = None
+  pt_wavefront = learned_denoiser = learned_denoiser_mesh = None
   if not args.no_pt:
     del rc, m, d
     torch.cuda.empty_cache()
@@ -311,6 +339,9 @@ def main(argv=None) -> int:
     torch.cuda.empty_cache()
     log("path tracing: learned denoiser vs a-trous at equal time")
     learned_denoiser = learned_section(device)
+    torch.cuda.empty_cache()
+    log("path tracing: learned denoiser vs a-trous, mesh-heavy scene at 256x256")
+    learned_denoiser_mesh = learned_mesh_section(device)
 
   line = {
     "metric": METRIC,
@@ -343,6 +374,7 @@ def main(argv=None) -> int:
     "cpu_baseline": baseline,
     "pt_wavefront": pt_wavefront,
     "learned_denoiser": learned_denoiser,
+    "learned_denoiser_mesh_256": learned_denoiser_mesh,
   }
   print(json.dumps(line), flush=True)
   return 0
