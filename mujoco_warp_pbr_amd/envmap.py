@@ -17,6 +17,7 @@ import numpy as np
 import torch
 
 from mujoco_warp_pbr_amd import _abi
+from mujoco_warp_pbr_amd import model as mdl
 from mujoco_warp_pbr_amd.material import LUMINANCE_WEIGHTS
 
 PREFILTER_WIDTH = 256
@@ -286,6 +287,82 @@ def empty_environments(device) -> Environments:
     conditional=torch.tensor([[[0.0, 1.0]]], dtype=torch.float32, device=device),
     images=[],
   )
+
+
+SKYBOX_WIDTH = 1024
+
+
+def uv_to_direction(u: np.ndarray, v: np.ndarray) -> np.ndarray:
+  """World direction of equirect coordinates: row 0 = +z, centre column = +x (envmap.py:163-169)."""
+  phi = (u - 0.5) * (2.0 * math.pi)
+  theta = v * math.pi
+  s = np.sin(theta)
+  return np.stack([s * np.cos(phi), s * np.sin(phi), np.cos(theta)], axis=-1)
+
+
+def _skybox_faces(mjm, tex_id: int) -> np.ndarray:
+  """(6, W, W, 3) faces of a skybox texture in [0, 1], order +X, -X, +Y, -Y, +Z, -Z.
+
+  MuJoCo stores a cube texture as its six square faces stacked along the rows; a texture that is
+  a single square image is used for every face.
+  """
+  w, h = int(mjm.tex_width[tex_id]), int(mjm.tex_height[tex_id])
+  nc = int(mjm.tex_nchannel[tex_id])
+  adr = int(mjm.tex_adr[tex_id])
+  img = np.asarray(mjm.tex_data[adr : adr + w * h * nc], dtype=np.float32).reshape(h, w, nc) / 255.0
+  img = np.repeat(img, 3, axis=2) if nc == 1 else img[..., :3]
+  if h == 6 * w:
+    return img.reshape(6, w, w, 3)
+  if h != w:
+    raise ValueError(f"skybox texture {tex_id} must be square or six stacked square faces, got {w}x{h}")
+  return np.broadcast_to(img, (6, w, w, 3))
+
+
+def _sample_cube(faces: np.ndarray, d: np.ndarray) -> np.ndarray:
+  """Bilinear lookup of cube-map faces along directions `d` (..., 3) in the cube's own frame.
+
+  Face selection and face coordinates follow the OpenGL cube-map convention (the major axis picks
+  the face; s, t in [0, 1] with t = 0 on the face's first row), clamped at the face edges.
+  """
+  x, y, z = d[..., 0], d[..., 1], d[..., 2]
+  ax, ay, az = np.abs(x), np.abs(y), np.abs(z)
+  face = np.where(
+    (ax >= ay) & (ax >= az), np.where(x >= 0, 0, 1), np.where(ay >= az, np.where(y >= 0, 2, 3), np.where(z >= 0, 4, 5))
+  )
+  # (sc, tc, ma) per face: +X (-z, -y), -X (z, -y), +Y (x, z), -Y (x, -z), +Z (x, -y), -Z (-x, -y).
+  sc = np.choose(face, [-z, z, x, x, x, -x])
+  tc = np.choose(face, [-y, -y, z, -z, -y, -y])
+  ma = np.maximum(np.choose(face, [ax, ax, ay, ay, az, az]), 1e-30)
+  n = faces.shape[1]
+  s = np.clip((sc / ma + 1.0) * 0.5 * n - 0.5, 0.0, n - 1.0)
+  t = np.clip((tc / ma + 1.0) * 0.5 * n - 0.5, 0.0, n - 1.0)
+  s0, t0 = np.floor(s).astype(np.int64), np.floor(t).astype(np.int64)
+  s1, t1 = np.minimum(s0 + 1, n - 1), np.minimum(t0 + 1, n - 1)
+  fs, ft = (s - s0)[..., None], (t - t0)[..., None]
+  top = faces[face, t0, s0] * (1.0 - fs) + faces[face, t0, s1] * fs
+  bottom = faces[face, t1, s0] * (1.0 - fs) + faces[face, t1, s1] * fs
+  return top * (1.0 - ft) + bottom * ft
+
+
+def skybox_image(mjm, width: int = SKYBOX_WIDTH) -> np.ndarray | None:
+  """The model's first skybox texture as a linear (width // 2, width, 3) equirect image, or None
+  if it has none (envmap.py:307-332).
+
+  Texels are display (sRGB) colours and are linearised after the lookup. MuJoCo draws its skybox
+  cube turned so that the cube's +y axis points along world +z: world direction (x, y, z) looks
+  up cube direction (x, z, -y).
+  """
+  ids = np.nonzero(np.asarray(mjm.tex_type) == mdl.TEXTURE_SKYBOX)[0] if mjm.ntex else []
+  if len(ids) == 0:
+    return None
+  from mujoco_warp_pbr_amd.material import srgb_to_linear
+
+  faces = _skybox_faces(mjm, int(ids[0]))
+  h = width // 2
+  v, u = np.meshgrid((np.arange(h) + 0.5) / h, (np.arange(width) + 0.5) / width, indexing="ij")
+  world = uv_to_direction(u, v)
+  cube = np.stack([world[..., 0], world[..., 2], -world[..., 1]], axis=-1)
+  return srgb_to_linear(_sample_cube(faces, cube)).astype(np.float32)
 
 
 def texel_angle(width: float) -> float:

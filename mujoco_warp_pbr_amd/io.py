@@ -306,10 +306,18 @@ def create_render_context(
   has_reflection = enable_reflections and bool(mjm.nmat and np.any(mjm.mat_reflectance[rendered_mats] > 0.0))
   background_linear = tuple(float(x) for x in material.srgb_to_linear(background_color[:3]))
 
+  # Explicit maps, else the model's skybox, else constant ambient (io.py:267-285). MuJoCo's own
+  # lights never come from the skybox: outside the physical light model it is background only.
   images = _envmap_images(envmap)
+  skybox_only = False
+  if not images and use_skybox:
+    sky = envmap_lib.skybox_image(mjm)
+    if sky is not None:
+      images.append(sky)
+      skybox_only = not physical
   if images:
     env = envmap_lib.build_environments(images, dev, prefilter=precompute)
-    has_env = any_env = True
+    has_env, any_env = not skybox_only, True
   else:
     env = envmap_lib.empty_environments(dev)
     has_env = any_env = False

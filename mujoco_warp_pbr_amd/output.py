@@ -28,7 +28,8 @@ def _check_out(out: torch.Tensor, rc: PbrRenderContext, h: int, w: int, tail: tu
   n = out.shape[0] if out.dim() else 0
   shape = (n, h, w, *tail)
   if out.dim() != len(shape) or tuple(out.shape) != shape or n < 1 or n > rc.base.nworld:
-    raise ValueError(f"{name} must have shape (n <= {rc.base.nworld}, {h}, {w}{''.join(', ' + str(t) for t in tail)}), got {tuple(out.shape)}")
+    per_pixel = f" of {tail[0]}-vectors" if tail else ""
+    raise ValueError(f"{name} must have shape (<= {rc.base.nworld}, {h}, {w}){per_pixel}, got {tuple(out.shape)}")
   if out.dtype != dtype:
     raise ValueError(f"{name} must have dtype {dtype}, got {out.dtype}")
   return n
@@ -57,13 +58,11 @@ def get_segmentation(rc: PbrRenderContext, camera_index: int, segmentation_out: 
   segmentation_out.copy_(rc.base.seg_data[:n, adr : adr + w * h].reshape(n, h, w, 2))
 
 
-def _get_rgb_image(rc: PbrRenderContext, buffer: torch.Tensor, camera_index: int, out: torch.Tensor, what: str) -> None:
+def _get_rgb_image(rc: PbrRenderContext, buffer: torch.Tensor, camera_index: int, out: torch.Tensor, name: str) -> None:
   adr, w, h = _camera(rc, camera_index, "rgb_adr", "rgb")
-  if buffer.shape[1] == 1 and w * h > 1:
-    raise ValueError(f"the context does not record {what}")
-  n = _check_out(out, rc, h, w, (3,), torch.float32, "out")
+  n = _check_out(out, rc, h, w, (3,), torch.float32, name)
   if not out.is_contiguous():
-    raise ValueError("out must be contiguous")
+    raise ValueError(f"{name} must be contiguous")
   _abi.call(
     "mjpbr_extract_rgb_image",
     _abi.ptr(buffer),
@@ -77,16 +76,25 @@ def _get_rgb_image(rc: PbrRenderContext, buffer: torch.Tensor, camera_index: int
   )
 
 
+def _check_guides(rc: PbrRenderContext) -> None:
+  if not rc.render_guides:
+    raise ValueError("create the render context with render_guides=True to read the albedo / normal guides")
+
+
 def get_hdr(rc: PbrRenderContext, camera_index: int, hdr_out: torch.Tensor) -> None:
   """Linear HDR radiance (before exposure and tone mapping), (n, h, w, 3); needs render_hdr."""
-  _get_rgb_image(rc, rc.hdr_data, camera_index, hdr_out, "HDR (create it with render_hdr=True)")
+  if not rc.render_hdr:
+    raise ValueError("create the render context with render_hdr=True to read HDR output")
+  _get_rgb_image(rc, rc.hdr_data, camera_index, hdr_out, "hdr_out")
 
 
 def get_albedo(rc: PbrRenderContext, camera_index: int, albedo_out: torch.Tensor) -> None:
   """First-hit linear base colour, (n, h, w, 3); needs render_guides."""
-  _get_rgb_image(rc, rc.guide_albedo, camera_index, albedo_out, "guides (create it with render_guides=True)")
+  _check_guides(rc)
+  _get_rgb_image(rc, rc.guide_albedo, camera_index, albedo_out, "albedo_out")
 
 
 def get_normal(rc: PbrRenderContext, camera_index: int, normal_out: torch.Tensor) -> None:
   """First-hit world shading normal, (n, h, w, 3); needs render_guides."""
-  _get_rgb_image(rc, rc.guide_normal, camera_index, normal_out, "guides (create it with render_guides=True)")
+  _check_guides(rc)
+  _get_rgb_image(rc, rc.guide_normal, camera_index, normal_out, "normal_out")

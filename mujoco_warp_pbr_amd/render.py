@@ -8,6 +8,7 @@ current torch stream of the data's device, so render() performs no host synchron
 from __future__ import annotations
 
 import ctypes
+import warnings
 
 import numpy as np
 import torch
@@ -275,12 +276,37 @@ def check_env_id(rc: PbrRenderContext) -> None:
   rc._host_cache["env_id"] = env_id
 
 
+def warn_ignored_overrides(m, rc: PbrRenderContext) -> None:
+  """Warns once per replaced array when it needs a flag that is off (launch.py:59-87).
+
+  has_transparency and has_reflection are derived from the model when the context is created;
+  a replaced m.geom_rgba with translucent alphas, or a replaced rc.mat_reflectance with
+  reflective entries, renders as before until the flag is set. The arrays seen by the first
+  render are the model's own and are not checked. Skipped under graph capture.
+  """
+  if m.geom_rgba.is_cuda and torch.cuda.is_current_stream_capturing():
+    return
+  cache = rc._host_cache
+  checks = (
+    ("geom_rgba", m.geom_rgba, rc.has_transparency, lambda a: bool(((a[..., 3] > 0.0) & (a[..., 3] < 1.0)).any()), "has_transparency"),
+    ("mat_reflectance", rc.mat_reflectance, rc.has_reflection, lambda a: bool((a > 0.0).any()), "has_reflection"),
+  )
+  for name, array, enabled, needs, flag in checks:
+    seen = cache.get(name)
+    cache[name] = array
+    if seen is None or seen is array or enabled:
+      continue
+    if needs(array):
+      warnings.warn(f"the replaced {name} is ignored by the renderer: set rc.{flag} = True to render it", stacklevel=3)
+
+
 def render_path_samples(m, d, rc: PbrRenderContext, seed: int = 0) -> None:
   """The path tracer alone: noisy HDR (and, with `rc.denoise`, the first-hit AOVs) into the
   context's buffers, no filter. Training-data generation for the learned denoiser uses it."""
   if rc.tier != Tier.PATH:
     raise ValueError("render_path_samples needs a path-traced context (Tier.PATH)")
   check_env_id(rc)
+  warn_ignored_overrides(m, rc)
   sc = scene_struct(m, d, rc)
   s = settings_struct(rc)
   out = outputs_struct(rc)
@@ -293,6 +319,7 @@ def render(m, d, rc: PbrRenderContext, seed: int = 0) -> None:
   Call `refit_bvh(m, d, rc)` first whenever geometry moved.
   """
   check_env_id(rc)
+  warn_ignored_overrides(m, rc)
   if d.nworld != rc.base.nworld:
     raise ValueError(f"data has {d.nworld} worlds, the context {rc.base.nworld}")
   sc = scene_struct(m, d, rc)

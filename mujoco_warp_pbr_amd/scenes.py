@@ -250,11 +250,11 @@ class _Builder:
     )
     return len(self.lists["mesh_vertadr"]) - 1
 
-  def texture(self, image: np.ndarray, colorspace: int):
+  def texture(self, image: np.ndarray, colorspace: int, tex_type: int = mdl.TEXTURE_2D):
     h, w, nc = image.shape
     adr = sum(len(t) for t in self.lists["tex_data"])
     self.add(
-      tex_type=mdl.TEXTURE_2D,
+      tex_type=tex_type,
       tex_width=w,
       tex_height=h,
       tex_nchannel=nc,

@@ -75,3 +75,66 @@ def gpu_outputs(rc) -> dict:
     "albedo": rc.guide_albedo.cpu().numpy(),
     "normal": rc.guide_normal.cpu().numpy(),
   }
+
+
+def xyaxes(x, y) -> np.ndarray:
+  """Row-major camera frame of MJCF `xyaxes`: columns x, y (orthogonalised), z = x cross y."""
+  x = np.asarray(x, np.float64) / np.linalg.norm(x)
+  y = np.asarray(y, np.float64)
+  y = y - x * np.dot(x, y)
+  y /= np.linalg.norm(y)
+  return np.stack([x, y, np.cross(x, y)], axis=1).astype(np.float32).reshape(9)
+
+
+def channels(rgb: np.ndarray) -> np.ndarray:
+  """Packed RGBA8 (uint32) -> (..., 3) floats in [0, 1], like the reference's get_rgb."""
+  rgb = np.asarray(rgb).view(np.uint32)
+  return np.stack([(rgb >> s) & 0xFF for s in (0, 8, 16)], axis=-1).astype(np.float32) / 255.0
+
+
+class Shot:
+  """One rendered frame, like the reference test fixture's Scene (tests/conftest.py:35-58).
+
+  On the CPU the oracle renders; on a GPU the HIP renderer does.
+  """
+
+  def __init__(self, mjm, device="cpu", nworld=1, cam_res=(32, 32), **kw):
+    self.mjm = mjm
+    self.m, self.d, self.rc = make(mjm, nworld, device=device, cam_res=cam_res, **kw)
+    self.out = None
+
+  @property
+  def on_gpu(self) -> bool:
+    return bool(self.d.geom_xpos.is_cuda)
+
+  def render(self, seed: int = 0) -> "Shot":
+    import oracle
+
+    if self.on_gpu:
+      mj.refit_bvh(self.m, self.d, self.rc)
+      mj.render(self.m, self.d, self.rc, seed=seed)
+      self.out = gpu_outputs(self.rc)
+    else:
+      from mujoco_warp_pbr_amd.render import check_env_id
+      from mujoco_warp_pbr_amd.render import warn_ignored_overrides
+
+      check_env_id(self.rc)
+      warn_ignored_overrides(self.m, self.rc)
+      fn = oracle.pt_render if self.rc.tier == mj.Tier.PATH else oracle.rt_render
+      self.out = fn(self.m, self.d, self.rc, seed=seed)
+    return self
+
+  def _image(self, key, cam):
+    w, h = (int(x) for x in self.rc.base.cam_res[cam])
+    adr = int(self.rc.base.rgb_adr[cam])
+    return self.out[key][:, adr : adr + w * h].reshape(-1, h, w, *self.out[key].shape[2:])
+
+  def rgb(self, cam: int = 0) -> np.ndarray:
+    return channels(self._image("rgb", cam))
+
+  def hdr(self, cam: int = 0) -> np.ndarray:
+    return self._image("hdr", cam)
+
+  @property
+  def exposure(self) -> float:
+    return float(self.rc.exposure.reshape(-1)[0])

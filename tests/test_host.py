@@ -251,3 +251,51 @@ def test_per_world_tables_are_batched():
   rc = mj.create_render_context(mjm, nworld=3, device="cpu", precompute=False)
   assert rc.geom_roughness.shape == (1, mjm.ngeom)
   assert rc.base.rgb_data.shape == (3, 8 * 8)
+
+
+@pytest.mark.parametrize(
+  "kwargs,match",
+  [
+    ({"tier": 5}, "tier must be one of"),
+    ({"light_model": 3}, "light_model must be one of"),
+    ({"render_order": 9}, "render_order must be one of"),
+    ({"unset_material": 2}, "unset_material must be one of"),
+    ({"path_samples": 0}, "path_samples must be at least 1"),
+    ({"max_layers": 0}, "max_layers must be at least 1"),
+    ({"default_material": {"roughness": 2.0}}, r"default_material\['roughness'\] must be in"),
+    ({"envmap": [np.zeros((8, 16, 3), np.float32), np.zeros((8, 16, 5), np.float32)]}, r"envmap\[1\]"),
+    ({"mesh_bvh_constructor": "median"}, "mesh_bvh_constructor must be one of"),
+    ({"block_dim": 0}, "block_dim must be at least 1"),
+  ],
+)
+def test_create_render_context_validation_more(kwargs, match):
+  """tests/test_context_validation.py:41-69, the remaining cases."""
+  with pytest.raises(ValueError, match=match):
+    mj.create_render_context(scenes.primitives_scene(0, 8), cam_res=(8, 8), device="cpu", precompute=False, **kwargs)
+
+
+def test_enum_arguments_accept_ints():
+  """tests/test_context_validation.py:72-74."""
+  rc = mj.create_render_context(scenes.primitives_scene(0, 8), cam_res=(8, 8), device="cpu", precompute=False, tone_mapper=1, render_order=0, tier=0)
+  assert rc.tone_mapper == mj.ToneMapper.ACES and rc.render_order == mj.RenderOrder.ROW
+
+
+def test_get_hdr_checks():
+  """tests/test_context_validation.py:77-82."""
+  rc = mj.create_render_context(scenes.primitives_scene(0, 8), cam_res=(8, 6), device="cpu", precompute=False, render_hdr=True)
+  with pytest.raises(ValueError, match="camera_index must be in"):
+    mj.get_hdr(rc, 1, torch.zeros((1, 6, 8, 3)))
+  with pytest.raises(ValueError, match=r"hdr_out must have shape \(<= 1, 6, 8\)"):
+    mj.get_hdr(rc, 0, torch.zeros((1, 8, 6, 3)))
+
+
+def test_env_id_range():
+  """tests/test_context_validation.py:85-95: an env_id past the last map is refused at render."""
+  from mujoco_warp_pbr_amd.render import check_env_id
+
+  maps = [np.full((8, 16, 3), 0.5, np.float32)] * 2
+  rc = mj.create_render_context(scenes.primitives_scene(0, 8), nworld=2, cam_res=(8, 8), device="cpu", precompute=False, envmap=maps)
+  check_env_id(rc)
+  rc.env_id = torch.tensor([0, 2], dtype=torch.int32)
+  with pytest.raises(ValueError, match=r"env_id must be in \[0, 2\)"):
+    check_env_id(rc)

@@ -121,3 +121,21 @@ def test_refit_follows_moving_geoms(cuda):
   ref = oracle.rt_render(m, d, rc)
   compare_rt(gpu, ref)
   assert np.any(gpu["seg"][0] != gpu["seg"][1])
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("scene", ["per_world_metallic", "display"])
+def test_render_behaviour_scenes_match_oracle(cuda, scene):
+  """tests/test_render.py:186-219 scenes on the GPU: per-world metallic (two worlds) and the
+  HDR / display consistency scene, every output against the oracle."""
+  from tests.helpers import Shot
+  from tests.test_oracle_rt import display_scene
+  from tests.test_oracle_rt import per_world_metallic_scene
+
+  if scene == "per_world_metallic":
+    s = Shot(per_world_metallic_scene(), cuda, nworld=2)
+    s.rc.geom_metallic = torch.tensor([[0.0], [1.0]], device=cuda)
+  else:
+    s = Shot(display_scene(), cuda, nworld=2, tone_mapper=mj.ToneMapper.CLAMP)
+  s.render()
+  compare_rt(s.out, oracle.rt_render(s.m, s.d, s.rc))
