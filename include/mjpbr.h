@@ -51,7 +51,7 @@ enum { MJPBR_TEX_RGBA8 = 0, MJPBR_TEX_RGBA16F = 1 };
 typedef struct MjpbrGeometry {
   int32_t nworld;
   int32_t ngeom;
-  int32_t nmesh;
+  int32_t nmesh;  /* model meshes; height field h is traced as mesh nmesh + h */
   int32_t pad0;
   const int32_t* geom_type;    /* [ngeom] */
   const int32_t* geom_dataid;  /* [n, ngeom] mesh id of mesh geoms */
@@ -74,12 +74,12 @@ typedef struct MjpbrGeometry {
   /* Mesh BVHs (shared by all worlds), same implicit layout, MJPBR_BLAS_LEAF triangles a leaf.
    * blas_tri[slot] = {v0.xyz, face id (int bits)}, {e1 = v1 - v0}, {e2 = v2 - v0} in the mesh
    * frame. */
-  const int32_t* blas_depth;   /* [nmesh] */
-  const int32_t* blas_node_adr;/* [nmesh] first inner node */
-  const int32_t* blas_tri_adr; /* [nmesh] first triangle slot */
+  const int32_t* blas_depth;   /* [nmesh + nhfield] */
+  const int32_t* blas_node_adr;/* [nmesh + nhfield] first inner node */
+  const int32_t* blas_tri_adr; /* [nmesh + nhfield] first triangle slot */
   const float* blas_box;       /* [sum inner nodes, 12] */
   const float* blas_tri;       /* [slots, 12] */
-  const float* mesh_aabb;      /* [nmesh, 6] mesh-frame bounds {lo.xyz, hi.xyz} */
+  const float* mesh_aabb;      /* [nmesh + nhfield, 6] mesh-frame bounds {lo.xyz, hi.xyz} */
 } MjpbrGeometry;
 
 /* Materials, mesh attributes and textures (reference: material.py:224-249 `Materials`). */
@@ -94,12 +94,12 @@ typedef struct MjpbrMaterials {
   const float* mat_texrepeat;    /* [n, nmat, 2] */
   int32_t geom_metallic_n, geom_roughness_n, geom_reflectance_n, mat_reflectance_n;
   int32_t mat_emission_n, mat_texid_n, mat_texuniform_n, mat_texrepeat_n;
-  /* Mesh attributes, mesh frame. */
-  const int32_t* mesh_vertadr;      /* [nmesh] */
-  const int32_t* mesh_faceadr;      /* [nmesh] */
-  const int32_t* mesh_facenum;      /* [nmesh] */
-  const int32_t* mesh_normaladr;    /* [nmesh] */
-  const int32_t* mesh_texcoordadr;  /* [nmesh], -1: no texture coordinates */
+  /* Mesh attributes, mesh frame; height fields follow the meshes (flat, no uv). */
+  const int32_t* mesh_vertadr;      /* [nmesh + nhfield] */
+  const int32_t* mesh_faceadr;      /* [nmesh + nhfield] */
+  const int32_t* mesh_facenum;      /* [nmesh + nhfield] */
+  const int32_t* mesh_normaladr;    /* [nmesh + nhfield] */
+  const int32_t* mesh_texcoordadr;  /* [nmesh + nhfield], -1: no texture coordinates */
   const float* mesh_vert;           /* [nvert, 3] */
   const float* mesh_normal;         /* [nnormal, 3] */
   const int32_t* mesh_face;         /* [nface, 3] */

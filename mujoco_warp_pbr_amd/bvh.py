@@ -137,13 +137,15 @@ def _inner_boxes_masked(llo, lhi, empty, depth):
 
 
 def build_mesh_bvhs(mjm: mdl.HostModel) -> MeshBvhs:
+  """BVHs of the model's meshes followed by those of its height fields (mesh nmesh + h)."""
+  tm = mdl.trace_mesh_arrays(mjm)
   depth, node_adr, tri_adr, boxes, tris, aabb = [], [], [], [], [], []
   nb = nt = 0
-  for m in range(mjm.nmesh):
-    v0, nv = int(mjm.mesh_vertadr[m]), int(mjm.mesh_vertnum[m])
-    f0, nf = int(mjm.mesh_faceadr[m]), int(mjm.mesh_facenum[m])
-    verts = mjm.mesh_vert[v0 : v0 + nv]
-    d, b, t = build_mesh_bvh(verts, mjm.mesh_face[f0 : f0 + nf])
+  for m in range(len(tm["mesh_vertadr"])):
+    v0, nv = int(tm["mesh_vertadr"][m]), int(tm["mesh_vertnum"][m])
+    f0, nf = int(tm["mesh_faceadr"][m]), int(tm["mesh_facenum"][m])
+    verts = tm["mesh_vert"][v0 : v0 + nv]
+    d, b, t = build_mesh_bvh(verts, tm["mesh_face"][f0 : f0 + nf])
     depth.append(d)
     node_adr.append(nb)
     tri_adr.append(nt)

@@ -38,8 +38,10 @@ MJ_DEV void geom_aabb(const MjpbrGeometry& g, int w, int gi, float* lo, float* h
       e = V3{sz[0], sz[1], sz[2]};
       break;
     case MJPBR_GEOM_MESH:
-    case MJPBR_GEOM_SDF: {
+    case MJPBR_GEOM_SDF:
+    case MJPBR_GEOM_HFIELD: {
       int mid = g.geom_dataid[(int64_t)(w % g.geom_dataid_n) * g.ngeom + gi];
+      if (mid >= 0 && type == MJPBR_GEOM_HFIELD) mid += g.nmesh;
       if (mid >= 0) {
         const float* a = g.mesh_aabb + 6 * mid;
         c = V3{0.5f * (a[0] + a[3]), 0.5f * (a[1] + a[4]), 0.5f * (a[2] + a[5])};

@@ -396,7 +396,8 @@ MJ_DEV void primitive_uv(int type, V3 p, V3 ln, V3 size, V2 repeat, bool uniform
   *rep = repeat;
   *off = V2{0.0f, 0.0f};
   switch (type) {
-    case MJPBR_GEOM_PLANE: {
+    case MJPBR_GEOM_PLANE:
+    case MJPBR_GEOM_HFIELD: {
       if (uniform || size.x <= 0.0f || size.y <= 0.0f) {
         *uv = V2{p.x, -p.y};
       } else {

@@ -39,6 +39,7 @@ MJ_DEV SceneView stage_world(const MjpbrScene& sc, int w, float* lds) {
     float4 rgba = geom_rgba(g, w, gi, &mid);
     int type = g.geom_type[gi];
     int dataid = bat(g.geom_dataid, g.geom_dataid_n, w, g.ngeom)[gi];
+    if (type == MJPBR_GEOM_HFIELD && dataid >= 0) dataid += g.nmesh;  // traced as a mesh
     int bits = type | (rgba.w == 0.0f ? 0x100 : 0) | ((dataid + 1) << 9);
     r[15] = __int_as_float(bits);
   }

@@ -7,7 +7,9 @@
 // Both BVHs are implicit perfect binary trees (heap order) whose inner nodes hold their two
 // children's boxes. Traversal is nearest-child-first with a one-register "trail": bit k set
 // means the far child at level k is still pending; popping recomputes the pending node from
-// the heap index and re-tests its box against the current cut, so no stack memory is used.
+// the heap index, so no stack memory is used. A closest-hit query re-tests the popped box
+// against the cut, which may have shrunk since the push; an any-hit query's cut is fixed, so
+// the popped child is known to intersect and is entered without another box load.
 #pragma once
 #include "../../include/mjpbr.h"
 #include "geom.hpp"
@@ -16,7 +18,8 @@ namespace mjpbr {
 
 // Staged geom record: 16 floats.
 //   [0..2] xpos  [3] size.x  [4..12] xmat  [13] size.y  [14] size.z  [15] bits
-// bits = type | alpha_zero << 8 | (dataid + 1) << 9
+// bits = type | alpha_zero << 8 | (mesh + 1) << 9, mesh = the traced mesh (dataid of a mesh or
+// SDF geom, nmesh + dataid of a height field)
 constexpr int kGeomRec = 16;
 
 struct Hit {
@@ -44,7 +47,7 @@ struct SceneView {
 MJ_DEV float cut_of(float dist) { return dist * (1.0f + kRayTolRel) + kRayTolAbs; }
 
 // Visits the leaves of an implicit BVH nearest-first. leaf(slot) returns true to stop.
-template <class CutFn, class LeafFn>
+template <bool kRetest, class CutFn, class LeafFn>
 MJ_DEV void traverse(const float* boxes, int depth, V3 o, V3 rcp, CutFn cut, LeafFn leaf) {
   const int ninner = (1 << depth) - 1;
   int node = 0, level = 0;
@@ -72,6 +75,7 @@ MJ_DEV void traverse(const float* boxes, int depth, V3 o, V3 rcp, CutFn cut, Lea
       int anc1 = (node + 1) >> (level - L - 1);
       node = (anc1 ^ 1) - 1;
       level = L + 1;
+      if (!kRetest) break;
       const float* pb = boxes + 12 * ((node - 1) >> 1) + 6 * ((node - 1) & 1);
       if (ray_aabb(o, rcp, pb, pb + 3, cut()) >= 0.0f) break;
     }
@@ -91,7 +95,7 @@ MJ_DEV bool mesh_closest(const SceneView& sv, int mesh, V3 lo, V3 ld, float max_
   float bb1 = 0.0f, bb2 = 0.0f;
   V3 be1 = v3s(0.0f), be2 = v3s(0.0f);
   V3 rcp = safe_rcp(ld);
-  traverse(
+  traverse<true>(
       boxes, depth, lo, rcp, [&]() { return best == INFINITY ? max_t : fminf(max_t, cut_of(best)); },
       [&](int slot) {
         const float4* tr = reinterpret_cast<const float4*>(tris) + 3 * slot * MJPBR_BLAS_LEAF;
@@ -131,7 +135,7 @@ MJ_DEV bool mesh_any(const SceneView& sv, int mesh, V3 lo, V3 ld, float max_t) {
   int depth = sv.blas_depth[mesh];
   bool hit = false;
   V3 rcp = safe_rcp(ld);
-  traverse(
+  traverse<false>(
       boxes, depth, lo, rcp, [&]() { return max_t; },
       [&](int slot) {
         const float4* tr = reinterpret_cast<const float4*>(tris) + 3 * slot * MJPBR_BLAS_LEAF;
@@ -173,7 +177,11 @@ MJ_DEV float intersect_geom(const SceneView& sv, const float* g, V3 o, V3 d, flo
       return ray_cylinder(pos, mat, size, o, d, n);
     case MJPBR_GEOM_BOX:
       return ray_box(pos, mat, size, o, d, n);
-    case MJPBR_GEOM_MESH: {
+    case MJPBR_GEOM_MESH:
+    case MJPBR_GEOM_SDF:
+    case MJPBR_GEOM_HFIELD: {
+      // SDF geoms trace their mesh (traverse.py:185-196); a height field traces its triangle
+      // surface and reports no mesh id, so it is shaded like a primitive (traverse.py:161-165).
       int mid = (bits >> 9) - 1;
       if (mid < 0) return -1.0f;
       V3 lo = mat_tmul(mat, o - pos);
@@ -190,7 +198,7 @@ MJ_DEV float intersect_geom(const SceneView& sv, const float* g, V3 o, V3 d, flo
       *bu = 1.0f - b1 - b2;
       *bv = b1;
       *face = f;
-      *mesh = mid;
+      *mesh = type == MJPBR_GEOM_HFIELD ? -1 : mid;
       return t;
     }
     default:
@@ -208,7 +216,7 @@ MJ_DEV Hit trace_closest(const SceneView& sv, V3 o, V3 d, float max_dist, bool c
   h.face_id = -1;
   h.mesh_id = -1;
   V3 rcp = safe_rcp(d);
-  traverse(
+  traverse<true>(
       sv.tlas_box, sv.depth, o, rcp, [&]() { return cut_of(h.dist); },
       [&](int slot) {
         int gi = sv.tlas_leaf[slot];
@@ -241,7 +249,7 @@ MJ_DEV bool trace_any(const SceneView& sv, V3 o, V3 d, float max_dist, bool cull
   V3 rcp = safe_rcp(d);
   bool hit = false;
   float cut = cut_of(max_dist);
-  traverse(
+  traverse<false>(
       sv.tlas_box, sv.depth, o, rcp, [&]() { return cut; },
       [&](int slot) {
         int gi = sv.tlas_leaf[slot];

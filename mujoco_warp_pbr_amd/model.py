@@ -123,6 +123,16 @@ class HostModel:
   headlight_specular: np.ndarray = dataclasses.field(default_factory=lambda: np.full(3, 0.5, np.float32))
   stat_center: np.ndarray = dataclasses.field(default_factory=lambda: np.zeros(3, np.float32))
   stat_extent: float = 1.0
+  # height fields: (radius x, radius y, elevation z, base z), grids of normalised elevations
+  hfield_size: np.ndarray = dataclasses.field(default_factory=lambda: np.zeros((0, 4), np.float32))
+  hfield_nrow: np.ndarray = dataclasses.field(default_factory=lambda: np.zeros(0, np.int32))
+  hfield_ncol: np.ndarray = dataclasses.field(default_factory=lambda: np.zeros(0, np.int32))
+  hfield_adr: np.ndarray = dataclasses.field(default_factory=lambda: np.zeros(0, np.int32))
+  hfield_data: np.ndarray = dataclasses.field(default_factory=lambda: np.zeros(0, np.float32))
+
+  @property
+  def nhfield(self) -> int:
+    return int(self.hfield_nrow.shape[0])
 
   @property
   def ngeom(self) -> int:
@@ -158,6 +168,76 @@ class HostModel:
       raise ValueError(f"mat_texid must have shape ({self.nmat}, {NTEXROLE})")
     if np.any(self.geom_matid >= self.nmat):
       raise ValueError("geom_matid out of range")
+    nh = self.nhfield
+    if self.hfield_size.shape != (nh, 4) or self.hfield_ncol.shape != (nh,) or self.hfield_adr.shape != (nh,):
+      raise ValueError("hfield_size, hfield_nrow, hfield_ncol and hfield_adr must describe the same height fields")
+    for h in range(nh):
+      nr, nc, adr = int(self.hfield_nrow[h]), int(self.hfield_ncol[h]), int(self.hfield_adr[h])
+      if nr < 2 or nc < 2 or adr < 0 or adr + nr * nc > self.hfield_data.shape[0]:
+        raise ValueError(f"height field {h}: needs at least 2 x 2 samples inside hfield_data")
+    for kind, count in ((GEOM_HFIELD, nh), (GEOM_MESH, self.nmesh), (GEOM_SDF, self.nmesh)):
+      ids = self.geom_dataid[self.geom_type == kind]
+      if np.any(ids >= count):
+        raise ValueError("geom_dataid out of range")
+
+
+def hfield_mesh(mjm: HostModel, h: int) -> tuple[np.ndarray, np.ndarray]:
+  """Closed triangle surface of height field `h` in its geom frame: (verts (n, 3), faces (m, 3)).
+
+  Sample (r, c) sits at x = sx (2c / (ncol - 1) - 1), y = sy (2r / (nrow - 1) - 1), z = elevation
+  * data[r, c] (MuJoCo's hfield layout); the solid extends down to z = -base. Each grid cell is
+  two triangles, each boundary segment a wall quad, and the base one quad; every face is wound
+  so that cross(v1 - v0, v2 - v0) points out of the solid.
+  """
+  nr, nc, adr = int(mjm.hfield_nrow[h]), int(mjm.hfield_ncol[h]), int(mjm.hfield_adr[h])
+  sx, sy, sz, base = (float(v) for v in mjm.hfield_size[h])
+  z = np.asarray(mjm.hfield_data[adr : adr + nr * nc], np.float64).reshape(nr, nc) * sz
+  ys, xs = np.meshgrid(np.linspace(-sy, sy, nr), np.linspace(-sx, sx, nc), indexing="ij")
+  top = np.stack([xs, ys, z], axis=-1).reshape(-1, 3)
+  idx = np.arange(nr * nc).reshape(nr, nc)
+  a, b, c, d = idx[:-1, :-1].ravel(), idx[:-1, 1:].ravel(), idx[1:, 1:].ravel(), idx[1:, :-1].ravel()
+  faces = [np.stack([a, b, c], 1), np.stack([a, c, d], 1)]
+  # boundary loop (counter-clockwise seen from above) and the base vertices below it
+  ring = np.concatenate([idx[0, :], idx[1:, -1], idx[-1, -2::-1], idx[-2:0:-1, 0]])
+  bottom = top[ring].copy()
+  bottom[:, 2] = -base
+  nb0 = len(top)
+  nxt = np.roll(np.arange(len(ring)), -1)
+  # wall quad per boundary segment: top[i], top[i+1], bottom[i+1], bottom[i]
+  t0, t1, b0, b1 = ring, ring[nxt], nb0 + np.arange(len(ring)), nb0 + nxt
+  faces += [np.stack([t0, b1, t1], 1), np.stack([t0, b0, b1], 1)]
+  corners = nb0 + np.array([0, nc - 1, nc + nr - 2, 2 * nc + nr - 3])
+  faces += [np.array([[corners[0], corners[2], corners[1]], [corners[0], corners[3], corners[2]]])]
+  verts = np.concatenate([top, bottom]).astype(np.float32)
+  return verts, np.concatenate(faces).astype(np.int32)
+
+
+def trace_mesh_arrays(mjm: HostModel) -> dict[str, np.ndarray]:
+  """The model's mesh arrays followed by one triangle mesh per height field: height field h is
+  traced as mesh nmesh + h (flat-shaded, no vertex normals or texture coordinates)."""
+  out = {k: np.asarray(getattr(mjm, k)) for k in PLAIN if k.startswith("mesh_")}
+  out["mesh_vertnum"] = np.asarray(mjm.mesh_vertnum)
+  if mjm.nhfield == 0:
+    return out
+  lists = {k: [v] for k, v in out.items()}
+  nv, nf = len(out["mesh_vert"]), len(out["mesh_face"])
+  nn = len(out["mesh_normal"])
+  for h in range(mjm.nhfield):
+    verts, faces = hfield_mesh(mjm, h)
+    lists["mesh_vertadr"].append(np.array([nv], np.int32))
+    lists["mesh_vertnum"].append(np.array([len(verts)], np.int32))
+    lists["mesh_faceadr"].append(np.array([nf], np.int32))
+    lists["mesh_facenum"].append(np.array([len(faces)], np.int32))
+    lists["mesh_normaladr"].append(np.array([nn], np.int32))
+    lists["mesh_texcoordadr"].append(np.array([-1], np.int32))
+    lists["mesh_vert"].append(verts)
+    lists["mesh_face"].append(faces)
+    lists["mesh_facenormal"].append(np.zeros_like(faces))
+    lists["mesh_facetexcoord"].append(np.zeros_like(faces))
+    nv += len(verts)
+    nf += len(faces)
+  lists["mesh_normal"].append(np.zeros((1, 3), np.float32))
+  return {k: np.concatenate(v).astype(out[k].dtype) for k, v in lists.items()}
 
 
 def empty_host_model() -> dict[str, Any]:
@@ -222,6 +302,11 @@ def empty_host_model() -> dict[str, Any]:
     tex_adr=np.zeros(0, np.int64),
     tex_data=np.zeros(0, np.uint8),
     tex_colorspace=_i32([]),
+    hfield_size=np.zeros((0, 4), np.float32),
+    hfield_nrow=_i32([]),
+    hfield_ncol=_i32([]),
+    hfield_adr=_i32([]),
+    hfield_data=_f32([]),
   )
 
 
@@ -271,11 +356,13 @@ class Model:
 
   def __init__(self, mjm: HostModel, device):
     self.ngeom, self.nmat, self.nlight, self.ncam, self.nmesh = mjm.ngeom, mjm.nmat, mjm.nlight, mjm.ncam, mjm.nmesh
+    self.nhfield = mjm.nhfield
     for name in BATCHED:
       a = np.ascontiguousarray(getattr(mjm, name))
       setattr(self, name, torch.as_tensor(a[None].copy()).to(device))
+    meshes = trace_mesh_arrays(mjm)
     for name in PLAIN:
-      a = np.ascontiguousarray(getattr(mjm, name))
+      a = np.ascontiguousarray(meshes[name] if name in meshes else getattr(mjm, name))
       setattr(self, name, torch.as_tensor(a.copy()).to(device))
 
 

@@ -264,6 +264,19 @@ class _Builder:
     )
     return len(self.lists["tex_type"]) - 1
 
+  def hfield(self, elevation: np.ndarray, size):
+    """Height field from a (nrow, ncol) grid of normalised elevations and (x, y, z, base) size."""
+    nrow, ncol = elevation.shape
+    adr = sum(len(t) for t in self.lists["hfield_data"])
+    self.add(
+      hfield_size=np.asarray(size, np.float32),
+      hfield_nrow=nrow,
+      hfield_ncol=ncol,
+      hfield_adr=adr,
+      hfield_data=np.asarray(elevation, np.float32).ravel(),
+    )
+    return len(self.lists["hfield_nrow"]) - 1
+
   def build(self, **extra) -> mdl.HostModel:
     kw = dict(self.kw)
     concat = {
@@ -274,6 +287,7 @@ class _Builder:
       "mesh_facetexcoord",
       "mesh_texcoord",
       "tex_data",
+      "hfield_data",
     }
     for k, items in self.lists.items():
       if not items:

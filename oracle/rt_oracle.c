@@ -200,9 +200,14 @@ static float intersect_o(const Ctx* c, int gi, v3 o, v3 d, float max_t, int any_
     case MJPBR_GEOM_CAPSULE: return capsule_o(pos, mat, size, o, d, n);
     case MJPBR_GEOM_CYLINDER: return cylinder_o(pos, mat, size, o, d, n);
     case MJPBR_GEOM_BOX: return box_o(pos, mat, size, o, d, n);
-    case MJPBR_GEOM_MESH: {
+    case MJPBR_GEOM_MESH:
+    case MJPBR_GEOM_SDF:
+    case MJPBR_GEOM_HFIELD: {
+      /* SDF geoms trace their mesh; height field h traces its triangle surface, mesh nmesh + h,
+         and reports no mesh id (traverse.py:161-165, 185-196). */
       int mid = g->geom_dataid[bidx(w, g->geom_dataid_n) * g->ngeom + gi];
       if (mid < 0) return -1.0f;
+      if (type == MJPBR_GEOM_HFIELD) mid += g->nmesh;
       v3 lo = mtmul(mat, sub3(o, pos)), ld = mtmul(mat, d);
       int fadr = m->mesh_faceadr[mid], vadr = m->mesh_vertadr[mid];
       /* Conservative early out: a ray missing the (inflated) mesh-frame bounds hits no triangle. */
@@ -245,7 +250,7 @@ static float intersect_o(const Ctx* c, int gi, v3 o, v3 d, float max_t, int any_
       *bu = 1.0f - bb1 - bb2;
       *bv = bb1;
       *face = best_face;
-      *mesh = mid;
+      *mesh = type == MJPBR_GEOM_HFIELD ? -1 : mid;
       return best;
     }
     default: return -1.0f;
@@ -622,7 +627,7 @@ static void prim_uv_o(int type, v3 p, v3 ln, v3 size, v2 repeat, int uniform, v2
   *rep = repeat;
   off->x = 0.0f; off->y = 0.0f;
   v2 r = {0.0f, 0.0f};
-  if (type == MJPBR_GEOM_PLANE) {
+  if (type == MJPBR_GEOM_PLANE || type == MJPBR_GEOM_HFIELD) {
     if (uniform || size.x <= 0.0f || size.y <= 0.0f) { r.x = p.x; r.y = -p.y; }
     else { r.x = p.x / (2.0f * size.x) + 0.5f; r.y = 0.5f - p.y / (2.0f * size.y); }
   } else if (type == MJPBR_GEOM_SPHERE || type == MJPBR_GEOM_ELLIPSOID) {
