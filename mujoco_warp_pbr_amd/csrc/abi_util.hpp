@@ -17,9 +17,10 @@ inline int check_launch(const char* what) {
   return MJPBR_OK;
 }
 
-inline size_t host_stage_bytes(const MjpbrGeometry* g) {
-  int nleaf = 1 << g->tlas_depth;
-  return (size_t)((nleaf - 1) * 12 + g->ngeom * kGeomRec + nleaf) * 4;
+inline size_t host_stage_bytes(const MjpbrScene* sc) {
+  int nleaf = 1 << sc->geo.tlas_depth;
+  return (size_t)((nleaf - 1) * 12 + sc->geo.ngeom * kGeomRec + staged_leaf_slots(nleaf) + sc->lit.nlight * kLightRec +
+                  sc->geo.ngeom * kSurfRec) * 4;
 }
 
 inline int make_tile_map(const MjpbrScene* sc, TileMap* tm, int* nlaunch) {

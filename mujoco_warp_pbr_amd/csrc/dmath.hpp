@@ -10,6 +10,7 @@
 #include <stdint.h>
 
 #define MJ_DEV __device__ __forceinline__
+#define MJ_HOST_DEV __host__ __device__
 
 namespace mjpbr {
 

@@ -192,18 +192,27 @@ MJ_DEV bool ray_triangle(V3 o, V3 d, V3 v0, V3 e1, V3 e2, float tmax, float* t, 
   V3 p = cross(d, e2);
   float det = dot(e1, p);
   if (fabsf(det) < 1.0e-20f) return false;
-  float inv = 1.0f / det;
+  // Edge tests on the unscaled numerators (sign-normalised by det), so a rejected triangle costs
+  // no division; an accepted one divides once. t and the barycentrics are those of dividing
+  // each numerator by det.
   V3 s = o - v0;
-  float u = dot(s, p) * inv;
-  if (u < 0.0f || u > 1.0f) return false;
+  float us = dot(s, p);
+  float adet = fabsf(det);
+  if (det < 0.0f) us = -us;
+  if (us < 0.0f || us > adet) return false;
   V3 q = cross(s, e1);
-  float v = dot(d, q) * inv;
-  if (v < 0.0f || u + v > 1.0f) return false;
-  float tt = dot(e2, q) * inv;
-  if (tt < 0.0f || tt >= tmax) return false;
+  float vs = dot(d, q), ts = dot(e2, q);
+  if (det < 0.0f) {
+    vs = -vs;
+    ts = -ts;
+  }
+  if (vs < 0.0f || us + vs > adet || ts < 0.0f) return false;
+  float inv = 1.0f / adet;
+  float tt = ts * inv;
+  if (tt >= tmax) return false;
   *t = tt;
-  *b1 = u;
-  *b2 = v;
+  *b1 = us * inv;
+  *b2 = vs * inv;
   return true;
 }
 

@@ -85,9 +85,9 @@ MJ_DEV V3 direct_lights(const PtCtx& c, const Lobes& lb, V3 point, V3 normal, V3
   V3 radiance = v3s(0.0f);
   const MjpbrLights& lt = c.sc.lit;
   for (int li = 0; li < lt.nlight; ++li) {
-    if (!bat(lt.light_active, lt.light_active_n, c.w, lt.nlight)[li]) continue;
+    if (!staged_light_active(c.sv.lights, li)) continue;
     V2 u = light_rand2(seed, sample, depth, lt.nlight, li);
-    radiance += light_sample(c, fetch_light(lt, c.w, li), lb, point, normal, t1, t2, v, u);
+    radiance += light_sample(c, staged_light(c.sv.lights, li), lb, point, normal, t1, t2, v, u);
   }
   if (c.s.headlight_active) {
     Light H = headlight(c.s, c.s.headlight_intensity / c.exposure, head_dir);
@@ -166,7 +166,7 @@ MJ_DEV bool path_step(const PtCtx& c, PathState* st, V3 head_dir, int seed, int 
   const int depth = st->depth;
   V3 point = st->origin + st->dir * hit.dist;
   V3 normal;
-  Surface sf = eval_surface(c.sc, s, c.w, hit, point, st->dir, footprint, &normal);
+  Surface sf = eval_surface(c.sc, c.sv, s, c.w, hit, point, st->dir, footprint, &normal);
   if (s.has_transparency) {
     if (prand(seed, sample, depth + st->passes, 0) >= sf.alpha && st->passes < 4 * s.max_layers) {
       st->origin = point + st->dir * kRayOffset;
@@ -186,7 +186,7 @@ MJ_DEV bool path_step(const PtCtx& c, PathState* st, V3 head_dir, int seed, int 
   }
   st->camera_ray = false;
   if (s.has_reflection) {
-    if (prand(seed, sample, depth, 1) < mirror_reflectance(c.sc, c.w, hit.geom_id)) {
+    if (prand(seed, sample, depth, 1) < c.sv.surfs[hit.geom_id * kSurfRec + 12]) {
       st->dir = st->dir - normal * (2.0f * dot(st->dir, normal));
       st->origin = point + normal * kRayOffset;
       st->delta_bounce = true;

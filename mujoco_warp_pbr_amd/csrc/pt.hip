@@ -108,8 +108,8 @@ extern "C" int mjpbr_pt_render(const MjpbrScene* scene, const MjpbrSettings* s, 
   if (s->path_samples < 1) return set_error(MJPBR_EINVAL, "path_samples must be at least 1");
   if (s->max_depth < 0 || s->max_depth + 4 * s->max_layers >= kJitterDepth)
     return set_error(MJPBR_EINVAL, "max_depth + 4 * max_layers must be below %d", kJitterDepth);
-  size_t lds = host_stage_bytes(&scene->geo);
-  if (lds > 64 * 1024) return set_error(MJPBR_EINVAL, "scene too large for LDS staging (%d geoms)", scene->geo.ngeom);
+  size_t lds = host_stage_bytes(scene);
+  if (lds > 64 * 1024) return set_error(MJPBR_EINVAL, "scene too large for LDS staging (%d geoms, %d lights)", scene->geo.ngeom, scene->lit.nlight);
   hipLaunchKernelGGL(pt_kernel, dim3(nlaunch), dim3(256), lds, (hipStream_t)stream, *scene, *s, *out, seed, tm);
   return check_launch("pt_kernel");
 }

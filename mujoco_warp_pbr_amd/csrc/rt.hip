@@ -101,8 +101,8 @@ MJ_DEV V3 shade_surface(const RtCtx& c, const Surface& sf, V3 point, V3 normal, 
     Light L;
     int nlight = lt.nlight, lidx = li;
     if (li < lt.nlight) {
-      if (!bat(lt.light_active, lt.light_active_n, c.w, lt.nlight)[li]) continue;
-      L = fetch_light(lt, c.w, li);
+      if (!staged_light_active(c.sv.lights, li)) continue;
+      L = staged_light(c.sv.lights, li);
     } else {
       L = headlight(s, s.headlight_intensity / c.exposure, head_dir);
       nlight = 1;
@@ -126,21 +126,27 @@ MJ_DEV V3 background(const RtCtx& c, V3 dir, float spread) {
   return ld3(env.background) / c.exposure;
 }
 
+MJ_DEV void write_guides(const MjpbrOutputs& out, int w, const Pixel& pix, V3 albedo, V3 normal) {
+  int64_t a = (int64_t)w * out.guide_stride + out.rgb_adr[pix.camid] + pix.pixelid;
+  st3(out.albedo + 3 * a, albedo);
+  st3(out.normal + 3 * a, normal);
+}
+
 // render.py:249-341. The camera-ray segments (translucent layers) and the mirror segment of
 // each layer share one trace / surface / shading site: a segment loop with a mode flag instead of
 // the reference's nested calls (shade_reflection), so the megakernel holds one inlined copy of
 // the traversal and shading code. Arithmetic and random streams are those of the reference:
 // the mirror segment traces from point + normal * RAY_OFFSET with max distance MJ_MAXVAL, is
 // shaded with stream + 1 and blends as shaded * (1 - mirror) + reflected * mirror.
+//
+// The first sample of a pixel (`write_first`) writes depth, segmentation and the guides of its
+// first hit as soon as they are known, so nothing of that hit stays live through shading.
 MJ_DEV V3 trace_pixel(const RtCtx& c, V3 origin, V3 dir, float max_dist, float footprint, float spread, V3 head_dir,
-                      int stream, Hit* first, V3* first_albedo, V3* first_normal) {
+                      int stream, bool write_first, const MjpbrOutputs& out, const Pixel& pix, V3 dir_cam) {
   const MjpbrSettings& s = c.s;
   const bool cull = s.cull_backfaces != 0;
   V3 radiance = v3s(0.0f);
   float throughput = 1.0f, travelled = 0.0f;
-  first->geom_id = -1;
-  *first_albedo = v3s(0.0f);
-  *first_normal = v3s(0.0f);
   const int nlayer = s.has_transparency ? s.max_layers : 1;
   int layer = 0;
   bool mirror_seg = false;
@@ -151,13 +157,15 @@ MJ_DEV V3 trace_pixel(const RtCtx& c, V3 origin, V3 dir, float max_dist, float f
   float rmax = max_dist - travelled;
   for (;;) {
     Hit hit = trace_closest(c.sv, ro, rd, rmax, cull);
-    if (!mirror_seg && layer == 0) *first = hit;
+    const bool first = write_first && !mirror_seg && layer == 0;
+    if (first) write_geometry(out, c.w, pix, hit, dir_cam);
     V3 shaded;
     V3 normal = v3s(0.0f);
     V3 point = v3s(0.0f);
     float cone = 0.0f;
     float alpha = 1.0f;
     if (hit.geom_id == -1) {
+      if (first && s.render_guides) write_guides(out, c.w, pix, v3s(0.0f), v3s(0.0f));
       shaded = background(c, rd, spread);
       if (!mirror_seg) {
         radiance += shaded * throughput;
@@ -166,16 +174,13 @@ MJ_DEV V3 trace_pixel(const RtCtx& c, V3 origin, V3 dir, float max_dist, float f
     } else {
       point = ro + rd * hit.dist;
       cone = mirror_seg ? layer_cone + spread * hit.dist : footprint + spread * (travelled + hit.dist);
-      Surface sf = eval_surface(c.sc, s, c.w, hit, point, rd, cone, &normal);
+      Surface sf = eval_surface(c.sc, c.sv, s, c.w, hit, point, rd, cone, &normal);
       alpha = sf.alpha;
-      if (!mirror_seg && layer == 0 && s.render_guides) {
-        *first_albedo = sf.base_color;
-        *first_normal = normal;
-      }
+      if (first && s.render_guides) write_guides(out, c.w, pix, sf.base_color, normal);
       shaded = shade_surface(c, sf, point, normal, -rd, head_dir, stream + (mirror_seg ? 1 : 0));
     }
     if (!mirror_seg) {
-      float mirror = s.has_reflection ? mirror_reflectance(c.sc, c.w, hit.geom_id) : 0.0f;
+      float mirror = s.has_reflection && hit.geom_id >= 0 ? c.sv.surfs[hit.geom_id * kSurfRec + 12] : 0.0f;
       if (mirror > 0.0f) {
         layer_shaded = shaded;
         layer_point = point;
@@ -258,17 +263,7 @@ rt_kernel(MjpbrScene sc, MjpbrSettings s, MjpbrOutputs out, int seed, TileMap tm
     V3 o, d, dc;
     float md;
     primary_ray(pix, s, n_aa, smp % n_aa, smp / n_aa, &o, &d, &md, &dc);
-    Hit first;
-    V3 alb, nrm;
-    V3 radiance = trace_pixel(c, o, d, md, width, spread, head_dir, 2 * smp, &first, &alb, &nrm);
-    if (smp == 0) {
-      write_geometry(out, w, pix, first, dc);
-      if (s.render_guides) {
-        int64_t a = (int64_t)w * out.guide_stride + out.rgb_adr[pix.camid] + pix.pixelid;
-        st3(out.albedo + 3 * a, alb);
-        st3(out.normal + 3 * a, nrm);
-      }
-    }
+    V3 radiance = trace_pixel(c, o, d, md, width, spread, head_dir, 2 * smp, smp == 0, out, pix, dc);
     display += display_transform(radiance * exposure, s.tone_mapper);
     hdr += radiance;
   }
@@ -289,8 +284,8 @@ extern "C" int mjpbr_rt_render(const MjpbrScene* scene, const MjpbrSettings* s, 
   int rc = make_tile_map(scene, &tm, &nlaunch);
   if (rc) return rc;
   if (s->samples_per_axis < 1) return set_error(MJPBR_EINVAL, "samples_per_axis must be at least 1");
-  size_t lds = host_stage_bytes(&scene->geo);
-  if (lds > 64 * 1024) return set_error(MJPBR_EINVAL, "scene too large for LDS staging (%d geoms)", scene->geo.ngeom);
+  size_t lds = host_stage_bytes(scene);
+  if (lds > 64 * 1024) return set_error(MJPBR_EINVAL, "scene too large for LDS staging (%d geoms, %d lights)", scene->geo.ngeom, scene->lit.nlight);
   hipLaunchKernelGGL(rt_kernel, dim3(nlaunch), dim3(256), lds, (hipStream_t)stream, *scene, *s, *out, seed,
                      tm);
   return check_launch("rt_kernel");

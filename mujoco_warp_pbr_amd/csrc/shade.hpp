@@ -260,6 +260,52 @@ MJ_DEV Light fetch_light(const MjpbrLights& lt, int w, int li) {
   return L;
 }
 
+// Staged light record (kLightRec floats): [0] type | castshadow << 8 | active << 9 (bits),
+// [1..3] pos, [4..6] dir, [7..9] diffuse, [10..12] specular, [13..15] attenuation, [16] cutoff,
+// [17] exponent, [18] intensity, [19] range, [20] bulbradius, [21..23] unused.
+MJ_DEV void stage_light(const MjpbrLights& lt, int w, int li, float* r) {
+  Light L = fetch_light(lt, w, li);
+  int active = bat(lt.light_active, lt.light_active_n, w, lt.nlight)[li] != 0;
+  r[0] = __int_as_float(L.type | (L.castshadow ? 0x100 : 0) | (active << 9));
+  V3 v[5] = {L.pos, L.dir, L.diffuse, L.specular, L.attenuation};
+#pragma unroll
+  for (int k = 0; k < 5; ++k) {
+    r[1 + 3 * k] = v[k].x;
+    r[2 + 3 * k] = v[k].y;
+    r[3 + 3 * k] = v[k].z;
+  }
+  r[16] = L.cutoff;
+  r[17] = L.exponent;
+  r[18] = L.intensity;
+  r[19] = L.range;
+  r[20] = L.bulbradius;
+  r[21] = r[22] = r[23] = 0.0f;
+}
+
+MJ_DEV bool staged_light_active(const float* lights, int li) {
+  return (__float_as_int(lights[li * kLightRec]) >> 9) & 1;
+}
+
+MJ_DEV Light staged_light(const float* lights, int li) {
+  const float4* q = reinterpret_cast<const float4*>(lights + li * kLightRec);
+  float4 a = q[0], b = q[1], c = q[2], d = q[3], e = q[4], f = q[5];
+  Light L;
+  int bits = __float_as_int(a.x);
+  L.type = bits & 0xff;
+  L.castshadow = (bits & 0x100) != 0;
+  L.pos = V3{a.y, a.z, a.w};
+  L.dir = V3{b.x, b.y, b.z};
+  L.diffuse = V3{b.w, c.x, c.y};
+  L.specular = V3{c.z, c.w, d.x};
+  L.attenuation = V3{d.y, d.z, d.w};
+  L.cutoff = e.x;
+  L.exponent = e.y;
+  L.intensity = e.z;
+  L.range = e.w;
+  L.bulbradius = f.x;
+  return L;
+}
+
 MJ_DEV Light headlight(const MjpbrSettings& s, float intensity, V3 dir) {
   Light L;
   L.type = MJPBR_LIGHT_DIRECTIONAL;
@@ -493,28 +539,29 @@ MJ_DEV float4 fetch_role(const MjpbrMaterials& m, int tex, V2 uv, float cone, fl
 }
 
 // material.py:300-501. Returns the surface and the viewer-facing world shading normal.
-MJ_DEV Surface eval_surface(const MjpbrScene& sc, const MjpbrSettings& s, int w, const Hit& hit, V3 point, V3 dir,
-                            float footprint, V3* shading_world) {
+// The hit geom's material constants, frame and size come from the world's staged records (LDS).
+MJ_DEV Surface eval_surface(const MjpbrScene& sc, const SceneView& sv, const MjpbrSettings& s, int w, const Hit& hit,
+                            V3 point, V3 dir, float footprint, V3* shading_world) {
   const MjpbrGeometry& g = sc.geo;
   const MjpbrMaterials& m = sc.mats;
   Surface sf;
   sf.occlusion = 1.0f;
   sf.emissive = v3s(0.0f);
   int gid = hit.geom_id;
-  int mat_id;
-  float4 rgba = geom_rgba(g, w, gid, &mat_id);
-  sf.base_color = V3{rgba.x, rgba.y, rgba.z};
-  sf.alpha = rgba.w;
-  sf.metallic = bat(m.geom_metallic, m.geom_metallic_n, w, g.ngeom)[gid];
-  sf.perceptual_roughness = bat(m.geom_roughness, m.geom_roughness_n, w, g.ngeom)[gid];
-  sf.reflectance = bat(m.geom_reflectance, m.geom_reflectance_n, w, g.ngeom)[gid];
-  float emission = mat_id >= 0 ? bat(m.mat_emission, m.mat_emission_n, w, g.nmat)[mat_id] : 0.0f;
-
-  int gtype = g.geom_type[gid];
-  const float* gmat = g.geom_xmat + ((int64_t)w * g.ngeom + gid) * 9;
+  const float4* sr = reinterpret_cast<const float4*>(sv.surfs + gid * kSurfRec);
+  float4 r0 = sr[0], r1 = sr[1], r2 = sr[2], r3 = sr[3];
+  sf.base_color = V3{r0.x, r0.y, r0.z};
+  sf.alpha = r0.w;
+  sf.metallic = r1.x;
+  sf.perceptual_roughness = r1.y;
+  sf.reflectance = r1.z;
+  float emission = r1.w;
+  int mat_id = __float_as_int(r2.x);
+  int gtype = __float_as_int(r3.y);
+  const float* grec = sv.geoms + kGeomRec * gid;
   float R[9];
 #pragma unroll
-  for (int k = 0; k < 9; ++k) R[k] = gmat[k];
+  for (int k = 0; k < 9; ++k) R[k] = grec[4 + k];
   V3 local_dir = mat_tmul(R, dir);
   V3 local_normal = mat_tmul(R, hit.normal);
   if (dot(local_normal, local_dir) > 0.0f) local_normal = -local_normal;
@@ -542,7 +589,7 @@ MJ_DEV Surface eval_surface(const MjpbrScene& sc, const MjpbrSettings& s, int w,
     }
   }
   V3 size = V3{0.0f, 0.0f, 0.0f};
-  if (!is_mesh) size = ld3(bat(g.geom_size, g.geom_size_n, w, 3 * g.ngeom) + 3 * gid);
+  if (!is_mesh) size = V3{grec[3], grec[13], grec[14]};
   if (!is_mesh && s.specular_aa) {
     if (gtype == MJPBR_GEOM_SPHERE || gtype == MJPBR_GEOM_CAPSULE || gtype == MJPBR_GEOM_CYLINDER)
       curvature = 1.0f / fmaxf(size.x, 1.0e-9f);
@@ -554,9 +601,8 @@ MJ_DEV Surface eval_surface(const MjpbrScene& sc, const MjpbrSettings& s, int w,
 
   if (s.textures && mat_id >= 0) {
     int mw = w % m.mat_texid_n;
-    const float* rp = bat(m.mat_texrepeat, m.mat_texrepeat_n, w, 2 * g.nmat) + 2 * mat_id;
-    V2 repeat = V2{rp[0], rp[1]};
-    bool uniform = bat(m.mat_texuniform, m.mat_texuniform_n, w, g.nmat)[mat_id] != 0;
+    V2 repeat = V2{r2.z, r2.w};
+    bool uniform = __float_as_int(r2.y) != 0;
     bool has_uv = false;
     V2 uv = V2{0.0f, 0.0f};
     V3 tangent = v3s(0.0f), bitangent = v3s(0.0f);
@@ -565,7 +611,7 @@ MJ_DEV Surface eval_surface(const MjpbrScene& sc, const MjpbrSettings& s, int w,
       if (mesh_id >= 0 && face_id >= 0)
         has_uv = mesh_triangle_frame(m, mesh_id, face_id, hit.bary_u, hit.bary_v, repeat, &uv, &tangent, &bitangent, &upl);
     } else {
-      V3 gpos = ld3(g.geom_xpos + ((int64_t)w * g.ngeom + gid) * 3);
+      V3 gpos = V3{grec[0], grec[1], grec[2]};
       V3 local = mat_tmul(R, point - gpos);
       V3 t1, t2;
       orthonormal_basis(local_normal, &t1, &t2);
@@ -659,6 +705,33 @@ MJ_DEV float mirror_reflectance(const MjpbrScene& sc, int w, int gid) {
   int mid = bat(sc.geo.geom_matid, sc.geo.geom_matid_n, w, sc.geo.ngeom)[gid];
   if (mid < 0) return 0.0f;
   return bat(sc.mats.mat_reflectance, sc.mats.mat_reflectance_n, w, sc.geo.nmat)[mid];
+}
+
+// Staged surface record of geom gi in world w (kSurfRec floats): [0..3] rgba (geom_rgba),
+// [4] metallic, [5] perceptual roughness, [6] reflectance, [7] material emission, [8] material
+// id (bits), [9] texuniform (bits), [10..11] texrepeat, [12] mirror reflectance, [13] geom type
+// (bits), [14..15] unused.
+MJ_DEV void stage_surface(const MjpbrScene& sc, int w, int gi, float* r) {
+  const MjpbrGeometry& g = sc.geo;
+  const MjpbrMaterials& m = sc.mats;
+  int mid;
+  float4 rgba = geom_rgba(g, w, gi, &mid);
+  r[0] = rgba.x;
+  r[1] = rgba.y;
+  r[2] = rgba.z;
+  r[3] = rgba.w;
+  r[4] = bat(m.geom_metallic, m.geom_metallic_n, w, g.ngeom)[gi];
+  r[5] = bat(m.geom_roughness, m.geom_roughness_n, w, g.ngeom)[gi];
+  r[6] = bat(m.geom_reflectance, m.geom_reflectance_n, w, g.ngeom)[gi];
+  r[7] = mid >= 0 ? bat(m.mat_emission, m.mat_emission_n, w, g.nmat)[mid] : 0.0f;
+  r[8] = __int_as_float(mid);
+  r[9] = __int_as_float(mid >= 0 ? bat(m.mat_texuniform, m.mat_texuniform_n, w, g.nmat)[mid] : 0);
+  const float* rp = mid >= 0 ? bat(m.mat_texrepeat, m.mat_texrepeat_n, w, 2 * g.nmat) + 2 * mid : nullptr;
+  r[10] = rp ? rp[0] : 0.0f;
+  r[11] = rp ? rp[1] : 0.0f;
+  r[12] = mirror_reflectance(sc, w, gi);
+  r[13] = __int_as_float(g.geom_type[gi]);
+  r[14] = r[15] = 0.0f;
 }
 
 MJ_DEV void env_lookup(const MjpbrEnvironment& env, int w, int* e, float* yaw, float* scale) {

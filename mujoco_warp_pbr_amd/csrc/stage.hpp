@@ -1,14 +1,11 @@
-// stage.hpp — cooperative load of one world's scene BVH and geom frames into LDS, so every
-// closest-hit and shadow query of the workgroup traverses LDS instead of HBM.
+// stage.hpp — cooperative load of one world's scene BVH, geom frames and light records into
+// LDS, so every closest-hit and shadow query of the workgroup traverses LDS instead of HBM, and
+// the shading code reads each light and each hit geom's material constants as LDS vectors.
 #pragma once
 #include "camera.hpp"
 
 namespace mjpbr {
 
-MJ_DEV size_t stage_bytes(const MjpbrGeometry& g) {
-  int nleaf = 1 << g.tlas_depth;
-  return (size_t)((nleaf - 1) * 12 + g.ngeom * kGeomRec + nleaf) * 4;
-}
 
 MJ_DEV SceneView stage_world(const MjpbrScene& sc, int w, float* lds) {
   const MjpbrGeometry& g = sc.geo;
@@ -17,6 +14,10 @@ MJ_DEV SceneView stage_world(const MjpbrScene& sc, int w, float* lds) {
   float* box = lds;
   float* geoms = box + ninner * 12;
   int* leaf = reinterpret_cast<int*>(geoms + g.ngeom * kGeomRec);
+  float* lights = reinterpret_cast<float*>(leaf + staged_leaf_slots(nleaf));
+  for (int li = threadIdx.x; li < sc.lit.nlight; li += blockDim.x) stage_light(sc.lit, w, li, lights + li * kLightRec);
+  float* surfs = lights + sc.lit.nlight * kLightRec;
+  for (int gi = threadIdx.x; gi < g.ngeom; gi += blockDim.x) stage_surface(sc, w, gi, surfs + gi * kSurfRec);
   const float4* src = reinterpret_cast<const float4*>(g.tlas_box + (int64_t)w * ninner * 12);
   float4* dst = reinterpret_cast<float4*>(box);
   for (int i = threadIdx.x; i < ninner * 3; i += blockDim.x) dst[i] = src[i];
@@ -39,8 +40,9 @@ MJ_DEV SceneView stage_world(const MjpbrScene& sc, int w, float* lds) {
     float4 rgba = geom_rgba(g, w, gi, &mid);
     int type = g.geom_type[gi];
     int dataid = bat(g.geom_dataid, g.geom_dataid_n, w, g.ngeom)[gi];
-    if (type == MJPBR_GEOM_HFIELD && dataid >= 0) dataid += g.nmesh;  // traced as a mesh
-    int bits = type | (rgba.w == 0.0f ? 0x100 : 0) | ((dataid + 1) << 9);
+    if (type == MJPBR_GEOM_HFIELD && dataid >= 0) dataid += g.nmesh;
+    int traced = (type == MJPBR_GEOM_SDF || type == MJPBR_GEOM_HFIELD) ? MJPBR_GEOM_MESH : type;
+    int bits = traced | (rgba.w == 0.0f ? 0x100 : 0) | ((dataid + 1) << 9);
     r[15] = __int_as_float(bits);
   }
   __syncthreads();
@@ -49,6 +51,8 @@ MJ_DEV SceneView stage_world(const MjpbrScene& sc, int w, float* lds) {
   sv.tlas_leaf = leaf;
   sv.geoms = geoms;
   sv.depth = g.tlas_depth;
+  sv.lights = lights;
+  sv.surfs = surfs;
   sv.blas_box = g.blas_box;
   sv.blas_tri = g.blas_tri;
   sv.blas_depth = g.blas_depth;

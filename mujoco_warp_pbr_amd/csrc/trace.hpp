@@ -18,9 +18,16 @@ namespace mjpbr {
 
 // Staged geom record: 16 floats.
 //   [0..2] xpos  [3] size.x  [4..12] xmat  [13] size.y  [14] size.z  [15] bits
-// bits = type | alpha_zero << 8 | (mesh + 1) << 9, mesh = the traced mesh (dataid of a mesh or
-// SDF geom, nmesh + dataid of a height field)
+// bits = type | alpha_zero << 8 | (mesh + 1) << 9. SDF geoms (traverse.py:185-196) and height
+// fields (traverse.py:161-165) are staged as type MESH: mesh = dataid for an SDF geom, nmesh +
+// dataid for a height field's triangle surface.
 constexpr int kGeomRec = 16;
+// Staged light record: 24 floats (see stage_light).
+constexpr int kLightRec = 24;
+// Staged surface record per geom: 16 floats (see stage_surface).
+constexpr int kSurfRec = 16;
+// Leaf ids are padded to a multiple of 4 so the light records after them are 16-byte aligned.
+MJ_HOST_DEV inline int staged_leaf_slots(int nleaf) { return (nleaf + 3) & ~3; }
 
 struct Hit {
   int geom_id;  // -1: miss
@@ -28,13 +35,15 @@ struct Hit {
   V3 normal;
   float bary_u, bary_v;  // weights of the triangle's first and second vertex (meshes)
   int face_id;
-  int mesh_id;
+  int mesh_id;  // traced mesh; read only for mesh and SDF geoms (height fields shade as primitives)
 };
 
 struct SceneView {
   const float* tlas_box;  // [ninner][12] (LDS)
   const int* tlas_leaf;   // [nleaf] (LDS)
   const float* geoms;     // [ngeom][16] (LDS)
+  const float* lights;    // [nlight][kLightRec] (LDS)
+  const float* surfs;     // [ngeom][kSurfRec] (LDS)
   int depth;
   // mesh BVHs (global)
   const float* blas_box;
@@ -177,11 +186,7 @@ MJ_DEV float intersect_geom(const SceneView& sv, const float* g, V3 o, V3 d, flo
       return ray_cylinder(pos, mat, size, o, d, n);
     case MJPBR_GEOM_BOX:
       return ray_box(pos, mat, size, o, d, n);
-    case MJPBR_GEOM_MESH:
-    case MJPBR_GEOM_SDF:
-    case MJPBR_GEOM_HFIELD: {
-      // SDF geoms trace their mesh (traverse.py:185-196); a height field traces its triangle
-      // surface and reports no mesh id, so it is shaded like a primitive (traverse.py:161-165).
+    case MJPBR_GEOM_MESH: {  // also SDF geoms and height fields, staged as meshes
       int mid = (bits >> 9) - 1;
       if (mid < 0) return -1.0f;
       V3 lo = mat_tmul(mat, o - pos);
@@ -198,7 +203,7 @@ MJ_DEV float intersect_geom(const SceneView& sv, const float* g, V3 o, V3 d, flo
       *bu = 1.0f - b1 - b2;
       *bv = b1;
       *face = f;
-      *mesh = type == MJPBR_GEOM_HFIELD ? -1 : mid;
+      *mesh = mid;
       return t;
     }
     default:

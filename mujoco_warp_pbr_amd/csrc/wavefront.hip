@@ -240,7 +240,7 @@ extern "C" int mjpbr_pt_wavefront_prepare(const MjpbrScene* scenes, const MjpbrO
       delete h;
       return rc;
     }
-    size_t b = host_stage_bytes(&scenes[g].geo);
+    size_t b = host_stage_bytes(&scenes[g]);
     if (b > 64 * 1024) {
       delete h;
       return set_error(MJPBR_EINVAL, "wavefront: group %d scene too large for LDS staging (%d geoms)", g, scenes[g].geo.ngeom);

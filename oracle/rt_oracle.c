@@ -135,19 +135,22 @@ static float box_o(v3 pos, const float* mat, v3 size, v3 pnt, v3 vec, v3* n) {
 
 /* Moller-Trumbore on (v0, e1, e2); 0 <= t < tmax; b1, b2 weights of v1, v2. */
 static int tri_o(v3 o, v3 d, v3 v0, v3 e1, v3 e2, float tmax, float* t, float* b1, float* b2) {
+  /* Moller-Trumbore with the edge tests on the numerators, sign-normalised by det. */
   v3 p = cross3(d, e2);
   float det = dot3(e1, p);
   if (fabsf(det) < 1.0e-20f) return 0;
-  float inv = 1.0f / det;
   v3 s = sub3(o, v0);
-  float u = dot3(s, p) * inv;
-  if (u < 0.0f || u > 1.0f) return 0;
+  float us = dot3(s, p), adet = fabsf(det);
+  if (det < 0.0f) us = -us;
+  if (us < 0.0f || us > adet) return 0;
   v3 q = cross3(s, e1);
-  float v = dot3(d, q) * inv;
-  if (v < 0.0f || u + v > 1.0f) return 0;
-  float tt = dot3(e2, q) * inv;
-  if (tt < 0.0f || tt >= tmax) return 0;
-  *t = tt; *b1 = u; *b2 = v;
+  float vs = dot3(d, q), ts = dot3(e2, q);
+  if (det < 0.0f) { vs = -vs; ts = -ts; }
+  if (vs < 0.0f || us + vs > adet || ts < 0.0f) return 0;
+  float inv = 1.0f / adet;
+  float tt = ts * inv;
+  if (tt >= tmax) return 0;
+  *t = tt; *b1 = us * inv; *b2 = vs * inv;
   return 1;
 }
 
@@ -204,7 +207,7 @@ static float intersect_o(const Ctx* c, int gi, v3 o, v3 d, float max_t, int any_
     case MJPBR_GEOM_SDF:
     case MJPBR_GEOM_HFIELD: {
       /* SDF geoms trace their mesh; height field h traces its triangle surface, mesh nmesh + h,
-         and reports no mesh id (traverse.py:161-165, 185-196). */
+         and shades as a primitive (traverse.py:161-165, 185-196). */
       int mid = g->geom_dataid[bidx(w, g->geom_dataid_n) * g->ngeom + gi];
       if (mid < 0) return -1.0f;
       if (type == MJPBR_GEOM_HFIELD) mid += g->nmesh;
@@ -250,7 +253,7 @@ static float intersect_o(const Ctx* c, int gi, v3 o, v3 d, float max_t, int any_
       *bu = 1.0f - bb1 - bb2;
       *bv = bb1;
       *face = best_face;
-      *mesh = type == MJPBR_GEOM_HFIELD ? -1 : mid;
+      *mesh = mid;
       return best;
     }
     default: return -1.0f;
