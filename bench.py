@@ -128,6 +128,62 @@ def cpu_baseline(mjm, m, d, rc, target_s: float = 12.0) -> dict:
   }
 
 
+# The reference's published real-time rates for its suite at 1024 worlds x 128^2 (frames/s x
+# 16384 = Mpixels/s): results/suite/suite-v2/README.md:54-55 (RTX PRO 6000 Blackwell), :129-130
+# (H200), :204-205 (A100-SXM4-80GB).
+SUITE_PUBLISHED_MPX = {
+  "material_grid": {"rtx_pro_6000": 9635.4, "h200": 4614.0, "a100": 2718.0},
+  "primitives": {"rtx_pro_6000": 5605.0, "h200": 1976.0, "a100": 1201.0},
+}
+
+
+def reference_suite_section(device, nworld: int = 1024, res: int = 128, reps: int = 20) -> dict:
+  """The reference suite's two procedural scenes (material_grid, primitives) at its batch shape:
+  1024 worlds x 128^2, one camera per world, identical cameras, rgb only, 1 spp, default
+  settings (shadows on, PBR neutral tone mapping, default EV100), a 1024 x 512 HDR environment
+  (procedural here; the suite's studio HDR file is not shipped). The render is timed with HIP
+  events over `reps` back-to-back launches, as the suite times graph replays of renders."""
+  import torch
+
+  import mujoco_warp_pbr_amd as mj
+  from mujoco_warp_pbr_amd import scenes
+
+  out = {}
+  stream = torch.cuda.current_stream()
+  for name, fn in (("material_grid", scenes.suite_material_grid), ("primitives", scenes.suite_primitives)):
+    mjm = fn(res)
+    m = mj.put_model(mjm, device)
+    d = mj.put_data(mjm, nworld, device)
+    rc = mj.create_render_context(mjm, nworld=nworld, cam_res=(res, res), render_rgb=True, render_depth=False, render_seg=False,
+                                  envmap=scenes.procedural_sky(3, 512, 1024), device=device)
+    mj.refit_bvh(m, d, rc)
+    for _ in range(3):
+      mj.render(m, d, rc)
+    torch.cuda.synchronize()
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record(stream)
+    for _ in range(reps):
+      mj.render(m, d, rc)
+    b.record(stream)
+    b.synchronize()
+    ms = a.elapsed_time(b) / reps
+    mpx = nworld * res * res / ms / 1e3
+    pub = SUITE_PUBLISHED_MPX[name]
+    log(f"suite {name}: {ms:.3f} ms = {mpx:.0f} Mpx/s (reference H200 {pub['h200']:.0f}, RTX PRO 6000 {pub['rtx_pro_6000']:.0f})")
+    out[name] = {
+      "ms": round(ms, 4),
+      "Mpixels_s": round(mpx, 1),
+      "frames_s": round(nworld / ms * 1e3, 1),
+      "reference_published_Mpixels_s": pub,
+      "vs_h200": round(mpx / pub["h200"], 3),
+      "vs_rtx_pro_6000": round(mpx / pub["rtx_pro_6000"], 3),
+    }
+    del rc, m, d
+    torch.cuda.empty_cache()
+  out["workload"] = f"reference suite scenes, {nworld} worlds x {res}^2, rgb only, 1 spp, shadows, 1024x512 HDR environment"
+  return out
+
+
 C2_GROUPS = ((2048, 64), (1536, 128), (512, 256))
 
 
@@ -330,10 +386,13 @@ def main(argv=None) -> int:
     log("cpu baseline (oracle)")
     baseline = cpu_baseline(mjm, m, d, rc)
 
+  del rc, m, d
+  torch.cuda.empty_cache()
+  log("reference suite scenes (material_grid, primitives)")
+  suite = reference_suite_section(device)
+
   pt_wavefront = learned_denoiser = learned_denoiser_mesh = None
   if not args.no_pt:
-    del rc, m, d
-    torch.cuda.empty_cache()
     log("path tracing: wavefront batch (4096 mixed-resolution worlds)")
     pt_wavefront = pt_wavefront_section(device)
     torch.cuda.empty_cache()
@@ -372,6 +431,7 @@ def main(argv=None) -> int:
       "bytes_per_pixel": round(bpp, 3),
     },
     "cpu_baseline": baseline,
+    "reference_suite": suite,
     "pt_wavefront": pt_wavefront,
     "learned_denoiser": learned_denoiser,
     "learned_denoiser_mesh_256": learned_denoiser_mesh,

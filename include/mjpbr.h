@@ -222,7 +222,9 @@ typedef struct MjpbrSettings {
   int32_t render_guides;
   /* path tracer */
   int32_t path_samples, max_depth, rr_depth;
-  int32_t pad3;
+  /* 1 when an enabled geom is a mesh, SDF or height field (reference io.py:317 `has_meshes`);
+   * 0 selects kernel instances without mesh traversal, which then skip such geoms. */
+  int32_t has_meshes;
 } MjpbrSettings;
 
 /* Output buffers (reference: kernel_inputs.py:190-204 `Outputs`). Per-world flat images:
@@ -310,6 +312,11 @@ int mjpbr_scene_bvh(const MjpbrGeometry* geo, int refit, void* stream);
  * Replaces `render` with tier REALTIME (reference render.py:430-460, megakernel :400-427). */
 int mjpbr_rt_render(const MjpbrScene* scene, const MjpbrSettings* s, MjpbrOutputs* out, int32_t seed,
                     void* stream);
+
+/* Feature mask of the real-time kernel instance mjpbr_rt_render launches for these settings
+ * (bits: 1 meshes, 2 supersampling, 4 soft shadows, 8 mirror, 16 layers, 32 textures): the
+ * reference's kernel specialisation key (kernel_inputs.py:321-337) as compiled instances. */
+int mjpbr_rt_kernel_features(const MjpbrSettings* s);
 
 /* Path-traced tier megakernel: one thread per (world, pixel), `path_samples` paths each
  * (reference pathtrace.py:340-438). With s->denoise it writes the noisy HDR and the first-hit

@@ -12,6 +12,17 @@ namespace mjpbr {
 
 constexpr float kMinThroughput = 1.0f / 255.0f;  // render.py:62
 
+// Compile-time feature mask of a kernel instance (the reference bakes the same choices into its
+// kernel key, kernel_inputs.py:321-337 and io.py:300-330). A bit that is clear removes code
+// whose runtime condition is false for every launch the instance is picked for:
+//   kFMesh    mesh / SDF / height-field geoms are rendered (MjpbrSettings.has_meshes)
+//   kFSS      samples_per_axis > 1
+//   kFSoft    shadows with shadow_samples > 1
+//   kFMirror  has_reflection
+//   kFLayer   has_transparency
+//   kFTex     textures
+enum : unsigned { kFMesh = 1, kFSS = 2, kFSoft = 4, kFMirror = 8, kFLayer = 16, kFTex = 32, kFAll = 63 };
+
 struct RtCtx {
   const MjpbrScene& sc;
   const MjpbrSettings& s;
@@ -28,15 +39,17 @@ MJ_DEV V2 shadow_rand2(int seed, int stream, int nlight, int li, int nsample, in
 }
 
 // render.py:95-127
+template <unsigned F>
 MJ_DEV float light_visibility(const RtCtx& c, const Light& L, V3 point, V3 normal, V3 l, float dist, int stream,
                               int nlight, int li) {
+  constexpr bool kMesh = F & kFMesh;
   const MjpbrSettings& s = c.s;
   V3 origin = point + normal * kRayOffset;
   bool cull = s.cull_backfaces != 0;
   if (L.type == MJPBR_LIGHT_DIRECTIONAL)
-    return trace_any(c.sv, origin, l, kUnboundedShadow, cull) ? s.shadow_light_fraction : 1.0f;
-  if (s.shadow_samples <= 1)
-    return trace_any(c.sv, origin, l, dist - kShadowGap, cull) ? s.shadow_light_fraction : 1.0f;
+    return trace_any<kMesh>(c.sv, origin, l, kUnboundedShadow, cull) ? s.shadow_light_fraction : 1.0f;
+  if (!(F & kFSoft) || s.shadow_samples <= 1)
+    return trace_any<kMesh>(c.sv, origin, l, dist - kShadowGap, cull) ? s.shadow_light_fraction : 1.0f;
   V3 t1, t2;
   orthonormal_basis(l, &t1, &t2);
   float visible = 0.0f;
@@ -47,13 +60,14 @@ MJ_DEV float light_visibility(const RtCtx& c, const Light& L, V3 point, V3 norma
     sincos_turns(u.y, &sp, &cp);
     V3 to_target = L.pos + (t1 * cp + t2 * sp) * r - origin;
     float d = len(to_target);
-    if (!trace_any(c.sv, origin, to_target / d, d - kShadowGap, cull)) visible += 1.0f;
+    if (!trace_any<kMesh>(c.sv, origin, to_target / d, d - kShadowGap, cull)) visible += 1.0f;
   }
   float v = visible / (float)s.shadow_samples;
   return v + (1.0f - v) * s.shadow_light_fraction;
 }
 
 // render.py:129-161
+template <unsigned F>
 MJ_DEV V3 shade_light(const RtCtx& c, const Light& L, const Lobes& lb, V3 point, V3 normal, V3 view, float nov,
                       int stream, int nlight, int li) {
   V3 l, ds, ss;
@@ -62,7 +76,7 @@ MJ_DEV V3 shade_light(const RtCtx& c, const Light& L, const Lobes& lb, V3 point,
   float nol = dot(normal, l);
   if (nol <= 0.0f || (len_sq(ds) == 0.0f && len_sq(ss) == 0.0f)) return v3s(0.0f);
   float vis = 1.0f;
-  if (c.s.use_shadows && L.castshadow) vis = light_visibility(c, L, point, normal, l, dist, stream, nlight, li);
+  if (c.s.use_shadows && L.castshadow) vis = light_visibility<F>(c, L, point, normal, l, dist, stream, nlight, li);
   V3 h = normalize(l + view);
   float noh = fmaxf(dot(normal, h), 0.0f);
   float loh = fmaxf(dot(l, h), 0.0f);
@@ -72,6 +86,7 @@ MJ_DEV V3 shade_light(const RtCtx& c, const Light& L, const Lobes& lb, V3 point,
 }
 
 // render.py:184-239
+template <unsigned F>
 MJ_DEV V3 shade_surface(const RtCtx& c, const Surface& sf, V3 point, V3 normal, V3 view, V3 head_dir, int stream) {
   const MjpbrScene& sc = c.sc;
   const MjpbrSettings& s = c.s;
@@ -108,7 +123,7 @@ MJ_DEV V3 shade_surface(const RtCtx& c, const Surface& sf, V3 point, V3 normal, 
       nlight = 1;
       lidx = 0;
     }
-    radiance += shade_light(c, L, lb, point, normal, view, nov, stream, nlight, lidx);
+    radiance += shade_light<F>(c, L, lb, point, normal, view, nov, stream, nlight, lidx);
   }
   return radiance + sf.emissive / c.exposure;
 }
@@ -141,13 +156,17 @@ MJ_DEV void write_guides(const MjpbrOutputs& out, int w, const Pixel& pix, V3 al
 //
 // The first sample of a pixel (`write_first`) writes depth, segmentation and the guides of its
 // first hit as soon as they are known, so nothing of that hit stays live through shading.
+template <unsigned F>
 MJ_DEV V3 trace_pixel(const RtCtx& c, V3 origin, V3 dir, float max_dist, float footprint, float spread, V3 head_dir,
                       int stream, bool write_first, const MjpbrOutputs& out, const Pixel& pix, V3 dir_cam) {
+  constexpr bool kMesh = F & kFMesh;
   const MjpbrSettings& s = c.s;
   const bool cull = s.cull_backfaces != 0;
+  const bool layers = (F & kFLayer) && s.has_transparency;
+  const bool reflect = (F & kFMirror) && s.has_reflection;
   V3 radiance = v3s(0.0f);
   float throughput = 1.0f, travelled = 0.0f;
-  const int nlayer = s.has_transparency ? s.max_layers : 1;
+  const int nlayer = layers ? s.max_layers : 1;
   int layer = 0;
   bool mirror_seg = false;
   // state of the layer whose mirror ray is being traced
@@ -156,7 +175,7 @@ MJ_DEV V3 trace_pixel(const RtCtx& c, V3 origin, V3 dir, float max_dist, float f
   V3 ro = origin, rd = dir;
   float rmax = max_dist - travelled;
   for (;;) {
-    Hit hit = trace_closest(c.sv, ro, rd, rmax, cull);
+    Hit hit = trace_closest<kMesh>(c.sv, ro, rd, rmax, cull);
     const bool first = write_first && !mirror_seg && layer == 0;
     if (first) write_geometry(out, c.w, pix, hit, dir_cam);
     V3 shaded;
@@ -174,13 +193,13 @@ MJ_DEV V3 trace_pixel(const RtCtx& c, V3 origin, V3 dir, float max_dist, float f
     } else {
       point = ro + rd * hit.dist;
       cone = mirror_seg ? layer_cone + spread * hit.dist : footprint + spread * (travelled + hit.dist);
-      Surface sf = eval_surface(c.sc, c.sv, s, c.w, hit, point, rd, cone, &normal);
+      Surface sf = eval_surface<kMesh, (F & kFTex) != 0>(c.sc, c.sv, s, c.w, hit, point, rd, cone, &normal);
       alpha = sf.alpha;
       if (first && s.render_guides) write_guides(out, c.w, pix, sf.base_color, normal);
-      shaded = shade_surface(c, sf, point, normal, -rd, head_dir, stream + (mirror_seg ? 1 : 0));
+      shaded = shade_surface<F>(c, sf, point, normal, -rd, head_dir, stream + (mirror_seg ? 1 : 0));
     }
     if (!mirror_seg) {
-      float mirror = s.has_reflection && hit.geom_id >= 0 ? c.sv.surfs[hit.geom_id * kSurfRec + 12] : 0.0f;
+      float mirror = reflect && hit.geom_id >= 0 ? c.sv.surfs[hit.geom_id * kSurfRec + 12] : 0.0f;
       if (mirror > 0.0f) {
         layer_shaded = shaded;
         layer_point = point;
@@ -201,7 +220,7 @@ MJ_DEV V3 trace_pixel(const RtCtx& c, V3 origin, V3 dir, float max_dist, float f
       shaded = layer_shaded * (1.0f - layer_mirror) + shaded * layer_mirror;
       mirror_seg = false;
     }
-    if (s.has_transparency) {
+    if (layers) {
       float opacity = clampf(layer_alpha, 0.0f, 1.0f);
       radiance += shaded * (throughput * opacity);
       throughput *= 1.0f - opacity;
@@ -221,6 +240,7 @@ MJ_DEV V3 trace_pixel(const RtCtx& c, V3 origin, V3 dir, float max_dist, float f
 
 // Occupancy over registers: the kernel is latency bound (dependent BVH and texture loads), and 8
 // waves per SIMD with spills to scratch beat 2 waves without (measured 1.9x at 1024 x 128^2).
+template <unsigned F>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8)))
 rt_kernel(MjpbrScene sc, MjpbrSettings s, MjpbrOutputs out, int seed, TileMap tm) {
   extern __shared__ float lds[];
@@ -247,11 +267,11 @@ rt_kernel(MjpbrScene sc, MjpbrSettings s, MjpbrOutputs out, int seed, TileMap tm
     V3 o, d, dc;
     float md;
     primary_ray(pix, s, 1, 0, 0, &o, &d, &md, &dc);
-    Hit hit = trace_closest(sv, o, d, md, cull);
+    Hit hit = trace_closest<(F & kFMesh) != 0>(sv, o, d, md, cull);
     write_geometry(out, w, pix, hit, dc);
     return;
   }
-  int n_aa = s.samples_per_axis;
+  const int n_aa = (F & kFSS) ? s.samples_per_axis : 1;
   float inv_aa = 1.0f / (float)n_aa;
   float pspread, pwidth;
   pixel_cone(pix, &pspread, &pwidth);
@@ -263,7 +283,7 @@ rt_kernel(MjpbrScene sc, MjpbrSettings s, MjpbrOutputs out, int seed, TileMap tm
     V3 o, d, dc;
     float md;
     primary_ray(pix, s, n_aa, smp % n_aa, smp / n_aa, &o, &d, &md, &dc);
-    V3 radiance = trace_pixel(c, o, d, md, width, spread, head_dir, 2 * smp, smp == 0, out, pix, dc);
+    V3 radiance = trace_pixel<F>(c, o, d, md, width, spread, head_dir, 2 * smp, smp == 0, out, pix, dc);
     display += display_transform(radiance * exposure, s.tone_mapper);
     hdr += radiance;
   }
@@ -277,6 +297,54 @@ rt_kernel(MjpbrScene sc, MjpbrSettings s, MjpbrOutputs out, int seed, TileMap tm
 
 using namespace mjpbr;
 
+namespace {
+
+using RtKernelFn = void (*)(MjpbrScene, MjpbrSettings, MjpbrOutputs, int, TileMap);
+
+struct RtInstance {
+  unsigned mask;
+  RtKernelFn fn;
+};
+
+// Compiled instances, each launched for the settings whose required features it covers.
+#define MJPBR_RT_INSTANCE(F) RtInstance{F, rt_kernel<F>}
+const RtInstance kRtInstances[] = {
+    MJPBR_RT_INSTANCE(0u),
+    MJPBR_RT_INSTANCE(kFTex),
+    MJPBR_RT_INSTANCE(kFMirror | kFTex),
+    MJPBR_RT_INSTANCE(kFMesh),
+    MJPBR_RT_INSTANCE(kFMesh | kFTex),
+    MJPBR_RT_INSTANCE(kFMesh | kFSS | kFSoft | kFMirror | kFTex),
+    MJPBR_RT_INSTANCE(kFAll),
+};
+#undef MJPBR_RT_INSTANCE
+
+unsigned required_features(const MjpbrSettings& s) {
+  unsigned f = 0;
+  if (s.has_meshes) f |= kFMesh;
+  if (s.samples_per_axis > 1) f |= kFSS;
+  if (s.use_shadows && s.shadow_samples > 1) f |= kFSoft;
+  if (s.has_reflection) f |= kFMirror;
+  if (s.has_transparency) f |= kFLayer;
+  if (s.textures) f |= kFTex;
+  return f;
+}
+
+// The instance with the fewest features among those covering `need` (kFAll covers everything).
+const RtInstance& pick_instance(unsigned need) {
+  const RtInstance* best = nullptr;
+  for (const RtInstance& k : kRtInstances)
+    if ((k.mask & need) == need && (!best || __builtin_popcount(k.mask) < __builtin_popcount(best->mask))) best = &k;
+  return *best;
+}
+
+}  // namespace
+
+extern "C" int mjpbr_rt_kernel_features(const MjpbrSettings* s) {
+  if (!s) return set_error(MJPBR_EINVAL, "settings must not be NULL");
+  return (int)pick_instance(required_features(*s)).mask;
+}
+
 extern "C" int mjpbr_rt_render(const MjpbrScene* scene, const MjpbrSettings* s, MjpbrOutputs* out, int32_t seed,
                                void* stream) {
   TileMap tm;
@@ -286,7 +354,7 @@ extern "C" int mjpbr_rt_render(const MjpbrScene* scene, const MjpbrSettings* s, 
   if (s->samples_per_axis < 1) return set_error(MJPBR_EINVAL, "samples_per_axis must be at least 1");
   size_t lds = host_stage_bytes(scene);
   if (lds > 64 * 1024) return set_error(MJPBR_EINVAL, "scene too large for LDS staging (%d geoms, %d lights)", scene->geo.ngeom, scene->lit.nlight);
-  hipLaunchKernelGGL(rt_kernel, dim3(nlaunch), dim3(256), lds, (hipStream_t)stream, *scene, *s, *out, seed,
-                     tm);
+  const RtInstance& k = pick_instance(required_features(*s));
+  hipLaunchKernelGGL(k.fn, dim3(nlaunch), dim3(256), lds, (hipStream_t)stream, *scene, *s, *out, seed, tm);
   return check_launch("rt_kernel");
 }

@@ -540,6 +540,9 @@ MJ_DEV float4 fetch_role(const MjpbrMaterials& m, int tex, V2 uv, float cone, fl
 
 // material.py:300-501. Returns the surface and the viewer-facing world shading normal.
 // The hit geom's material constants, frame and size come from the world's staged records (LDS).
+// kMesh = false / kTex = false drop the mesh-attribute and texture branches (scenes without
+// mesh-like geoms / settings without textures).
+template <bool kMesh = true, bool kTex = true>
 MJ_DEV Surface eval_surface(const MjpbrScene& sc, const SceneView& sv, const MjpbrSettings& s, int w, const Hit& hit,
                             V3 point, V3 dir, float footprint, V3* shading_world) {
   const MjpbrGeometry& g = sc.geo;
@@ -567,7 +570,7 @@ MJ_DEV Surface eval_surface(const MjpbrScene& sc, const SceneView& sv, const Mjp
   if (dot(local_normal, local_dir) > 0.0f) local_normal = -local_normal;
   V3 shading = local_normal;
   float curvature = 0.0f;
-  bool is_mesh = gtype == MJPBR_GEOM_MESH || gtype == MJPBR_GEOM_SDF;
+  bool is_mesh = kMesh && (gtype == MJPBR_GEOM_MESH || gtype == MJPBR_GEOM_SDF);
   int mesh_id = hit.mesh_id, face_id = hit.face_id;
 
   if (is_mesh && s.vertex_normals && mesh_id >= 0 && face_id >= 0) {
@@ -599,7 +602,7 @@ MJ_DEV Surface eval_surface(const MjpbrScene& sc, const SceneView& sv, const Mjp
 
   float cone = footprint / fmaxf(fabsf(dot(local_normal, local_dir)), 0.05f);
 
-  if (s.textures && mat_id >= 0) {
+  if (kTex && s.textures && mat_id >= 0) {
     int mw = w % m.mat_texid_n;
     V2 repeat = V2{r2.z, r2.w};
     bool uniform = __float_as_int(r2.y) != 0;

@@ -164,8 +164,9 @@ MJ_DEV bool mesh_any(const SceneView& sv, int mesh, V3 lo, V3 ld, float max_t) {
   return hit;
 }
 
-// One geom against the ray (traverse.py:131-192). Returns the distance or -1.
-template <bool kAnyHit>
+// One geom against the ray (traverse.py:131-192). Returns the distance or -1. kMesh = false
+// drops the mesh / SDF / height-field branch (for scenes without such geoms).
+template <bool kAnyHit, bool kMesh = true>
 MJ_DEV float intersect_geom(const SceneView& sv, const float* g, V3 o, V3 d, float max_t, V3* n, float* bu,
                             float* bv, int* face, int* mesh) {
   int bits = __float_as_int(g[15]);
@@ -187,6 +188,7 @@ MJ_DEV float intersect_geom(const SceneView& sv, const float* g, V3 o, V3 d, flo
     case MJPBR_GEOM_BOX:
       return ray_box(pos, mat, size, o, d, n);
     case MJPBR_GEOM_MESH: {  // also SDF geoms and height fields, staged as meshes
+      if (!kMesh) return -1.0f;
       int mid = (bits >> 9) - 1;
       if (mid < 0) return -1.0f;
       V3 lo = mat_tmul(mat, o - pos);
@@ -211,6 +213,7 @@ MJ_DEV float intersect_geom(const SceneView& sv, const float* g, V3 o, V3 d, flo
   }
 }
 
+template <bool kMesh = true>
 MJ_DEV Hit trace_closest(const SceneView& sv, V3 o, V3 d, float max_dist, bool cull) {
   Hit h;
   h.geom_id = -1;
@@ -232,7 +235,7 @@ MJ_DEV Hit trace_closest(const SceneView& sv, V3 o, V3 d, float max_dist, bool c
         V3 n = v3s(0.0f);
         float bu = 0.0f, bv = 0.0f;
         int face = -1, mesh = -1;
-        float dd = intersect_geom<false>(sv, g, o, d, cut_of(h.dist), &n, &bu, &bv, &face, &mesh);
+        float dd = intersect_geom<false, kMesh>(sv, g, o, d, cut_of(h.dist), &n, &bu, &bv, &face, &mesh);
         if (cull && dd >= 0.0f && dot(d, n) > 0.0f) dd = -1.0f;
         float tol = kRayTolRel * fmaxf(h.dist, 1.0f);
         bool closer = (dd < h.dist - tol) || (fabsf(dd - h.dist) <= tol && (h.geom_id < 0 || gi > h.geom_id));
@@ -250,6 +253,7 @@ MJ_DEV Hit trace_closest(const SceneView& sv, V3 o, V3 d, float max_dist, bool c
   return h;
 }
 
+template <bool kMesh = true>
 MJ_DEV bool trace_any(const SceneView& sv, V3 o, V3 d, float max_dist, bool cull) {
   V3 rcp = safe_rcp(d);
   bool hit = false;
@@ -265,7 +269,7 @@ MJ_DEV bool trace_any(const SceneView& sv, V3 o, V3 d, float max_dist, bool cull
         V3 n = v3s(0.0f);
         float bu, bv;
         int face, mesh;
-        float dd = intersect_geom<true>(sv, g, o, d, max_dist, &n, &bu, &bv, &face, &mesh);
+        float dd = intersect_geom<true, kMesh>(sv, g, o, d, max_dist, &n, &bu, &bv, &face, &mesh);
         if (cull && dd >= 0.0f && dot(d, n) > 0.0f) dd = -1.0f;
         if (dd >= 0.0f && dd < max_dist) {
           hit = true;

@@ -289,6 +289,7 @@ def create_render_context(
     headlight_diffuse=tuple(float(x) for x in mjm.headlight_diffuse),
     headlight_specular=tuple(float(x) for x in mjm.headlight_specular),
     has_spot_lights=bool(mjm.nlight and np.any(mjm.light_type == mdl.LIGHT_SPOT)),
+    has_meshes=bool(np.any(np.isin(mjm.geom_type[rendered], (mdl.GEOM_MESH, mdl.GEOM_SDF, mdl.GEOM_HFIELD)))),
   )
 
   physical = light_model == LightModel.PHYSICAL

@@ -225,6 +225,7 @@ def settings_struct(rc: PbrRenderContext):
   s.path_samples = int(rc.path_samples)
   s.max_depth = int(rc.max_depth)
   s.rr_depth = int(rc.rr_depth)
+  s.has_meshes = int(bool(base.has_meshes))
   return s
 
 

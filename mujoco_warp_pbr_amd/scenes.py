@@ -391,6 +391,86 @@ def mesh_scene(seed: int = 0, res: int = 256, nmesh: int = 16, instances: int = 
   return b.build(stat_center=np.array([0, 0, 0.3], np.float32), stat_extent=2.0)
 
 
+def xyaxes_frame(xyaxes) -> np.ndarray:
+  """Camera frame (3, 3) from MJCF `xyaxes` (x axis, then y axis, in world coordinates): y is
+  made orthogonal to x, z = x × y; columns are the axes."""
+  v = np.asarray(xyaxes, np.float64)
+  x = v[:3] / np.linalg.norm(v[:3])
+  y = v[3:] - x * np.dot(x, v[3:])
+  y /= np.linalg.norm(y)
+  return np.stack([x, y, np.cross(x, y)], axis=1)
+
+
+def checker_texture(size: int, rgb1, rgb2) -> np.ndarray:
+  """MuJoCo's builtin 2 x 2 checker (rgb1 on the diagonal quadrants), RGB8."""
+  half = size // 2
+  y, x = np.mgrid[0:size, 0:size]
+  first = (x < half) == (y < half)
+  img = np.where(first[..., None], np.asarray(rgb1, np.float64), np.asarray(rgb2, np.float64))
+  return np.round(np.clip(img, 0.0, 1.0) * 255).astype(np.uint8)
+
+
+def suite_material_grid(res: int = 128, n: int = 7) -> mdl.HostModel:
+  """The reference benchmark suite's `material_grid` (benchmarks/scenes.py:46-70): n x n spheres
+  of radius 0.09 on a floor, metallic rising bottom to top and roughness left to right, a
+  shadow-casting sun, the headlight at ambient 0.2 / diffuse 0.3 / specular 0.3."""
+  b = _Builder()
+  floor = b.material(rgba=(0.5, 0.5, 0.5, 1), roughness=0.9)
+  b.light(mdl.LIGHT_DIRECTIONAL, (0, 0, 0), (-0.4, 0.6, -0.7), diffuse=(0.7, 0.7, 0.7))
+  b.geom(mdl.GEOM_PLANE, (3.0, 3.0, 0.1), (0, 0, 0), matid=floor)
+  spacing = 0.22
+  for i in range(n):
+    for j in range(n):
+      mat = b.material(rgba=(0.95, 0.64, 0.54, 1), metallic=i / (n - 1), roughness=j / (n - 1))
+      b.geom(mdl.GEOM_SPHERE, (0.09, 0, 0), ((j - (n - 1) / 2) * spacing, 0.0, 0.12 + i * spacing), matid=mat)
+  height = 0.12 + (n - 1) * spacing
+  pos = (0.0, -0.55 - 1.9 * height, 0.5 * height + 0.06)
+  b.add(
+    cam_xpos=np.asarray(pos, np.float32),
+    cam_xmat=xyaxes_frame((1, 0, 0, 0, 0, 1)).astype(np.float32).reshape(9),
+    cam_fovy=45.0,
+    cam_projection=0,
+    cam_sensorsize=np.zeros(2, np.float32),
+    cam_intrinsic=np.zeros(4, np.float32),
+    cam_resolution=np.asarray((res, res), np.int32),
+  )
+  return b.build(
+    stat_center=np.array([0, 0, 0.5 * height], np.float32),
+    stat_extent=2.0,
+    headlight_active=True,
+    headlight_ambient=np.full(3, 0.2, np.float32),
+    headlight_diffuse=np.full(3, 0.3, np.float32),
+    headlight_specular=np.full(3, 0.3, np.float32),
+  )
+
+
+def suite_primitives(res: int = 128) -> mdl.HostModel:
+  """The reference benchmark suite's `primitives` (benchmarks/scenes.py:73-100, MuJoCo Warp's
+  render benchmark layout): an infinite checker-textured plane with reflectance 0.2, a table
+  box, 25 spheres and 25 capsules, a directional light straight down and MuJoCo's default
+  headlight."""
+  b = _Builder()
+  tex = b.texture(checker_texture(512, (0.1, 0.2, 0.3), (0.2, 0.3, 0.4)), mdl.COLORSPACE_AUTO)
+  grid = b.material(rgba=(1, 1, 1, 1), reflectance=0.2, texid={mdl.ROLE_RGBA: tex}, repeat=(1, 1), uniform=1)
+  b.geom(mdl.GEOM_PLANE, (0.0, 0.0, 0.05), (0, 0, 0), matid=grid)
+  b.light(mdl.LIGHT_DIRECTIONAL, (0, 0, 3), (0, 0, -1), diffuse=(0.7, 0.7, 0.7))
+  b.geom(mdl.GEOM_BOX, (0.5, 1.0, 0.01), (0.8, 0.0, 0.5))
+  for kind, size, z, rgba in ((mdl.GEOM_SPHERE, (0.05, 0, 0), 0.8, (1, 0, 0, 1)), (mdl.GEOM_CAPSULE, (0.04, 0.05, 0), 0.95, (0, 1, 0, 1))):
+    for i in range(5):
+      for j in range(5):
+        b.geom(kind, size, (0.6 + 0.1 * i, -0.4 + 0.2 * j, z), rgba=rgba)
+  b.add(
+    cam_xpos=np.asarray((3.0, 0.0, 2.0), np.float32),
+    cam_xmat=xyaxes_frame((0, 1, 0, -0.6, 0, 1)).astype(np.float32).reshape(9),
+    cam_fovy=45.0,
+    cam_projection=0,
+    cam_sensorsize=np.zeros(2, np.float32),
+    cam_intrinsic=np.zeros(4, np.float32),
+    cam_resolution=np.asarray((res, res), np.int32),
+  )
+  return b.build(stat_center=np.array([0.8, 0, 0.5], np.float32), stat_extent=2.0, headlight_active=True)
+
+
 def randomize_worlds(m, rc, nworld: int, seed: int = 1) -> None:
   """Per-world appearance: roughness U[0.05, 1], metallic Bernoulli(0.3), rgba U[0, 1] (alpha 1)."""
   import torch

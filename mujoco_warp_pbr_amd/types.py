@@ -111,6 +111,8 @@ class RenderBase:
   headlight_diffuse: tuple = (0.4, 0.4, 0.4)
   headlight_specular: tuple = (0.5, 0.5, 0.5)
   has_spot_lights: bool = False
+  # an enabled geom is a mesh, SDF or height field (reference io.py:317 `has_meshes`)
+  has_meshes: bool = True
 
 
 @dataclasses.dataclass

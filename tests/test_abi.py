@@ -38,3 +38,40 @@ def test_errors_are_reported_without_a_gpu():
   rc = lib.mjpbr_scene_bvh(ctypes.addressof(geo), 0, None)
   assert rc == -1
   assert b"scene BVH holds" in lib.mjpbr_last_error()
+
+
+def test_realtime_kernel_instance_covers_the_settings():
+  """mjpbr_rt_kernel_features names the compiled real-time instance a launch uses: it covers
+  every feature the settings need (mesh 1, supersampling 2, soft shadows 4, mirror 8, layers 16,
+  textures 32) and is the smallest such instance for the common cases."""
+  lib = _abi.library()
+  S = _abi.structs()
+
+  def mask(**kw):
+    s = S["MjpbrSettings"]()
+    s.samples_per_axis = 1
+    s.shadow_samples = 1
+    s.use_shadows = 1
+    for k, v in kw.items():
+      setattr(s, k, v)
+    return lib.mjpbr_rt_kernel_features(ctypes.addressof(s))
+
+  assert mask() == 0
+  assert mask(textures=1) == 32
+  assert mask(textures=1, has_reflection=1) == 40
+  assert mask(has_meshes=1) == 1
+  assert mask(has_meshes=1, textures=1) == 33
+  full = dict(has_meshes=1, samples_per_axis=2, shadow_samples=4, has_reflection=1, textures=1)
+  assert mask(**full) == 47
+  assert mask(**full, has_transparency=1) == 63
+  # soft shadows only count with shadows on; any needed bit is covered
+  assert mask(shadow_samples=4, use_shadows=0) == 0
+  for bits in range(64):
+    kw = {}
+    if bits & 1: kw["has_meshes"] = 1
+    if bits & 2: kw["samples_per_axis"] = 3
+    if bits & 4: kw["shadow_samples"] = 2
+    if bits & 8: kw["has_reflection"] = 1
+    if bits & 16: kw["has_transparency"] = 1
+    if bits & 32: kw["textures"] = 1
+    assert mask(**kw) & bits == bits

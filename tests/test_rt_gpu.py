@@ -42,6 +42,22 @@ def test_primitives_scene_matches_oracle(cuda):
   assert np.mean(gpu["seg"][..., 0] >= 0) > 0.3
 
 
+@pytest.mark.parametrize("scene", ["material_grid", "primitives"])
+def test_reference_suite_scenes_match_oracle(cuda, scene):
+  """The reference suite's material_grid (no textures, no mirror: the leanest kernel instance)
+  and primitives (checker texture, reflective plane) scenes with an HDR sky and the headlight."""
+  fn = scenes.suite_material_grid if scene == "material_grid" else scenes.suite_primitives
+  mjm = fn(48)
+  gpu, ref, (m, d, rc) = _render_both(mjm, 3, (48, 48), envmap=scenes.procedural_sky(3, 64, 128))
+  from mujoco_warp_pbr_amd import _abi
+  from mujoco_warp_pbr_amd.render import settings_struct
+
+  expected = 0 if scene == "material_grid" else 40
+  assert _abi.library().mjpbr_rt_kernel_features(_abi.byref(settings_struct(rc))) == expected
+  compare_rt(gpu, ref)
+  assert np.mean(gpu["seg"][..., 0] >= 0) > 0.3
+
+
 def test_full_scene_matches_oracle(cuda):
   """Full-feature real-time case at reduced size: meshes, textures, HDR sky, mirror plane,
   2x2 supersampling, soft shadows, per-world materials and cameras."""
