@@ -80,6 +80,11 @@ typedef struct MjpbrGeometry {
   const float* blas_box;       /* [sum inner nodes, 12] */
   const float* blas_tri;       /* [slots, 12] */
   const float* mesh_aabb;      /* [nmesh + nhfield, 6] mesh-frame bounds {lo.xyz, hi.xyz} */
+  /* Optional [nworld, mjpbr_world_stage_floats] buffer: mjpbr_rt_render / mjpbr_pt_render first
+   * resolve every world's scene BVH, geom, light and surface records into it (one small launch),
+   * then each workgroup copies its world's block into LDS. NULL: each workgroup resolves its
+   * world itself (same results). The wavefront tracer ignores it. */
+  float* world_stage;
 } MjpbrGeometry;
 
 /* Materials, mesh attributes and textures (reference: material.py:224-249 `Materials`). */
@@ -299,9 +304,15 @@ typedef struct MjpbrWavefrontPlan {
 /* ---------------------------------------------------------------------------------------- */
 /* Library */
 int mjpbr_abi_version(void);
+/* First 16 hex digits of the SHA-256 of this header as the library was built against it (the
+ * bindings compare it with the header they read and refuse a stale library). */
+const char* mjpbr_header_hash(void);
 const char* mjpbr_last_error(void);
 /* sizeof of every ABI struct, in declaration order, so bindings can check their mirrors. */
 int mjpbr_struct_sizes(int64_t* sizes, int n);
+
+/* Floats a world takes in MjpbrGeometry.world_stage (a multiple of 4), or a negative code. */
+int64_t mjpbr_world_stage_floats(const MjpbrScene* scene);
 
 /* Per-world scene BVH over the enabled geoms: build (leaf assignment by median split + boxes)
  * or refit (boxes only, leaf assignment kept). Replaces `mujoco_warp.refit_bvh` and the BVH of

@@ -7,6 +7,7 @@ The ctypes.Structure mirrors are generated from the header's typedefs, so the tw
 from __future__ import annotations
 
 import ctypes
+import hashlib
 import os
 import re
 from functools import lru_cache
@@ -35,11 +36,13 @@ STRUCT_ORDER = (
 _VP = ctypes.c_void_p
 FUNCTIONS = {
   "mjpbr_abi_version": (ctypes.c_int, []),
+  "mjpbr_header_hash": (ctypes.c_char_p, []),
   "mjpbr_last_error": (ctypes.c_char_p, []),
   "mjpbr_struct_sizes": (ctypes.c_int, [_VP, ctypes.c_int]),
   "mjpbr_scene_bvh": (ctypes.c_int, [_VP, ctypes.c_int, _VP]),
   "mjpbr_rt_render": (ctypes.c_int, [_VP, _VP, _VP, ctypes.c_int32, _VP]),
   "mjpbr_rt_kernel_features": (ctypes.c_int, [_VP]),
+  "mjpbr_world_stage_floats": (ctypes.c_int64, [_VP]),
   "mjpbr_pt_render": (ctypes.c_int, [_VP, _VP, _VP, ctypes.c_int32, _VP]),
   "mjpbr_atrous": (ctypes.c_int, [_VP, _VP, _VP, _VP, _VP]),
   "mjpbr_dfg_lut": (ctypes.c_int, [_VP, ctypes.c_int, ctypes.c_int, _VP]),
@@ -163,8 +166,24 @@ def library() -> ctypes.CDLL:
         continue
       fn.restype = res
       fn.argtypes = args
+    _check_build(lib, path)
     _LIB = lib
   return _LIB
+
+
+def _check_build(lib: ctypes.CDLL, path: Path) -> None:
+  """Refuses a library built against another version of include/mjpbr.h: its structs would be
+  read at other offsets than the ctypes mirrors write (host-side checks, no device call)."""
+  want = hashlib.sha256(HEADER.read_bytes()).hexdigest()[:16]
+  got = lib.mjpbr_header_hash().decode() if hasattr(lib, "mjpbr_header_hash") else "none"
+  if got != want:
+    raise MjpbrError(f"{path} was built against another include/mjpbr.h (header hash {got}, expected {want}): rebuild it")
+  sizes = (ctypes.c_int64 * len(STRUCT_ORDER))()
+  lib.mjpbr_struct_sizes(ctypes.cast(sizes, ctypes.c_void_p), len(STRUCT_ORDER))
+  st = structs()
+  for i, name in enumerate(STRUCT_ORDER):
+    if ctypes.sizeof(st[name]) != int(sizes[i]):
+      raise MjpbrError(f"{path}: {name} is {int(sizes[i])} bytes in the library, {ctypes.sizeof(st[name])} in the bindings: rebuild it")
 
 
 def call(name: str, *args) -> None:

@@ -15,7 +15,13 @@ int set_error(int code, const char* fmt, ...) {
 
 }  // namespace mjpbr
 
+#ifndef MJPBR_HEADER_HASH
+#define MJPBR_HEADER_HASH "unknown"
+#endif
+
 extern "C" int mjpbr_abi_version(void) { return MJPBR_ABI_VERSION; }
+
+extern "C" const char* mjpbr_header_hash(void) { return MJPBR_HEADER_HASH; }
 
 extern "C" const char* mjpbr_last_error(void) { return mjpbr::g_error; }
 

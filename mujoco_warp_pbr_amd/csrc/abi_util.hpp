@@ -5,7 +5,7 @@
 #include <stdarg.h>
 #include <stdio.h>
 #include "../../include/mjpbr.h"
-#include "camera.hpp"
+#include "stage.hpp"
 
 namespace mjpbr {
 
@@ -17,11 +17,11 @@ inline int check_launch(const char* what) {
   return MJPBR_OK;
 }
 
-inline size_t host_stage_bytes(const MjpbrScene* sc) {
-  int nleaf = 1 << sc->geo.tlas_depth;
-  return (size_t)((nleaf - 1) * 12 + sc->geo.ngeom * kGeomRec + staged_leaf_slots(nleaf) + sc->lit.nlight * kLightRec +
-                  sc->geo.ngeom * kSurfRec) * 4;
-}
+inline size_t host_stage_bytes(const MjpbrScene* sc) { return (size_t)stage_floats(*sc) * 4; }
+
+// Enqueues the per-world record resolution into sc->geo.world_stage (no-op when it is NULL);
+// the render kernels launched after it on the stream copy the records into LDS.
+int launch_stage_prep(const MjpbrScene* sc, hipStream_t stream);
 
 inline int make_tile_map(const MjpbrScene* sc, TileMap* tm, int* nlaunch) {
   const MjpbrCameras& c = sc->cams;

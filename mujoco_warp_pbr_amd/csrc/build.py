@@ -10,6 +10,7 @@ from __future__ import annotations
 
 import argparse
 import concurrent.futures
+import hashlib
 import os
 import shutil
 import subprocess
@@ -40,13 +41,19 @@ CFLAGS = [
 ]
 
 
+def header_hash() -> str:
+  """First 16 hex digits of the SHA-256 of include/mjpbr.h: compiled into the library so the
+  bindings refuse a library built against another version of the ABI structs."""
+  return hashlib.sha256((ROOT / "include" / "mjpbr.h").read_bytes()).hexdigest()[:16]
+
+
 def sources() -> list[Path]:
   return sorted(HERE.glob("*.hip"))
 
 
 def _compile(src: Path) -> Path:
   obj = BUILD / (src.stem + ".o")
-  cmd = [HIPCC, *CFLAGS, "-c", str(src), "-o", str(obj)]
+  cmd = [HIPCC, *CFLAGS, f'-DMJPBR_HEADER_HASH="{header_hash()}"', "-c", str(src), "-o", str(obj)]
   proc = subprocess.run(cmd, capture_output=True, text=True)
   if proc.returncode != 0:
     raise RuntimeError(f"hipcc failed on {src.name}:\n{proc.stderr}")

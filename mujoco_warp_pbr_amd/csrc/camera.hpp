@@ -24,6 +24,7 @@ struct Pixel {
   bool want_rgb, want_depth, want_seg;
   int projection;
   float fovy;
+  double tan_half_fovy;  // tan(fovy / 2) in double (pinhole cameras without intrinsics)
   float sensor[2];
   float intr[4];
   float xmat[9];
@@ -73,7 +74,7 @@ MJ_DEV void camera_frame_ray(const Pixel& p, int subdiv, int sx, int sy, V3* dir
     float cy = p.intr[3] / p.sensor[1] * (float)H;
     d = V3{(x + 0.5f - 0.5f * (float)W - cx) / fx, -(y + 0.5f - 0.5f * (float)H - cy) / fy, -1.0f};
   } else {
-    double h = tan((double)p.fovy * (3.14159265358979323846 / 360.0));
+    double h = p.tan_half_fovy;
     float ax = (float)(2.0 * h * (double)W / (double)H);
     float ay = (float)(2.0 * h);
     d = V3{u * ax, -v * ay, -1.0f};
@@ -217,6 +218,9 @@ MJ_DEV bool setup_pixel(const MjpbrCameras& cams, const MjpbrOutputs& out, const
   p->fovy = cams.cam_fovy[(int64_t)(w % cams.cam_fovy_n) * cams.ncam + cam];
   p->sensor[0] = cams.cam_sensorsize[2 * cam];
   p->sensor[1] = cams.cam_sensorsize[2 * cam + 1];
+  p->tan_half_fovy = p->projection != 1 && !(p->sensor[0] != 0.0f && p->sensor[1] != 0.0f)
+                         ? tan((double)p->fovy * (3.14159265358979323846 / 360.0))
+                         : 0.0;
   const float* in = cams.cam_intrinsic + ((int64_t)(w % cams.cam_intrinsic_n) * cams.ncam + cam) * 4;
 #pragma unroll
   for (int k = 0; k < 4; ++k) p->intr[k] = in[k];

@@ -7,11 +7,21 @@
 namespace mjpbr {
 
 
-MJ_DEV SceneView stage_world(const MjpbrScene& sc, int w, float* lds) {
+// Floats of one world's staged records: scene BVH boxes, geom records, leaf ids (padded to a
+// multiple of 4), light records, surface records. A multiple of 4, so records copy as float4.
+MJ_HOST_DEV inline int64_t stage_floats(const MjpbrScene& sc) {
+  int nleaf = 1 << sc.geo.tlas_depth;
+  return (int64_t)(nleaf - 1) * 12 + (int64_t)sc.geo.ngeom * kGeomRec + staged_leaf_slots(nleaf) +
+         (int64_t)sc.lit.nlight * kLightRec + (int64_t)sc.geo.ngeom * kSurfRec;
+}
+
+// Resolves world w's records into dst (LDS or global memory) with the block's threads: the
+// per-world material, light and geom arrays (indexed w % n) are read here and nowhere else.
+MJ_DEV void stage_world_to(const MjpbrScene& sc, int w, float* dst) {
   const MjpbrGeometry& g = sc.geo;
   int nleaf = 1 << g.tlas_depth;
   int ninner = nleaf - 1;
-  float* box = lds;
+  float* box = dst;
   float* geoms = box + ninner * 12;
   int* leaf = reinterpret_cast<int*>(geoms + g.ngeom * kGeomRec);
   float* lights = reinterpret_cast<float*>(leaf + staged_leaf_slots(nleaf));
@@ -19,10 +29,10 @@ MJ_DEV SceneView stage_world(const MjpbrScene& sc, int w, float* lds) {
   float* surfs = lights + sc.lit.nlight * kLightRec;
   for (int gi = threadIdx.x; gi < g.ngeom; gi += blockDim.x) stage_surface(sc, w, gi, surfs + gi * kSurfRec);
   const float4* src = reinterpret_cast<const float4*>(g.tlas_box + (int64_t)w * ninner * 12);
-  float4* dst = reinterpret_cast<float4*>(box);
-  for (int i = threadIdx.x; i < ninner * 3; i += blockDim.x) dst[i] = src[i];
+  float4* bdst = reinterpret_cast<float4*>(box);
+  for (int i = threadIdx.x; i < ninner * 3; i += blockDim.x) bdst[i] = src[i];
   const int* lsrc = g.tlas_leaf + (int64_t)w * nleaf;
-  for (int i = threadIdx.x; i < nleaf; i += blockDim.x) leaf[i] = lsrc[i];
+  for (int i = threadIdx.x; i < staged_leaf_slots(nleaf); i += blockDim.x) leaf[i] = i < nleaf ? lsrc[i] : -1;
   for (int gi = threadIdx.x; gi < g.ngeom; gi += blockDim.x) {
     float* r = geoms + gi * kGeomRec;
     const float* xp = g.geom_xpos + ((int64_t)w * g.ngeom + gi) * 3;
@@ -45,20 +55,39 @@ MJ_DEV SceneView stage_world(const MjpbrScene& sc, int w, float* lds) {
     int bits = traced | (rgba.w == 0.0f ? 0x100 : 0) | ((dataid + 1) << 9);
     r[15] = __int_as_float(bits);
   }
-  __syncthreads();
+}
+
+MJ_DEV SceneView scene_view(const MjpbrScene& sc, float* lds) {
+  const MjpbrGeometry& g = sc.geo;
+  int nleaf = 1 << g.tlas_depth;
   SceneView sv;
-  sv.tlas_box = box;
-  sv.tlas_leaf = leaf;
-  sv.geoms = geoms;
+  sv.tlas_box = lds;
+  sv.geoms = lds + (nleaf - 1) * 12;
+  sv.tlas_leaf = reinterpret_cast<const int*>(sv.geoms + g.ngeom * kGeomRec);
+  sv.lights = reinterpret_cast<const float*>(sv.tlas_leaf + staged_leaf_slots(nleaf));
+  sv.surfs = sv.lights + sc.lit.nlight * kLightRec;
   sv.depth = g.tlas_depth;
-  sv.lights = lights;
-  sv.surfs = surfs;
   sv.blas_box = g.blas_box;
   sv.blas_tri = g.blas_tri;
   sv.blas_depth = g.blas_depth;
   sv.blas_node_adr = g.blas_node_adr;
   sv.blas_tri_adr = g.blas_tri_adr;
   return sv;
+}
+
+// World w's records into LDS: a float4 copy of the prepared records when geo.world_stage is set
+// (stage_prep_kernel ran first on the stream), else resolved here.
+MJ_DEV SceneView stage_world(const MjpbrScene& sc, int w, float* lds) {
+  if (sc.geo.world_stage) {
+    const int64_t n4 = stage_floats(sc) / 4;
+    const float4* src = reinterpret_cast<const float4*>(sc.geo.world_stage) + (int64_t)w * n4;
+    float4* dst = reinterpret_cast<float4*>(lds);
+    for (int64_t i = threadIdx.x; i < n4; i += blockDim.x) dst[i] = src[i];
+  } else {
+    stage_world_to(sc, w, lds);
+  }
+  __syncthreads();
+  return scene_view(sc, lds);
 }
 
 }  // namespace mjpbr

@@ -233,6 +233,7 @@ extern "C" int mjpbr_pt_wavefront_prepare(const MjpbrScene* scenes, const MjpbrO
   for (int g = 0; g < ngroup; ++g) {
     WfGroup& G = h->groups[g];
     G.sc = scenes[g];
+    G.sc.geo.world_stage = nullptr;  // items stage their world directly (no per-render prep)
     G.out = outs[g];
     int nlaunch;
     int rc = make_tile_map(&scenes[g], &G.tm, &nlaunch);

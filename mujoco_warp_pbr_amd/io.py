@@ -292,6 +292,8 @@ def create_render_context(
     has_meshes=bool(np.any(np.isin(mjm.geom_type[rendered], (mdl.GEOM_MESH, mdl.GEOM_SDF, mdl.GEOM_HFIELD)))),
   )
 
+  base.world_stage = torch.zeros((nworld, world_stage_floats(depth, mjm.ngeom, mjm.nlight)), dtype=torch.float32, device=dev)
+
   physical = light_model == LightModel.PHYSICAL
   lighting = material.resolve_lighting(mjm, light_model, ev100)
   metallic, roughness, reflectance = material.resolve_geom_materials(mjm, unset_material, default_material)
@@ -406,6 +408,19 @@ def create_render_context(
 
 def render_struct_max_cams() -> int:
   return 16
+
+
+def world_stage_floats(tlas_depth: int, ngeom: int, nlight: int) -> int:
+  """Floats one world takes in the per-world record buffer (the library's layout)."""
+  S = _abi.structs()
+  sc = S["MjpbrScene"]()
+  sc.geo.tlas_depth = tlas_depth
+  sc.geo.ngeom = ngeom
+  sc.lit.nlight = nlight
+  n = _abi.library().mjpbr_world_stage_floats(_abi.byref(sc))
+  if n < 0:
+    raise _abi.MjpbrError(_abi.library().mjpbr_last_error().decode())
+  return int(n)
 
 
 def build_scene_bvh(m, d, rc) -> None:

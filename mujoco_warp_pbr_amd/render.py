@@ -71,6 +71,7 @@ def geometry_struct(m, d, rc: PbrRenderContext):
   g.blas_box = _abi.ptr(base.blas_box)
   g.blas_tri = _abi.ptr(base.blas_tri)
   g.mesh_aabb = _abi.ptr(base.mesh_aabb)
+  g.world_stage = _abi.ptr(base.world_stage)
   return g
 
 

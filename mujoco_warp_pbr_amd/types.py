@@ -113,6 +113,9 @@ class RenderBase:
   has_spot_lights: bool = False
   # an enabled geom is a mesh, SDF or height field (reference io.py:317 `has_meshes`)
   has_meshes: bool = True
+  # [nworld, floats] per-world records the render kernels resolve once a render and copy into
+  # LDS (MjpbrGeometry.world_stage); None: each workgroup resolves its world itself
+  world_stage: Any = None
 
 
 @dataclasses.dataclass

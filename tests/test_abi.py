@@ -75,3 +75,16 @@ def test_realtime_kernel_instance_covers_the_settings():
     if bits & 16: kw["has_transparency"] = 1
     if bits & 32: kw["textures"] = 1
     assert mask(**kw) & bits == bits
+
+
+def test_library_built_against_another_header_is_refused(tmp_path, monkeypatch):
+  """A library whose compiled-in header hash differs from the header the bindings read is
+  refused before any call: its structs would be read at other offsets (host-side check)."""
+  other = tmp_path / "mjpbr.h"
+  other.write_text(_abi.HEADER.read_text() + "\n/* another version */\n")
+  monkeypatch.setattr(_abi, "HEADER", other)
+  monkeypatch.setattr(_abi, "_LIB", None)
+  import pytest
+
+  with pytest.raises(_abi.MjpbrError, match="another include/mjpbr.h"):
+    _abi.library()
