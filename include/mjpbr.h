@@ -70,7 +70,13 @@ typedef struct MjpbrGeometry {
   int32_t nbvh;                /* number of geoms in the BVH (enabled geom groups) */
   float* tlas_box;             /* [nworld, 2^depth - 1, 12] */
   int32_t* tlas_leaf;          /* [nworld, 2^depth] */
-  const int32_t* bvh_geom;     /* [nbvh] enabled geom ids */
+  const int32_t* bvh_geom;     /* [nbvh] enabled geom ids in the BVH */
+  /* Enabled geoms every ray query tests directly, before the BVH: planes of unbounded extent,
+   * whose boxes would widen every ancestor node. Ascending ids, disjoint from bvh_geom; the
+   * enabled set is bvh_geom and direct_geom together. */
+  const int32_t* direct_geom;  /* [ndirect] */
+  int32_t ndirect;
+  int32_t pad1;
   /* Mesh BVHs (shared by all worlds), same implicit layout, MJPBR_BLAS_LEAF triangles a leaf.
    * blas_tri[slot] = {v0.xyz, face id (int bits)}, {e1 = v1 - v0}, {e2 = v2 - v0} in the mesh
    * frame. */

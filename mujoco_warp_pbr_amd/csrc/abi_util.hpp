@@ -29,6 +29,8 @@ inline int make_tile_map(const MjpbrScene* sc, TileMap* tm, int* nlaunch) {
     return set_error(MJPBR_EINVAL, "nrender must be in [1, %d], got %d", MJPBR_MAX_CAMS, c.nrender);
   if (sc->geo.tlas_depth < 1 || sc->geo.tlas_depth > 12)
     return set_error(MJPBR_EINVAL, "scene BVH depth must be in [1, 12], got %d", sc->geo.tlas_depth);
+  if (sc->geo.ndirect < 0 || sc->geo.ndirect > sc->geo.ngeom || (sc->geo.ndirect > 0 && !sc->geo.direct_geom))
+    return set_error(MJPBR_EINVAL, "ndirect must be in [0, ngeom] with direct_geom set, got %d", sc->geo.ndirect);
   tm->nrender = c.nrender;
   int base = 0, rays = 0;
   for (int i = 0; i < c.nrender; ++i) {

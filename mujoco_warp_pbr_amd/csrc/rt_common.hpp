@@ -34,7 +34,10 @@ MJ_DEV V2 shadow_rand2(int seed, int stream, int nlight, int li, int nsample, in
   return V2{rand_at(seed, counter), rand_at(seed, counter + 1)};
 }
 
-// render.py:95-127
+// render.py:95-127. The reference's three cases (directional: one unbounded ray; one sample: one
+// ray to the light; soft: `shadow_samples` rays to the bulb disc) go through one any-hit site, so
+// the kernel holds one inlined traversal for shadows. With one ray, v is 0 or 1 and
+// v + (1 - v) * fraction equals the reference's `hit ? fraction : 1` exactly.
 template <unsigned F>
 MJ_DEV float light_visibility(const RtCtx& c, const Light& L, V3 point, V3 normal, V3 l, float dist, int stream,
                               int nlight, int li) {
@@ -42,23 +45,28 @@ MJ_DEV float light_visibility(const RtCtx& c, const Light& L, V3 point, V3 norma
   const MjpbrSettings& s = c.s;
   V3 origin = point + normal * kRayOffset;
   bool cull = s.cull_backfaces != 0;
-  if (L.type == MJPBR_LIGHT_DIRECTIONAL)
-    return trace_any<kMesh>(c.sv, origin, l, kUnboundedShadow, cull) ? s.shadow_light_fraction : 1.0f;
-  if (!(F & kFSoft) || s.shadow_samples <= 1)
-    return trace_any<kMesh>(c.sv, origin, l, dist - kShadowGap, cull) ? s.shadow_light_fraction : 1.0f;
-  V3 t1, t2;
-  orthonormal_basis(l, &t1, &t2);
+  const bool directional = L.type == MJPBR_LIGHT_DIRECTIONAL;
+  const bool soft = (F & kFSoft) && !directional && s.shadow_samples > 1;
+  const int n = soft ? s.shadow_samples : 1;
+  V3 t1 = v3s(0.0f), t2 = v3s(0.0f);
+  if (soft) orthonormal_basis(l, &t1, &t2);
   float visible = 0.0f;
-  for (int k = 0; k < s.shadow_samples; ++k) {
-    V2 u = shadow_rand2(c.seed, stream, nlight, li, s.shadow_samples, k);
-    float r = L.bulbradius * sqrtf(u.x);
-    float sp, cp;
-    sincos_turns(u.y, &sp, &cp);
-    V3 to_target = L.pos + (t1 * cp + t2 * sp) * r - origin;
-    float d = len(to_target);
-    if (!trace_any<kMesh>(c.sv, origin, to_target / d, d - kShadowGap, cull)) visible += 1.0f;
+  for (int k = 0; k < n; ++k) {
+    V3 dir = l;
+    float tmax = directional ? kUnboundedShadow : dist - kShadowGap;
+    if (soft) {
+      V2 u = shadow_rand2(c.seed, stream, nlight, li, s.shadow_samples, k);
+      float r = L.bulbradius * sqrtf(u.x);
+      float sp, cp;
+      sincos_turns(u.y, &sp, &cp);
+      V3 to_target = L.pos + (t1 * cp + t2 * sp) * r - origin;
+      float d = len(to_target);
+      dir = to_target / d;
+      tmax = d - kShadowGap;
+    }
+    if (!trace_any<kMesh>(c.sv, origin, dir, tmax, cull)) visible += 1.0f;
   }
-  float v = visible / (float)s.shadow_samples;
+  float v = visible / (float)n;
   return v + (1.0f - v) * s.shadow_light_fraction;
 }
 

@@ -437,59 +437,66 @@ MJ_DEV float4 geom_rgba(const MjpbrGeometry& g, int w, int gi, int* mat_id) {
 
 MJ_DEV V2 wrap_delta(V2 d) { return V2{d.x - rintf(d.x), d.y - rintf(d.y)}; }
 
-// Analytic UVs of primitives (own convention: unrepeated uv, repeat, offset).
-MJ_DEV void primitive_uv(int type, V3 p, V3 ln, V3 size, V2 repeat, bool uniform, V2* uv, V2* rep, V2* off) {
-  *rep = repeat;
-  *off = V2{0.0f, 0.0f};
+// Analytic UVs of primitives (own convention: unrepeated uv), at the three points of the
+// finite-difference UV Jacobian (material.py:412-428) in one switch over the geom type, so a
+// wave whose lanes hit different primitive types branches once, not three times.
+MJ_DEV void primitive_uv3(int type, const V3 (&p)[3], V3 ln, V3 size, bool uniform, V2 (&uv)[3]) {
   switch (type) {
     case MJPBR_GEOM_PLANE:
     case MJPBR_GEOM_HFIELD: {
-      if (uniform || size.x <= 0.0f || size.y <= 0.0f) {
-        *uv = V2{p.x, -p.y};
-      } else {
-        *uv = V2{p.x / (2.0f * size.x) + 0.5f, 0.5f - p.y / (2.0f * size.y)};
-      }
+      const bool plain = uniform || size.x <= 0.0f || size.y <= 0.0f;
+#pragma unroll
+      for (int k = 0; k < 3; ++k)
+        uv[k] = plain ? V2{p[k].x, -p[k].y} : V2{p[k].x / (2.0f * size.x) + 0.5f, 0.5f - p[k].y / (2.0f * size.y)};
       break;
     }
     case MJPBR_GEOM_SPHERE:
     case MJPBR_GEOM_ELLIPSOID: {
-      V3 q = type == MJPBR_GEOM_SPHERE ? p : V3{p.x / size.x, p.y / size.y, p.z / size.z};
-      float r = len(q);
-      float u = atan2f(q.y, q.x) * (0.5f * kInvPi) + 0.5f;
-      float v = acosf(clampf(r > 0.0f ? q.z / r : 1.0f, -1.0f, 1.0f)) * kInvPi;
-      *uv = V2{u, v};
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        V3 q = type == MJPBR_GEOM_SPHERE ? p[k] : V3{p[k].x / size.x, p[k].y / size.y, p[k].z / size.z};
+        float r = len(q);
+        float u = atan2f(q.y, q.x) * (0.5f * kInvPi) + 0.5f;
+        float v = acosf(clampf(r > 0.0f ? q.z / r : 1.0f, -1.0f, 1.0f)) * kInvPi;
+        uv[k] = V2{u, v};
+      }
       break;
     }
     case MJPBR_GEOM_CYLINDER: {
-      if (fabsf(ln.z) > 0.5f) {
-        *uv = V2{p.x / (2.0f * size.x) + 0.5f, 0.5f - p.y / (2.0f * size.x)};
-      } else {
-        *uv = V2{atan2f(p.y, p.x) * (0.5f * kInvPi) + 0.5f, 0.5f - p.z / (2.0f * size.y)};
+      const bool cap = fabsf(ln.z) > 0.5f;
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        if (cap)
+          uv[k] = V2{p[k].x / (2.0f * size.x) + 0.5f, 0.5f - p[k].y / (2.0f * size.x)};
+        else
+          uv[k] = V2{atan2f(p[k].y, p[k].x) * (0.5f * kInvPi) + 0.5f, 0.5f - p[k].z / (2.0f * size.y)};
       }
       break;
     }
     case MJPBR_GEOM_CAPSULE: {
-      *uv = V2{atan2f(p.y, p.x) * (0.5f * kInvPi) + 0.5f, 0.5f - p.z / (2.0f * (size.y + size.x))};
+#pragma unroll
+      for (int k = 0; k < 3; ++k)
+        uv[k] = V2{atan2f(p[k].y, p[k].x) * (0.5f * kInvPi) + 0.5f, 0.5f - p[k].z / (2.0f * (size.y + size.x))};
       break;
     }
     case MJPBR_GEOM_BOX: {
+      // the face is chosen by the local normal, the same for the three points
       float ax = fabsf(ln.x), ay = fabsf(ln.y), az = fabsf(ln.z);
-      float a, b, sa, sb;
-      if (ax >= ay && ax >= az) {
-        a = p.y; b = p.z; sa = size.y; sb = size.z;
-      } else if (ay >= az) {
-        a = p.x; b = p.z; sa = size.x; sb = size.z;
-      } else {
-        a = p.x; b = p.y; sa = size.x; sb = size.y;
+      const int ia = (ax >= ay && ax >= az) ? 1 : 0;
+      const int ib = (ax >= ay && ax >= az) || ay >= az ? 2 : 1;
+      const float sa = ia == 1 ? size.y : size.x;
+      const float sb = ib == 2 ? size.z : size.y;
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        float a = ia == 1 ? p[k].y : p[k].x;
+        float b = ib == 2 ? p[k].z : p[k].y;
+        uv[k] = uniform ? V2{a, -b} : V2{a / (2.0f * sa) + 0.5f, 0.5f - b / (2.0f * sb)};
       }
-      if (uniform)
-        *uv = V2{a, -b};
-      else
-        *uv = V2{a / (2.0f * sa) + 0.5f, 0.5f - b / (2.0f * sb)};
       break;
     }
     default:
-      *uv = V2{0.0f, 0.0f};
+#pragma unroll
+      for (int k = 0; k < 3; ++k) uv[k] = V2{0.0f, 0.0f};
   }
 }
 
@@ -619,10 +626,11 @@ MJ_DEV Surface eval_surface(const MjpbrScene& sc, const SceneView& sv, const Mjp
       V3 t1, t2;
       orthonormal_basis(local_normal, &t1, &t2);
       float eps = 1.0e-3f * fmaxf(fmaxf(size.x, size.y), fmaxf(size.z, 1.0e-3f));
-      V2 uv0, rep, off, uv1, uv2, r1, o1;
-      primitive_uv(gtype, local, local_normal, size, repeat, uniform, &uv0, &rep, &off);
-      primitive_uv(gtype, local + t1 * eps, local_normal, size, repeat, uniform, &uv1, &r1, &o1);
-      primitive_uv(gtype, local + t2 * eps, local_normal, size, repeat, uniform, &uv2, &r1, &o1);
+      const V3 pts[3] = {local, local + t1 * eps, local + t2 * eps};
+      V2 uvs[3];
+      primitive_uv3(gtype, pts, local_normal, size, uniform, uvs);
+      const V2 uv0 = uvs[0], uv1 = uvs[1], uv2 = uvs[2];
+      const V2 rep = repeat, off = V2{0.0f, 0.0f};
       V2 d1 = wrap_delta(V2{uv1.x - uv0.x, uv1.y - uv0.y});
       V2 d2 = wrap_delta(V2{uv2.x - uv0.x, uv2.y - uv0.y});
       d1 = V2{d1.x * rep.x, d1.y * rep.y};

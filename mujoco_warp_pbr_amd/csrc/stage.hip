@@ -24,5 +24,7 @@ extern "C" int64_t mjpbr_world_stage_floats(const MjpbrScene* scene) {
   if (!scene) return mjpbr::set_error(MJPBR_EINVAL, "scene must not be NULL");
   if (scene->geo.tlas_depth < 1 || scene->geo.tlas_depth > 12)
     return mjpbr::set_error(MJPBR_EINVAL, "scene BVH depth must be in [1, 12], got %d", scene->geo.tlas_depth);
+  if (scene->geo.ndirect < 0 || scene->geo.ndirect > scene->geo.ngeom)
+    return mjpbr::set_error(MJPBR_EINVAL, "ndirect must be in [0, ngeom], got %d", scene->geo.ndirect);
   return mjpbr::stage_floats(*scene);
 }

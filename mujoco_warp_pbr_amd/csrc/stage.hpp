@@ -7,12 +7,13 @@
 namespace mjpbr {
 
 
-// Floats of one world's staged records: scene BVH boxes, geom records, leaf ids (padded to a
-// multiple of 4), light records, surface records. A multiple of 4, so records copy as float4.
+// Floats of one world's staged records: scene BVH boxes, geom records, leaf ids and direct geom
+// ids (each padded to a multiple of 4), light records, surface records. A multiple of 4, so
+// records copy as float4.
 MJ_HOST_DEV inline int64_t stage_floats(const MjpbrScene& sc) {
   int nleaf = 1 << sc.geo.tlas_depth;
   return (int64_t)(nleaf - 1) * 12 + (int64_t)sc.geo.ngeom * kGeomRec + staged_leaf_slots(nleaf) +
-         (int64_t)sc.lit.nlight * kLightRec + (int64_t)sc.geo.ngeom * kSurfRec;
+         staged_leaf_slots(sc.geo.ndirect) + (int64_t)sc.lit.nlight * kLightRec + (int64_t)sc.geo.ngeom * kSurfRec;
 }
 
 // Resolves world w's records into dst (LDS or global memory) with the block's threads: the
@@ -24,7 +25,9 @@ MJ_DEV void stage_world_to(const MjpbrScene& sc, int w, float* dst) {
   float* box = dst;
   float* geoms = box + ninner * 12;
   int* leaf = reinterpret_cast<int*>(geoms + g.ngeom * kGeomRec);
-  float* lights = reinterpret_cast<float*>(leaf + staged_leaf_slots(nleaf));
+  int* direct = leaf + staged_leaf_slots(nleaf);
+  for (int i = threadIdx.x; i < staged_leaf_slots(g.ndirect); i += blockDim.x) direct[i] = i < g.ndirect ? g.direct_geom[i] : -1;
+  float* lights = reinterpret_cast<float*>(direct + staged_leaf_slots(g.ndirect));
   for (int li = threadIdx.x; li < sc.lit.nlight; li += blockDim.x) stage_light(sc.lit, w, li, lights + li * kLightRec);
   float* surfs = lights + sc.lit.nlight * kLightRec;
   for (int gi = threadIdx.x; gi < g.ngeom; gi += blockDim.x) stage_surface(sc, w, gi, surfs + gi * kSurfRec);
@@ -64,7 +67,9 @@ MJ_DEV SceneView scene_view(const MjpbrScene& sc, float* lds) {
   sv.tlas_box = lds;
   sv.geoms = lds + (nleaf - 1) * 12;
   sv.tlas_leaf = reinterpret_cast<const int*>(sv.geoms + g.ngeom * kGeomRec);
-  sv.lights = reinterpret_cast<const float*>(sv.tlas_leaf + staged_leaf_slots(nleaf));
+  sv.direct = sv.tlas_leaf + staged_leaf_slots(nleaf);
+  sv.ndirect = g.ndirect;
+  sv.lights = reinterpret_cast<const float*>(sv.direct + staged_leaf_slots(g.ndirect));
   sv.surfs = sv.lights + sc.lit.nlight * kLightRec;
   sv.depth = g.tlas_depth;
   sv.blas_box = g.blas_box;

@@ -14,6 +14,8 @@
 #include "../../include/mjpbr.h"
 #include "geom.hpp"
 
+#include <type_traits>
+
 namespace mjpbr {
 
 // Staged geom record: 16 floats.
@@ -41,6 +43,8 @@ struct Hit {
 struct SceneView {
   const float* tlas_box;  // [ninner][12] (LDS)
   const int* tlas_leaf;   // [nleaf] (LDS)
+  const int* direct;      // [ndirect] geoms tested before the BVH (LDS)
+  int ndirect;
   const float* geoms;     // [ngeom][16] (LDS)
   const float* lights;    // [nlight][kLightRec] (LDS)
   const float* surfs;     // [ngeom][kSurfRec] (LDS)
@@ -166,7 +170,8 @@ MJ_DEV bool mesh_any(const SceneView& sv, int mesh, V3 lo, V3 ld, float max_t) {
 
 // One geom against the ray (traverse.py:131-192). Returns the distance or -1. kMesh = false
 // drops the mesh / SDF / height-field branch (for scenes without such geoms).
-template <bool kAnyHit, bool kMesh = true>
+// kPlaneOnly: the geom is known to be a plane (the direct geoms).
+template <bool kAnyHit, bool kMesh = true, bool kPlaneOnly = false>
 MJ_DEV float intersect_geom(const SceneView& sv, const float* g, V3 o, V3 d, float max_t, V3* n, float* bu,
                             float* bv, int* face, int* mesh) {
   int bits = __float_as_int(g[15]);
@@ -174,6 +179,7 @@ MJ_DEV float intersect_geom(const SceneView& sv, const float* g, V3 o, V3 d, flo
   V3 pos = V3{g[0], g[1], g[2]};
   const float* mat = g + 4;
   V3 size = V3{g[3], g[13], g[14]};
+  if (kPlaneOnly) return type == MJPBR_GEOM_PLANE ? ray_plane(pos, mat, size, o, d, n) : -1.0f;
   switch (type) {
     case MJPBR_GEOM_PLANE:
       return ray_plane(pos, mat, size, o, d, n);
@@ -224,10 +230,8 @@ MJ_DEV Hit trace_closest(const SceneView& sv, V3 o, V3 d, float max_dist, bool c
   h.face_id = -1;
   h.mesh_id = -1;
   V3 rcp = safe_rcp(d);
-  traverse<true>(
-      sv.tlas_box, sv.depth, o, rcp, [&]() { return cut_of(h.dist); },
-      [&](int slot) {
-        int gi = sv.tlas_leaf[slot];
+  // kPlane: a direct geom (a plane), tested without the other geom types' code
+  auto visit = [&](int gi, auto kPlane) {
         if (gi < 0) return false;
         const float* g = sv.geoms + kGeomRec * gi;
         int bits = __float_as_int(g[15]);
@@ -235,7 +239,8 @@ MJ_DEV Hit trace_closest(const SceneView& sv, V3 o, V3 d, float max_dist, bool c
         V3 n = v3s(0.0f);
         float bu = 0.0f, bv = 0.0f;
         int face = -1, mesh = -1;
-        float dd = intersect_geom<false, kMesh>(sv, g, o, d, cut_of(h.dist), &n, &bu, &bv, &face, &mesh);
+        float dd = intersect_geom<false, kMesh, decltype(kPlane)::value>(sv, g, o, d, cut_of(h.dist), &n, &bu, &bv,
+                                                                         &face, &mesh);
         if (cull && dd >= 0.0f && dot(d, n) > 0.0f) dd = -1.0f;
         float tol = kRayTolRel * fmaxf(h.dist, 1.0f);
         bool closer = (dd < h.dist - tol) || (fabsf(dd - h.dist) <= tol && (h.geom_id < 0 || gi > h.geom_id));
@@ -249,7 +254,11 @@ MJ_DEV Hit trace_closest(const SceneView& sv, V3 o, V3 d, float max_dist, bool c
           h.mesh_id = mesh;
         }
         return false;
-      });
+      };
+  // unbounded planes first (their hit, if any, becomes the cut of the BVH traversal)
+  for (int k = 0; k < sv.ndirect; ++k) visit(sv.direct[k], std::true_type{});
+  traverse<true>(sv.tlas_box, sv.depth, o, rcp, [&]() { return cut_of(h.dist); },
+                 [&](int slot) { return visit(sv.tlas_leaf[slot], std::false_type{}); });
   return h;
 }
 
@@ -258,10 +267,7 @@ MJ_DEV bool trace_any(const SceneView& sv, V3 o, V3 d, float max_dist, bool cull
   V3 rcp = safe_rcp(d);
   bool hit = false;
   float cut = cut_of(max_dist);
-  traverse<false>(
-      sv.tlas_box, sv.depth, o, rcp, [&]() { return cut; },
-      [&](int slot) {
-        int gi = sv.tlas_leaf[slot];
+  auto visit = [&](int gi, auto kPlane) {
         if (gi < 0) return false;
         const float* g = sv.geoms + kGeomRec * gi;
         int bits = __float_as_int(g[15]);
@@ -269,14 +275,18 @@ MJ_DEV bool trace_any(const SceneView& sv, V3 o, V3 d, float max_dist, bool cull
         V3 n = v3s(0.0f);
         float bu, bv;
         int face, mesh;
-        float dd = intersect_geom<true, kMesh>(sv, g, o, d, max_dist, &n, &bu, &bv, &face, &mesh);
+        float dd = intersect_geom<true, kMesh, decltype(kPlane)::value>(sv, g, o, d, max_dist, &n, &bu, &bv, &face, &mesh);
         if (cull && dd >= 0.0f && dot(d, n) > 0.0f) dd = -1.0f;
         if (dd >= 0.0f && dd < max_dist) {
           hit = true;
           return true;
         }
         return false;
-      });
+      };
+  for (int k = 0; k < sv.ndirect; ++k)
+    if (visit(sv.direct[k], std::true_type{})) return true;
+  traverse<false>(sv.tlas_box, sv.depth, o, rcp, [&]() { return cut; },
+                  [&](int slot) { return visit(sv.tlas_leaf[slot], std::false_type{}); });
   return hit;
 }
 

@@ -251,7 +251,14 @@ def create_render_context(
 
   dev = torch.device(device)
   enabled = np.nonzero(rendered)[0].astype(np.int32)
-  depth = bvh_lib.scene_depth(len(enabled))
+  # Planes of unbounded extent (a nonpositive half-size) are tested by every ray before the
+  # scene BVH instead of sitting in it, where their box would cover every ancestor node.
+  gsize = np.asarray(mjm.geom_size, np.float32).reshape(-1, 3)
+  unbounded = (mjm.geom_type[enabled] == mdl.GEOM_PLANE) & ((gsize[enabled, 0] <= 0) | (gsize[enabled, 1] <= 0))
+  if np.all(unbounded):
+    unbounded[:] = False
+  bvh_ids, direct_ids = enabled[~unbounded], enabled[unbounded]
+  depth = bvh_lib.scene_depth(len(bvh_ids))
   tlas_box, tlas_leaf = bvh_lib.alloc_scene_bvh(nworld, depth, dev)
   meshes = bvh_lib.build_mesh_bvhs(mjm)
 
@@ -292,7 +299,10 @@ def create_render_context(
     has_meshes=bool(np.any(np.isin(mjm.geom_type[rendered], (mdl.GEOM_MESH, mdl.GEOM_SDF, mdl.GEOM_HFIELD)))),
   )
 
-  base.world_stage = torch.zeros((nworld, world_stage_floats(depth, mjm.ngeom, mjm.nlight)), dtype=torch.float32, device=dev)
+  base.bvh_geom_ids = torch.as_tensor(bvh_ids).to(dev)
+  base.direct_geom_ids = torch.as_tensor(direct_ids).to(dev)
+  base.world_stage = torch.zeros((nworld, world_stage_floats(depth, mjm.ngeom, mjm.nlight, len(direct_ids))),
+                                 dtype=torch.float32, device=dev)
 
   physical = light_model == LightModel.PHYSICAL
   lighting = material.resolve_lighting(mjm, light_model, ev100)
@@ -410,12 +420,13 @@ def render_struct_max_cams() -> int:
   return 16
 
 
-def world_stage_floats(tlas_depth: int, ngeom: int, nlight: int) -> int:
+def world_stage_floats(tlas_depth: int, ngeom: int, nlight: int, ndirect: int = 0) -> int:
   """Floats one world takes in the per-world record buffer (the library's layout)."""
   S = _abi.structs()
   sc = S["MjpbrScene"]()
   sc.geo.tlas_depth = tlas_depth
   sc.geo.ngeom = ngeom
+  sc.geo.ndirect = ndirect
   sc.lit.nlight = nlight
   n = _abi.library().mjpbr_world_stage_floats(_abi.byref(sc))
   if n < 0:

@@ -61,10 +61,14 @@ def geometry_struct(m, d, rc: PbrRenderContext):
   g.mat_rgba_n = _n(m.mat_rgba)
   g.nmat = m.nmat
   g.tlas_depth = base.tlas_depth
-  g.nbvh = int(base.enabled_geom_ids.shape[0])
+  bvh_ids = base.enabled_geom_ids if base.bvh_geom_ids is None else base.bvh_geom_ids
+  g.nbvh = int(bvh_ids.shape[0])
   g.tlas_box = _abi.ptr(base.tlas_box)
   g.tlas_leaf = _abi.ptr(base.tlas_leaf)
-  g.bvh_geom = _abi.ptr(base.enabled_geom_ids)
+  g.bvh_geom = _abi.ptr(bvh_ids)
+  if base.direct_geom_ids is not None and base.direct_geom_ids.numel():
+    g.direct_geom = _abi.ptr(base.direct_geom_ids)
+    g.ndirect = int(base.direct_geom_ids.shape[0])
   g.blas_depth = _abi.ptr(base.blas_depth)
   g.blas_node_adr = _abi.ptr(base.blas_node_adr)
   g.blas_tri_adr = _abi.ptr(base.blas_tri_adr)

@@ -72,7 +72,7 @@ class RenderBase:
     depth_data: planar depth                                          (nworld, ndepth)
     seg_data: (geom id, object type), (-1, -1) for the background     (nworld, nseg, 2)
     znear / zfar: clip distances (zfar <= 0: unbounded).
-    enabled_geom_ids: geoms in the scene BVH                          (nbvh,)
+    enabled_geom_ids: rendered geoms (enabled groups)                (nenabled,)
     tlas_depth: depth of the per-world scene BVH.
     tlas_box / tlas_leaf: per-world scene BVH (see include/mjpbr.h)
     blas_*: mesh BVHs shared by all worlds.
@@ -116,6 +116,10 @@ class RenderBase:
   # [nworld, floats] per-world records the render kernels resolve once a render and copy into
   # LDS (MjpbrGeometry.world_stage); None: each workgroup resolves its world itself
   world_stage: Any = None
+  # enabled geoms split into the scene BVH's (bounded) and the directly tested ones (unbounded
+  # planes, MjpbrGeometry.direct_geom); None: every enabled geom is in the BVH
+  bvh_geom_ids: Any = None
+  direct_geom_ids: Any = None
 
 
 @dataclasses.dataclass

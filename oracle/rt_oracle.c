@@ -267,13 +267,22 @@ static int geom_skipped(const Ctx* c, int gi) {
   return rgba[3] == 0.0f;
 }
 
+/* The k-th enabled geom in index order: the enabled set is bvh_geom and direct_geom together,
+ * both ascending (the renderer tests the direct geoms before its BVH; brute force keeps index
+ * order). */
+static int enabled_geom_o(const MjpbrGeometry* g, int k, int* ib, int* id) {
+  if (*id >= g->ndirect || (*ib < g->nbvh && g->bvh_geom[*ib] < g->direct_geom[*id])) return g->bvh_geom[(*ib)++];
+  return g->direct_geom[(*id)++];
+}
+
 /* traverse.py:194-323 with first_hit = False, all geoms in index order. */
 static HitO closest_o(const Ctx* c, v3 o, v3 d, float max_dist) {
   const MjpbrGeometry* g = &c->sc->geo;
   int cull = c->s->cull_backfaces;
   HitO h = {-1, max_dist, {0, 0, 0}, 0, 0, -1, -1};
-  for (int k = 0; k < g->nbvh; ++k) {
-    int gi = g->bvh_geom[k];
+  int ib = 0, id = 0;
+  for (int k = 0; k < g->nbvh + g->ndirect; ++k) {
+    int gi = enabled_geom_o(g, k, &ib, &id);
     if (geom_skipped(c, gi)) continue;
     v3 n = mk3(0, 0, 0);
     float bu = 0, bv = 0;
@@ -292,8 +301,9 @@ static HitO closest_o(const Ctx* c, v3 o, v3 d, float max_dist) {
 static int any_o(const Ctx* c, v3 o, v3 d, float max_dist) {
   const MjpbrGeometry* g = &c->sc->geo;
   int cull = c->s->cull_backfaces;
-  for (int k = 0; k < g->nbvh; ++k) {
-    int gi = g->bvh_geom[k];
+  int ib = 0, id = 0;
+  for (int k = 0; k < g->nbvh + g->ndirect; ++k) {
+    int gi = enabled_geom_o(g, k, &ib, &id);
     if (geom_skipped(c, gi)) continue;
     v3 n = mk3(0, 0, 0);
     float bu, bv;

@@ -275,3 +275,28 @@ def test_hdr_is_consistent_with_display_output():
   s = Shot(display_scene(), render_hdr=True, tone_mapper=mj.ToneMapper.CLAMP, ev100=io.DEFAULT_EV100).render()
   expected = _srgb(np.clip(s.hdr()[0] * s.exposure, 0.0, 1.0))
   np.testing.assert_allclose(s.rgb()[0], expected, atol=1.5 / 255)
+
+
+def test_unbounded_planes_are_tested_outside_the_scene_bvh():
+  """Planes of unbounded extent go to MjpbrGeometry.direct_geom (tested by every ray before the
+  scene BVH); finite planes and other geoms stay in the BVH. The oracle's brute force walks the
+  two lists merged in index order, so the image equals the one over the single enabled list."""
+  b = _Builder()
+  b.geom(mdl.GEOM_SPHERE, (0.2, 0, 0), (0, 0, 0.2))
+  b.geom(mdl.GEOM_PLANE, (0, 0, 0.1), (0, 0, 0))
+  b.geom(mdl.GEOM_PLANE, (0.3, 0.3, 0.1), (0.4, 0.4, 0.3))
+  b.geom(mdl.GEOM_BOX, (0.1, 0.1, 0.1), (-0.4, 0.1, 0.1))
+  b.light(mdl.LIGHT_DIRECTIONAL, (0, 0, 3), (0.3, 0.2, -1), intensity=2.0, castshadow=1)
+  b.camera((0, -2.0, 1.2), (0, 0, 0.1), 45, (24, 24))
+  mjm = b.build()
+  m, d, rc = make(mjm, cam_res=(24, 24))
+  assert rc.base.direct_geom_ids.tolist() == [1]
+  assert rc.base.bvh_geom_ids.tolist() == [0, 2, 3]
+  assert rc.base.tlas_depth == mj.bvh.scene_depth(3)
+  split = oracle.rt_render(m, d, rc)
+  rc.base.bvh_geom_ids, rc.base.direct_geom_ids = None, None
+  single = oracle.rt_render(m, d, rc)
+  for k in ("seg", "depth", "hdr", "rgb"):
+    np.testing.assert_array_equal(split[k], single[k])
+  seg = split["seg"][..., 0]
+  assert {0, 1, 2, 3} <= set(np.unique(seg).tolist())
