@@ -34,10 +34,12 @@ MJ_DEV V2 shadow_rand2(int seed, int stream, int nlight, int li, int nsample, in
   return V2{rand_at(seed, counter), rand_at(seed, counter + 1)};
 }
 
-// render.py:95-127. The reference's three cases (directional: one unbounded ray; one sample: one
-// ray to the light; soft: `shadow_samples` rays to the bulb disc) go through one any-hit site, so
-// the kernel holds one inlined traversal for shadows. With one ray, v is 0 or 1 and
-// v + (1 - v) * fraction equals the reference's `hit ? fraction : 1` exactly.
+// render.py:95-127. Directional lights and lights sampled once take one any-hit ray; soft shadows
+// trace `shadow_samples` rays to points of the bulb's disc, four at a time as one packet query
+// (they share the origin). With one ray, v is 0 or 1 and v + (1 - v) * fraction equals the
+// reference's `hit ? fraction : 1` exactly; `visible` counts unoccluded rays, an exact sum.
+constexpr int kShadowPacket = 4;
+
 template <unsigned F>
 MJ_DEV float light_visibility(const RtCtx& c, const Light& L, V3 point, V3 normal, V3 l, float dist, int stream,
                               int nlight, int li) {
@@ -47,24 +49,31 @@ MJ_DEV float light_visibility(const RtCtx& c, const Light& L, V3 point, V3 norma
   bool cull = s.cull_backfaces != 0;
   const bool directional = L.type == MJPBR_LIGHT_DIRECTIONAL;
   const bool soft = (F & kFSoft) && !directional && s.shadow_samples > 1;
-  const int n = soft ? s.shadow_samples : 1;
-  V3 t1 = v3s(0.0f), t2 = v3s(0.0f);
-  if (soft) orthonormal_basis(l, &t1, &t2);
+  if (!soft) {
+    const bool hit = trace_any<kMesh>(c.sv, origin, l, directional ? kUnboundedShadow : dist - kShadowGap, cull);
+    return hit ? s.shadow_light_fraction : 1.0f;
+  }
+  const int n = s.shadow_samples;
+  V3 t1, t2;
+  orthonormal_basis(l, &t1, &t2);
   float visible = 0.0f;
-  for (int k = 0; k < n; ++k) {
-    V3 dir = l;
-    float tmax = directional ? kUnboundedShadow : dist - kShadowGap;
-    if (soft) {
-      V2 u = shadow_rand2(c.seed, stream, nlight, li, s.shadow_samples, k);
+  for (int k0 = 0; k0 < n; k0 += kShadowPacket) {
+    V3 dirs[kShadowPacket];
+    float tmax[kShadowPacket];
+#pragma unroll
+    for (int j = 0; j < kShadowPacket; ++j) {
+      V2 u = shadow_rand2(c.seed, stream, nlight, li, n, min(k0 + j, n - 1));
       float r = L.bulbradius * sqrtf(u.x);
       float sp, cp;
       sincos_turns(u.y, &sp, &cp);
       V3 to_target = L.pos + (t1 * cp + t2 * sp) * r - origin;
       float d = len(to_target);
-      dir = to_target / d;
-      tmax = d - kShadowGap;
+      dirs[j] = to_target / d;
+      tmax[j] = d - kShadowGap;
     }
-    if (!trace_any<kMesh>(c.sv, origin, dir, tmax, cull)) visible += 1.0f;
+    const int m = min(kShadowPacket, n - k0);
+    const uint32_t hit = trace_any_packet<kMesh, kShadowPacket>(c.sv, origin, dirs, tmax, m, cull);
+    visible += (float)(m - __builtin_popcount(hit));
   }
   float v = visible / (float)n;
   return v + (1.0f - v) * s.shadow_light_fraction;

@@ -290,4 +290,108 @@ MJ_DEV bool trace_any(const SceneView& sv, V3 o, V3 d, float max_dist, bool cull
   return hit;
 }
 
+// Any-hit queries of up to N rays from one origin (the soft-shadow rays towards one light), in
+// one traversal of the scene BVH. Returns bit k set when ray k (k < n) hits something before
+// tmax[k]. A node is entered when the hull of the rays' slab intervals meets its box: per axis,
+// (bound - o) * r is monotonic in the reciprocal direction r, so the extreme rays' float products
+// bound every ray's, and the hull test accepts every box a ray's own test accepts (an axis on
+// which the rays' directions differ in sign or have a zero component constrains nothing). Each
+// unresolved ray is tested against each reached leaf's geom exactly as in trace_any, so every
+// ray's result is that of trace_any: a geom a ray hits lies in its padded box, which its own slab
+// test, and so the hull test, accepts.
+template <bool kMesh, int N>
+MJ_DEV uint32_t trace_any_packet(const SceneView& sv, V3 o, const V3 (&d)[N], const float (&tmax)[N], int n, bool cull) {
+  const uint32_t all = n >= 32 ? 0xffffffffu : ((1u << n) - 1u);
+  uint32_t hit = 0u;
+  // reciprocal ranges per axis; free[a]: the axis does not constrain the packet
+  float rlo[3], rhi[3];
+  bool free_axis[3];
+  float cut = 0.0f;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    rlo[a] = INFINITY;
+    rhi[a] = -INFINITY;
+    free_axis[a] = false;
+  }
+  bool pos[3] = {false, false, false}, neg[3] = {false, false, false};
+#pragma unroll
+  for (int k = 0; k < N; ++k) {
+    if (k >= n) break;
+    const float c[3] = {d[k].x, d[k].y, d[k].z};
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      const float r = 1.0f / c[a];
+      if (c[a] > 0.0f) pos[a] = true;
+      else if (c[a] < 0.0f) neg[a] = true;
+      else free_axis[a] = true;
+      if (!(fabsf(r) < INFINITY)) free_axis[a] = true;
+      rlo[a] = fminf(rlo[a], r);
+      rhi[a] = fmaxf(rhi[a], r);
+    }
+    cut = fmaxf(cut, cut_of(tmax[k]));
+  }
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+    if (pos[a] && neg[a]) free_axis[a] = true;
+  const float oc[3] = {o.x, o.y, o.z};
+  // entry distance of the packet's hull into box [lo, hi], or -1 when it misses before the cut
+  auto hull_box = [&](const float* lo, const float* hi) {
+    float tmin = 0.0f, tmx = cut;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      if (free_axis[a]) continue;
+      const float t1 = lo[a] - oc[a], t2 = hi[a] - oc[a];
+      const float p1 = t1 * rlo[a], p2 = t1 * rhi[a], p3 = t2 * rlo[a], p4 = t2 * rhi[a];
+      tmin = fmaxf(tmin, fminf(fminf(p1, p2), fminf(p3, p4)));
+      tmx = fminf(tmx, fmaxf(fmaxf(p1, p2), fmaxf(p3, p4)));
+    }
+    return (tmx >= tmin && tmin < cut) ? tmin : -1.0f;
+  };
+  // tests the unresolved rays against geom gi; true when every ray is resolved
+  auto visit = [&](int gi, auto kPlane) {
+    if (gi < 0) return false;
+    const float* g = sv.geoms + kGeomRec * gi;
+    const int bits = __float_as_int(g[15]);
+    if (bits & 0x100) return false;
+#pragma unroll
+    for (int k = 0; k < N; ++k) {
+      if (k >= n || (hit >> k) & 1u) continue;
+      V3 nrm = v3s(0.0f);
+      float bu, bv;
+      int face, mesh;
+      float dd = intersect_geom<true, kMesh, decltype(kPlane)::value>(sv, g, o, d[k], tmax[k], &nrm, &bu, &bv, &face, &mesh);
+      if (cull && dd >= 0.0f && dot(d[k], nrm) > 0.0f) dd = -1.0f;
+      if (dd >= 0.0f && dd < tmax[k]) hit |= 1u << k;
+    }
+    return hit == all;
+  };
+  for (int k = 0; k < sv.ndirect; ++k)
+    if (visit(sv.direct[k], std::true_type{})) return hit;
+  const int depth = sv.depth;
+  const int ninner = (1 << depth) - 1;
+  int node = 0, level = 0;
+  uint32_t trail = 0u;
+  for (;;) {
+    if (level < depth) {
+      const float* b = sv.tlas_box + 12 * node;
+      const float ta = hull_box(b, b + 3), tb = hull_box(b + 6, b + 9);
+      if (ta >= 0.0f || tb >= 0.0f) {
+        const int nearside = (tb >= 0.0f && (ta < 0.0f || tb < ta)) ? 1 : 0;
+        if (ta >= 0.0f && tb >= 0.0f) trail |= 1u << level;
+        node = 2 * node + 1 + nearside;
+        ++level;
+        continue;
+      }
+    } else if (visit(sv.tlas_leaf[node - ninner], std::false_type{})) {
+      return hit;
+    }
+    if (trail == 0u) return hit;
+    const int L = 31 - __builtin_clz(trail);
+    trail &= ~(1u << L);
+    const int anc1 = (node + 1) >> (level - L - 1);
+    node = (anc1 ^ 1) - 1;
+    level = L + 1;
+  }
+}
+
 }  // namespace mjpbr
