@@ -358,8 +358,10 @@ int mjpbr_pt_wavefront_launch(void* workspace, const MjpbrWavefrontPlan* plan, c
 /* À-trous denoiser over the path tracer's noisy HDR and first-hit features (out->hdr, aov_*):
  * demodulate, `iterations` edge-avoiding passes, remodulate + tone map + encode into out->rgb
  * (and out->hdr with render_hdr) (reference denoise.py:214-248). work0 / work1: two
- * [nworld, stride, 4] f32 buffers. */
-int mjpbr_atrous(const MjpbrAtrousParams* p, const MjpbrOutputs* out, float* work0, float* work1,
+ * [nworld, stride, 4] f32 buffers; guides: one more [nworld, stride, 4] 32-bit buffer that the
+ * demodulation fills with each pixel's normal, albedo (f16, as in aov_normal / aov_albedo) and
+ * depth packed into 16 bytes, so every filter tap reads its guides in one load. */
+int mjpbr_atrous(const MjpbrAtrousParams* p, const MjpbrOutputs* out, float* work0, float* work1, float* guides,
                  void* stream);
 
 /* Context precompute (reference ibl.py:62-104, envmap.py:171-221). */

@@ -44,7 +44,7 @@ FUNCTIONS = {
   "mjpbr_rt_kernel_features": (ctypes.c_int, [_VP]),
   "mjpbr_world_stage_floats": (ctypes.c_int64, [_VP]),
   "mjpbr_pt_render": (ctypes.c_int, [_VP, _VP, _VP, ctypes.c_int32, _VP]),
-  "mjpbr_atrous": (ctypes.c_int, [_VP, _VP, _VP, _VP, _VP]),
+  "mjpbr_atrous": (ctypes.c_int, [_VP, _VP, _VP, _VP, _VP, _VP]),
   "mjpbr_dfg_lut": (ctypes.c_int, [_VP, ctypes.c_int, ctypes.c_int, _VP]),
   "mjpbr_env_prefilter": (
     ctypes.c_int,

@@ -6,7 +6,12 @@
 // noise estimate) and albedo differences, variance propagated as sum w^2 var / (sum w)^2  ->
 // remodulate, tone map, encode. The remodulation is fused into the last pass (one launch and one
 // image round trip less than the reference's separate finish kernel). One thread per pixel; the
-// per-world images are small enough that the 5x5 neighbourhood reads are served by L2.
+// per-world images are small enough that the 5x5 neighbourhood reads are served by L2. The
+// demodulation packs each pixel's guides (normal and albedo as their f16 bits, depth) into one
+// 16-byte record, so a tap reads its guides with one load beside its colour. The edge weight's
+// power and exponential use the hardware exp2 / log2 (exp2(s log2 x), exp2(-d log2 e)); they
+// differ from correctly rounded powf / expf by a few ulp of the weight, far inside the 1e-4
+// relative agreement the filtered image is held to.
 #pragma clang fp contract(off)
 #include "abi_util.hpp"
 
@@ -29,6 +34,19 @@ MJ_DEV Locate locate(const MjpbrAtrousParams& p, int index) {
 
 MJ_DEV V3 ld_half3(const uint16_t* p) { return V3{half_to_float(p[0]), half_to_float(p[1]), half_to_float(p[2])}; }
 
+// Guide record: {n.x | n.y << 16, n.z | a.x << 16, a.y | a.z << 16, depth bits} (f16 bit patterns).
+MJ_DEV uint4 pack_guides(const uint16_t* n, const uint16_t* a, float depth) {
+  return make_uint4((uint32_t)n[0] | ((uint32_t)n[1] << 16), (uint32_t)n[2] | ((uint32_t)a[0] << 16),
+                    (uint32_t)a[1] | ((uint32_t)a[2] << 16), __float_as_uint(depth));
+}
+MJ_DEV V3 guide_normal(uint4 g) {
+  return V3{half_to_float((uint16_t)(g.x & 0xffffu)), half_to_float((uint16_t)(g.x >> 16)), half_to_float((uint16_t)(g.y & 0xffffu))};
+}
+MJ_DEV V3 guide_albedo(uint4 g) {
+  return V3{half_to_float((uint16_t)(g.y >> 16)), half_to_float((uint16_t)(g.z & 0xffffu)), half_to_float((uint16_t)(g.z >> 16))};
+}
+MJ_DEV float guide_depth(uint4 g) { return __uint_as_float(g.w); }
+
 MJ_DEV float b3(int i) { return i == 2 ? 0.375f : ((i == 1 || i == 3) ? 0.25f : 0.0625f); }
 
 MJ_DEV V3 demodulated(V3 c, V3 albedo) {
@@ -47,12 +65,13 @@ MJ_DEV void finish_pixel(const MjpbrAtrousParams& p, const MjpbrOutputs& out, in
 }
 
 // denoise.py:95-102
-__global__ void demodulate_kernel(MjpbrAtrousParams p, MjpbrOutputs out, float4* dst, int finish) {
+__global__ void demodulate_kernel(MjpbrAtrousParams p, MjpbrOutputs out, float4* dst, uint4* guides, int finish) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= (int64_t)p.nworld * p.stride) return;
   int w = (int)(i / p.stride), index = (int)(i - (int64_t)w * p.stride);
   if (p.world_select && !p.world_select[w]) return;
   int64_t a = (int64_t)w * out.aov_stride + index;
+  if (!finish) guides[i] = pack_guides(out.aov_normal + 3 * a, out.aov_albedo + 3 * a, out.aov_depth[a]);
   V3 albedo = ld_half3(out.aov_albedo + 3 * a);
   V3 c = ld3(out.hdr + 3 * ((int64_t)w * out.hdr_stride + index)) * p.exposure[w % p.exposure_n] - ld3(out.aov_emission + 3 * a);
   c = demodulated(V3{fmaxf(c.x, 0.0f), fmaxf(c.y, 0.0f), fmaxf(c.z, 0.0f)}, albedo);
@@ -65,7 +84,8 @@ __global__ void demodulate_kernel(MjpbrAtrousParams p, MjpbrOutputs out, float4*
 }
 
 // denoise.py:105-178; `finish` fuses denoise.py:181-201 into the last pass.
-__global__ void atrous_kernel(MjpbrAtrousParams p, MjpbrOutputs out, int step, const float4* src, float4* dst, int finish) {
+__global__ void atrous_kernel(MjpbrAtrousParams p, MjpbrOutputs out, int step, const float4* src, const uint4* guides,
+                              float4* dst, int finish) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= (int64_t)p.nworld * p.stride) return;
   int w = (int)(i / p.stride), index = (int)(i - (int64_t)w * p.stride);
@@ -73,8 +93,9 @@ __global__ void atrous_kernel(MjpbrAtrousParams p, MjpbrOutputs out, int step, c
   Locate loc = locate(p, index);
   if (loc.adr < 0) return;
   const float4* srcw = src + (int64_t)w * p.stride;
-  int64_t fa = (int64_t)w * out.aov_stride;
-  V3 n0 = ld_half3(out.aov_normal + 3 * (fa + index));
+  const uint4* gw = guides + (int64_t)w * p.stride;
+  const uint4 g0 = gw[index];
+  V3 n0 = guide_normal(g0);
   float4 s0 = srcw[index];
   if (len_sq(n0) == 0.0f) {
     if (finish)
@@ -97,8 +118,8 @@ __global__ void atrous_kernel(MjpbrAtrousParams p, MjpbrOutputs out, int step, c
   }
   float inv_sigma_l = 1.0f / (p.sigma_color * sqrtf(fmaxf(variance / vweight, 0.0f)) + 1.0e-6f);
   float l0 = luminance(V3{s0.x, s0.y, s0.z});
-  float z0 = out.aov_depth[fa + index];
-  V3 a0 = ld_half3(out.aov_albedo + 3 * (fa + index));
+  float z0 = guide_depth(g0);
+  V3 a0 = guide_albedo(g0);
   float inv_sigma_a = 1.0f / p.sigma_albedo;
   float inv_sigma_z = 1.0f / fmaxf(p.sigma_depth * z0 * (float)step, 1.0e-6f);
   V3 total = v3s(0.0f);
@@ -111,11 +132,13 @@ __global__ void atrous_kernel(MjpbrAtrousParams p, MjpbrOutputs out, int step, c
       if (x < 0 || x >= W) continue;
       int q = loc.adr + y * W + x;
       float4 s = srcw[q];
+      const uint4 gq = gw[q];
       V3 c = V3{s.x, s.y, s.z};
-      float wn = powf(fmaxf(dot(n0, ld_half3(out.aov_normal + 3 * (fa + q))), 0.0f), p.sigma_normal);
-      float dist = fabsf(out.aov_depth[fa + q] - z0) * inv_sigma_z + fabsf(luminance(c) - l0) * inv_sigma_l +
-                   len(ld_half3(out.aov_albedo + 3 * (fa + q)) - a0) * inv_sigma_a;
-      float weight = b3(k) * b3(j) * wn * expf(-dist);
+      const float nd = fmaxf(dot(n0, guide_normal(gq)), 0.0f);
+      float wn = nd > 0.0f ? __powf(nd, p.sigma_normal) : powf(0.0f, p.sigma_normal);
+      float dist = fabsf(guide_depth(gq) - z0) * inv_sigma_z + fabsf(luminance(c) - l0) * inv_sigma_l +
+                   len(guide_albedo(gq) - a0) * inv_sigma_a;
+      float weight = b3(k) * b3(j) * wn * __expf(-dist);
       total += c * weight;
       total_var += weight * weight * s.w;
       wsum += weight;
@@ -133,7 +156,7 @@ __global__ void atrous_kernel(MjpbrAtrousParams p, MjpbrOutputs out, int step, c
 
 using namespace mjpbr;
 
-extern "C" int mjpbr_atrous(const MjpbrAtrousParams* p, const MjpbrOutputs* out, float* work0, float* work1,
+extern "C" int mjpbr_atrous(const MjpbrAtrousParams* p, const MjpbrOutputs* out, float* work0, float* work1, float* guides,
                             void* stream) {
   if (p->nworld < 1 || p->stride < 1) return set_error(MJPBR_EINVAL, "atrous: empty image");
   if (p->nrender < 1 || p->nrender > MJPBR_MAX_CAMS) return set_error(MJPBR_EINVAL, "atrous: bad camera count");
@@ -144,12 +167,14 @@ extern "C" int mjpbr_atrous(const MjpbrAtrousParams* p, const MjpbrOutputs* out,
   hipStream_t st = (hipStream_t)stream;
   float4* a = reinterpret_cast<float4*>(work0);
   float4* b = reinterpret_cast<float4*>(work1);
-  hipLaunchKernelGGL(demodulate_kernel, dim3(blocks), dim3(threads), 0, st, *p, *out, a, p->iterations == 0 ? 1 : 0);
+  if (p->iterations > 0 && !guides) return set_error(MJPBR_EINVAL, "atrous: the guide buffer must not be NULL");
+  uint4* g = reinterpret_cast<uint4*>(guides);
+  hipLaunchKernelGGL(demodulate_kernel, dim3(blocks), dim3(threads), 0, st, *p, *out, a, g, p->iterations == 0 ? 1 : 0);
   int rc = check_launch("demodulate_kernel");
   if (rc) return rc;
   for (int it = 0; it < p->iterations; ++it) {
     int last = it == p->iterations - 1;
-    hipLaunchKernelGGL(atrous_kernel, dim3(blocks), dim3(threads), 0, st, *p, *out, 1 << it, a, b, last);
+    hipLaunchKernelGGL(atrous_kernel, dim3(blocks), dim3(threads), 0, st, *p, *out, 1 << it, a, g, b, last);
     rc = check_launch("atrous_kernel");
     if (rc) return rc;
     float4* t = a;

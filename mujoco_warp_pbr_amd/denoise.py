@@ -34,5 +34,5 @@ def denoise(rc: PbrRenderContext, nworld: int, out, stream: int) -> None:
   """Filters the noisy HDR in `rc.hdr_data` (with the first-hit AOVs) into the RGB (and HDR)
   outputs, in place, on `stream`."""
   p = atrous_params(rc, nworld)
-  a, b = rc.denoise_buffers
-  _abi.call("mjpbr_atrous", _abi.byref(p), _abi.byref(out), _abi.ptr(a), _abi.ptr(b), stream)
+  a, b, guides = rc.denoise_buffers
+  _abi.call("mjpbr_atrous", _abi.byref(p), _abi.byref(out), _abi.ptr(a), _abi.ptr(b), _abi.ptr(guides), stream)

@@ -399,7 +399,7 @@ def create_render_context(
     aov_depth=zeros((nworld, aov)),
     aov_emission=zeros((nworld, aov, 3)),
     aov_variance=zeros((nworld, aov)),
-    denoise_buffers=(zeros((nworld, aov, 4)), zeros((nworld, aov, 4))),
+    denoise_buffers=(zeros((nworld, aov, 4)), zeros((nworld, aov, 4)), zeros((nworld, aov, 4))),
     render_hdr=render_hdr,
     hdr_data=zeros((nworld, npixel if render_hdr or denoise else 1, 3)),
     render_guides=render_guides,

@@ -399,8 +399,8 @@ class Denoiser:
       )
     p = atrous.atrous_params(rc, nworld)
     p.world_select = _abi.ptr(self.fallback)
-    a, b = rc.denoise_buffers
-    _abi.call("mjpbr_atrous", _abi.byref(p), _abi.byref(out_struct), _abi.ptr(a), _abi.ptr(b), stream)
+    a, b, guides = rc.denoise_buffers
+    _abi.call("mjpbr_atrous", _abi.byref(p), _abi.byref(out_struct), _abi.ptr(a), _abi.ptr(b), _abi.ptr(guides), stream)
     for cam in cams:
       x = self.inputs[cam]
       w, h = int(base.cam_res[cam][0]), int(base.cam_res[cam][1])
