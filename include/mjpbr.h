@@ -71,9 +71,10 @@ typedef struct MjpbrGeometry {
   float* tlas_box;             /* [nworld, 2^depth - 1, 12] */
   int32_t* tlas_leaf;          /* [nworld, 2^depth] */
   const int32_t* bvh_geom;     /* [nbvh] enabled geom ids in the BVH */
-  /* Enabled geoms every ray query tests directly, before the BVH: planes of unbounded extent,
-   * whose boxes would widen every ancestor node. Ascending ids, disjoint from bvh_geom; the
-   * enabled set is bvh_geom and direct_geom together. */
+  /* Enabled geoms every ray query tests directly, before the BVH: ground planes, whose boxes
+   * would widen every ancestor node. Planes only (the direct test is the plane routine; other
+   * types listed here would never be hit). Ascending ids, disjoint from bvh_geom; the enabled
+   * set is bvh_geom and direct_geom together. */
   const int32_t* direct_geom;  /* [ndirect] */
   int32_t ndirect;
   int32_t pad1;

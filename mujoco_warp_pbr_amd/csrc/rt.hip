@@ -172,10 +172,11 @@ struct RtInstance {
 
 // Compiled instances, each launched for the settings whose required features it covers, at the
 // occupancy measured best for it (waves per SIMD at 1024 worlds x 128^2: the reflective-plane
-// textured scene runs 6% faster at 6 than at 8, the others are fastest at 8).
+// textured scene runs 6 % faster at 6 than at 8 and 3 % slower at 5; the leanest instance 2 %
+// faster at 7 than at 8 or 6; the full-feature one 1 % slower at 7 than at 8).
 #define MJPBR_RT_INSTANCE(F, W) RtInstance{F, rt_kernel<F, W>}
 const RtInstance kRtInstances[] = {
-    MJPBR_RT_INSTANCE(0u, 8),
+    MJPBR_RT_INSTANCE(0u, 7),
     MJPBR_RT_INSTANCE(kFTex, 8),
     MJPBR_RT_INSTANCE(kFMirror | kFTex, 6),
     MJPBR_RT_INSTANCE(kFMesh, 8),

@@ -251,13 +251,18 @@ def create_render_context(
 
   dev = torch.device(device)
   enabled = np.nonzero(rendered)[0].astype(np.int32)
-  # Planes of unbounded extent (a nonpositive half-size) are tested by every ray before the
-  # scene BVH instead of sitting in it, where their box would cover every ancestor node.
+  # Ground planes are tested by every ray before the scene BVH instead of sitting in it, where a
+  # plane's box (unbounded, or a floor as wide as the scene) would cover every ancestor node:
+  # planes of unbounded extent (a nonpositive half-size) always, and every plane when the model
+  # has at most MAX_DIRECT_PLANES of them (a plane test costs about one box test).
   gsize = np.asarray(mjm.geom_size, np.float32).reshape(-1, 3)
-  unbounded = (mjm.geom_type[enabled] == mdl.GEOM_PLANE) & ((gsize[enabled, 0] <= 0) | (gsize[enabled, 1] <= 0))
-  if np.all(unbounded):
-    unbounded[:] = False
-  bvh_ids, direct_ids = enabled[~unbounded], enabled[unbounded]
+  planes = mjm.geom_type[enabled] == mdl.GEOM_PLANE
+  direct = planes & ((gsize[enabled, 0] <= 0) | (gsize[enabled, 1] <= 0))
+  if np.count_nonzero(planes) <= MAX_DIRECT_PLANES:
+    direct = planes.copy()
+  if np.all(direct):
+    direct[:] = False
+  bvh_ids, direct_ids = enabled[~direct], enabled[direct]
   depth = bvh_lib.scene_depth(len(bvh_ids))
   tlas_box, tlas_leaf = bvh_lib.alloc_scene_bvh(nworld, depth, dev)
   meshes = bvh_lib.build_mesh_bvhs(mjm)
@@ -418,6 +423,10 @@ def create_render_context(
 
 def render_struct_max_cams() -> int:
   return 16
+
+
+# Planes tested before the scene BVH when a model has at most this many (see create_render_context).
+MAX_DIRECT_PLANES = 4
 
 
 def world_stage_floats(tlas_depth: int, ngeom: int, nlight: int, ndirect: int = 0) -> int:

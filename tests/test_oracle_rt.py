@@ -277,10 +277,11 @@ def test_hdr_is_consistent_with_display_output():
   np.testing.assert_allclose(s.rgb()[0], expected, atol=1.5 / 255)
 
 
-def test_unbounded_planes_are_tested_outside_the_scene_bvh():
-  """Planes of unbounded extent go to MjpbrGeometry.direct_geom (tested by every ray before the
-  scene BVH); finite planes and other geoms stay in the BVH. The oracle's brute force walks the
-  two lists merged in index order, so the image equals the one over the single enabled list."""
+def test_ground_planes_are_tested_outside_the_scene_bvh():
+  """Planes go to MjpbrGeometry.direct_geom (tested by every ray before the scene BVH) when the
+  model has at most io.MAX_DIRECT_PLANES of them, otherwise only the unbounded ones; other geoms
+  stay in the BVH. The oracle's brute force walks the two lists merged in index order, so the
+  image equals the one over the single enabled list."""
   b = _Builder()
   b.geom(mdl.GEOM_SPHERE, (0.2, 0, 0), (0, 0, 0.2))
   b.geom(mdl.GEOM_PLANE, (0, 0, 0.1), (0, 0, 0))
@@ -290,9 +291,9 @@ def test_unbounded_planes_are_tested_outside_the_scene_bvh():
   b.camera((0, -2.0, 1.2), (0, 0, 0.1), 45, (24, 24))
   mjm = b.build()
   m, d, rc = make(mjm, cam_res=(24, 24))
-  assert rc.base.direct_geom_ids.tolist() == [1]
-  assert rc.base.bvh_geom_ids.tolist() == [0, 2, 3]
-  assert rc.base.tlas_depth == mj.bvh.scene_depth(3)
+  assert rc.base.direct_geom_ids.tolist() == [1, 2]
+  assert rc.base.bvh_geom_ids.tolist() == [0, 3]
+  assert rc.base.tlas_depth == mj.bvh.scene_depth(2)
   split = oracle.rt_render(m, d, rc)
   rc.base.bvh_geom_ids, rc.base.direct_geom_ids = None, None
   single = oracle.rt_render(m, d, rc)
@@ -300,3 +301,11 @@ def test_unbounded_planes_are_tested_outside_the_scene_bvh():
     np.testing.assert_array_equal(split[k], single[k])
   seg = split["seg"][..., 0]
   assert {0, 1, 2, 3} <= set(np.unique(seg).tolist())
+  # with more planes than the limit, only the unbounded one is tested directly
+  b = _Builder()
+  for k in range(mj.io.MAX_DIRECT_PLANES + 1):
+    b.geom(mdl.GEOM_PLANE, (0.1, 0.1, 0.1) if k else (0, 0, 0.1), (0.3 * k, 0, 0.01 * k))
+  b.geom(mdl.GEOM_SPHERE, (0.2, 0, 0), (0, 0, 0.2))
+  b.camera((0, -2.0, 1.2), (0, 0, 0.1), 45, (8, 8))
+  _, _, rc2 = make(b.build(), cam_res=(8, 8))
+  assert rc2.base.direct_geom_ids.tolist() == [0]
