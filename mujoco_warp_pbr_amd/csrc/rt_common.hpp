@@ -110,7 +110,10 @@ MJ_DEV V3 shade_light(const RtCtx& c, const Light& L, const Lobes& lb, V3 point,
 }
 
 // The image-based part of render.py:184-221: split-sum diffuse and specular with Lagarde
-// specular occlusion; also returns the lobes and n.v the lights need.
+// specular occlusion; also returns the lobes and n.v the lights need. kStagedYaw: the sky
+// rotation's cosine and sine come from the world's staged record (the same values, evaluated once
+// per world) instead of being evaluated here; instances pick it where it measured faster.
+template <bool kStagedYaw>
 MJ_DEV V3 ambient_radiance(const RtCtx& c, const Surface& sf, V3 normal, V3 view, Lobes* lobes, float* nov_out) {
   const MjpbrEnvironment& env = c.sc.env;
   float nov = fmaxf(dot(normal, view), 1.0e-4f);
@@ -122,7 +125,8 @@ MJ_DEV V3 ambient_radiance(const RtCtx& c, const Surface& sf, V3 normal, V3 view
     int e;
     float yaw, scale;
     env_lookup(env, c.w, &e, &yaw, &scale);
-    irradiance = ambient + eval_sh9(env.env_sh + e * 27, rotate_yaw(normal, -yaw)) * scale;
+    const V3 rn = kStagedYaw ? rotate_cs(normal, c.sv.env[0], c.sv.env[1]) : rotate_yaw(normal, -yaw);
+    irradiance = ambient + eval_sh9(env.env_sh + e * 27, rn) * scale;
     V3 r = view - normal * (2.0f * dot(view, normal));
     V3 dominant = normalize(-r * (1.0f - alpha * alpha) + normal * (alpha * alpha));
     prefiltered = ambient + sample_specular(env, e, dominant, yaw, lb.perceptual_roughness) * scale;
@@ -139,7 +143,8 @@ template <unsigned F>
 MJ_DEV V3 shade_surface(const RtCtx& c, const Surface& sf, V3 point, V3 normal, V3 view, V3 head_dir, int stream) {
   Lobes lb;
   float nov;
-  V3 radiance = ambient_radiance(c, sf, normal, view, &lb, &nov);
+  // staged sky rotation: 1.5 % faster without meshes, 4.5 % slower with them (1024 x 128^2)
+  V3 radiance = ambient_radiance<(F & kFMesh) == 0>(c, sf, normal, view, &lb, &nov);
   const MjpbrLights& lt = c.sc.lit;
   int nl = lt.nlight + (c.s.headlight_active ? 1 : 0);
   for (int li = 0; li < nl; ++li) {

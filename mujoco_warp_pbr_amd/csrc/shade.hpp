@@ -134,6 +134,9 @@ MJ_DEV V3 uv_to_direction(V2 uv) {
   return V3{st * cp, st * sp, ct};
 }
 
+// d rotated about z by the angle whose cosine and sine are c, s.
+MJ_DEV V3 rotate_cs(V3 d, float c, float s) { return V3{c * d.x - s * d.y, s * d.x + c * d.y, d.z}; }
+
 MJ_DEV V3 rotate_yaw(V3 d, float yaw) {
   float c = cosf(yaw), s = sinf(yaw);
   return V3{c * d.x - s * d.y, s * d.x + c * d.y, d.z};

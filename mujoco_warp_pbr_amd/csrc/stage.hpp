@@ -8,12 +8,13 @@ namespace mjpbr {
 
 
 // Floats of one world's staged records: scene BVH boxes, geom records, leaf ids and direct geom
-// ids (each padded to a multiple of 4), light records, surface records. A multiple of 4, so
-// records copy as float4.
+// ids (each padded to a multiple of 4), light records, surface records, the environment record
+// (kEnvRec). A multiple of 4, so records copy as float4.
 MJ_HOST_DEV inline int64_t stage_floats(const MjpbrScene& sc) {
   int nleaf = 1 << sc.geo.tlas_depth;
   return (int64_t)(nleaf - 1) * 12 + (int64_t)sc.geo.ngeom * kGeomRec + staged_leaf_slots(nleaf) +
-         staged_leaf_slots(sc.geo.ndirect) + (int64_t)sc.lit.nlight * kLightRec + (int64_t)sc.geo.ngeom * kSurfRec;
+         staged_leaf_slots(sc.geo.ndirect) + (int64_t)sc.lit.nlight * kLightRec + (int64_t)sc.geo.ngeom * kSurfRec +
+         kEnvRec;
 }
 
 // Resolves world w's records into dst (LDS or global memory) with the block's threads: the
@@ -31,6 +32,15 @@ MJ_DEV void stage_world_to(const MjpbrScene& sc, int w, float* dst) {
   for (int li = threadIdx.x; li < sc.lit.nlight; li += blockDim.x) stage_light(sc.lit, w, li, lights + li * kLightRec);
   float* surfs = lights + sc.lit.nlight * kLightRec;
   for (int gi = threadIdx.x; gi < g.ngeom; gi += blockDim.x) stage_surface(sc, w, gi, surfs + gi * kSurfRec);
+  // environment record: cosine and sine of the negated sky rotation (the irradiance lookup
+  // rotates the normal by -yaw), evaluated once per world
+  if (threadIdx.x == 0) {
+    float* env = surfs + g.ngeom * kSurfRec;
+    const float yaw = sc.env.env_yaw[w % sc.env.env_yaw_n];
+    env[0] = cosf(-yaw);
+    env[1] = sinf(-yaw);
+    env[2] = env[3] = 0.0f;
+  }
   const float4* src = reinterpret_cast<const float4*>(g.tlas_box + (int64_t)w * ninner * 12);
   float4* bdst = reinterpret_cast<float4*>(box);
   for (int i = threadIdx.x; i < ninner * 3; i += blockDim.x) bdst[i] = src[i];
@@ -71,6 +81,7 @@ MJ_DEV SceneView scene_view(const MjpbrScene& sc, float* lds) {
   sv.ndirect = g.ndirect;
   sv.lights = reinterpret_cast<const float*>(sv.direct + staged_leaf_slots(g.ndirect));
   sv.surfs = sv.lights + sc.lit.nlight * kLightRec;
+  sv.env = sv.surfs + g.ngeom * kSurfRec;
   sv.depth = g.tlas_depth;
   sv.blas_box = g.blas_box;
   sv.blas_tri = g.blas_tri;

@@ -28,6 +28,8 @@ constexpr int kGeomRec = 16;
 constexpr int kLightRec = 24;
 // Staged surface record per geom: 16 floats (see stage_surface).
 constexpr int kSurfRec = 16;
+// Staged environment record per world: 4 floats (see stage_world_to).
+constexpr int kEnvRec = 4;
 // Leaf ids are padded to a multiple of 4 so the light records after them are 16-byte aligned.
 MJ_HOST_DEV inline int staged_leaf_slots(int nleaf) { return (nleaf + 3) & ~3; }
 
@@ -48,6 +50,7 @@ struct SceneView {
   const float* geoms;     // [ngeom][16] (LDS)
   const float* lights;    // [nlight][kLightRec] (LDS)
   const float* surfs;     // [ngeom][kSurfRec] (LDS)
+  const float* env;       // [kEnvRec]: cos(-yaw), sin(-yaw) of the world's sky rotation (LDS)
   int depth;
   // mesh BVHs (global)
   const float* blas_box;
