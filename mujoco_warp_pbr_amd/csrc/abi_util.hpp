@@ -32,6 +32,7 @@ inline int make_tile_map(const MjpbrScene* sc, TileMap* tm, int* nlaunch) {
   if (sc->geo.ndirect < 0 || sc->geo.ndirect > sc->geo.ngeom || (sc->geo.ndirect > 0 && !sc->geo.direct_geom))
     return set_error(MJPBR_EINVAL, "ndirect must be in [0, ngeom] with direct_geom set, got %d", sc->geo.ndirect);
   tm->nrender = c.nrender;
+  tm->mesh_cache_nodes = tm->mesh_cache_n = 0;
   int base = 0, rays = 0;
   for (int i = 0; i < c.nrender; ++i) {
     int w = c.cam_res[i][0], h = c.cam_res[i][1];

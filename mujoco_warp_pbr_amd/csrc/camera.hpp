@@ -17,6 +17,7 @@ struct TileMap {
   int tile_base[MJPBR_MAX_CAMS + 1];
   int tiles_x[MJPBR_MAX_CAMS];
   int ray_offset[MJPBR_MAX_CAMS];
+  int mesh_cache_nodes, mesh_cache_n;  // mesh-BVH top levels staged in LDS (render kernels)
 };
 
 struct Pixel {

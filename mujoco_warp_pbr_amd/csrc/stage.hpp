@@ -83,6 +83,9 @@ MJ_DEV SceneView scene_view(const MjpbrScene& sc, float* lds) {
   sv.surfs = sv.lights + sc.lit.nlight * kLightRec;
   sv.env = sv.surfs + g.ngeom * kSurfRec;
   sv.depth = g.tlas_depth;
+  sv.mesh_cache = nullptr;
+  sv.mesh_cache_nodes = 0;
+  sv.mesh_cache_n = 0;
   sv.blas_box = g.blas_box;
   sv.blas_tri = g.blas_tri;
   sv.blas_depth = g.blas_depth;
@@ -91,9 +94,24 @@ MJ_DEV SceneView scene_view(const MjpbrScene& sc, float* lds) {
   return sv;
 }
 
+// Copies the first min(nodes, 2^depth - 1) inner nodes of meshes 0 .. n - 1 into dst (mesh m at
+// dst + 12 * nodes * m), with the block's threads.
+MJ_DEV void stage_mesh_cache(const MjpbrGeometry& g, int nodes, int n, float* dst) {
+  const int per = 3 * nodes;  // float4 a mesh
+  for (int i = threadIdx.x; i < per * n; i += blockDim.x) {
+    const int m = i / per, k = i - m * per;
+    const int inner = (1 << g.blas_depth[m]) - 1;
+    if (k < 3 * inner)
+      reinterpret_cast<float4*>(dst)[i] = reinterpret_cast<const float4*>(g.blas_box + 12 * (int64_t)g.blas_node_adr[m])[k];
+  }
+}
+
 // World w's records into LDS: a float4 copy of the prepared records when geo.world_stage is set
-// (stage_prep_kernel ran first on the stream), else resolved here.
-MJ_DEV SceneView stage_world(const MjpbrScene& sc, int w, float* lds) {
+// (stage_prep_kernel ran first on the stream), else resolved here. With cache_n > 0 the top
+// levels (cache_nodes inner nodes) of the first cache_n mesh BVHs follow the records.
+MJ_DEV SceneView stage_world(const MjpbrScene& sc, int w, float* lds, int cache_nodes = 0, int cache_n = 0) {
+  float* cache = lds + stage_floats(sc);
+  if (cache_n > 0) stage_mesh_cache(sc.geo, cache_nodes, cache_n, cache);
   if (sc.geo.world_stage) {
     const int64_t n4 = stage_floats(sc) / 4;
     const float4* src = reinterpret_cast<const float4*>(sc.geo.world_stage) + (int64_t)w * n4;
@@ -103,7 +121,33 @@ MJ_DEV SceneView stage_world(const MjpbrScene& sc, int w, float* lds) {
     stage_world_to(sc, w, lds);
   }
   __syncthreads();
-  return scene_view(sc, lds);
+  SceneView sv = scene_view(sc, lds);
+  if (cache_n > 0) {
+    sv.mesh_cache = cache;
+    sv.mesh_cache_nodes = cache_nodes;
+    sv.mesh_cache_n = cache_n;
+  }
+  return sv;
+}
+
+// Mesh-BVH top levels an RT / PT workgroup caches in LDS: the largest 2^L - 1 inner nodes a mesh
+// (L <= 8) whose copies for min(nmesh, kMaxCachedMeshes) meshes fit in `budget_bytes`.
+constexpr int kMaxCachedMeshes = 16;
+MJ_HOST_DEV inline void mesh_cache_shape(const MjpbrScene& sc, int budget_bytes, int* nodes, int* n) {
+  *nodes = 0;
+  *n = sc.geo.nmesh < kMaxCachedMeshes ? sc.geo.nmesh : kMaxCachedMeshes;
+  if (*n <= 0) {
+    *n = 0;
+    return;
+  }
+  for (int L = 8; L >= 2; --L) {
+    const int k = (1 << L) - 1;
+    if ((int64_t)k * 48 * *n <= budget_bytes) {
+      *nodes = k;
+      return;
+    }
+  }
+  *n = 0;
 }
 
 }  // namespace mjpbr

@@ -52,6 +52,10 @@ struct SceneView {
   const float* surfs;     // [ngeom][kSurfRec] (LDS)
   const float* env;       // [kEnvRec]: cos(-yaw), sin(-yaw) of the world's sky rotation (LDS)
   int depth;
+  // top levels of the mesh BVHs copied into LDS: mesh m < mesh_cache_n has its first
+  // min(mesh_cache_nodes, 2^depth - 1) inner nodes at mesh_cache + 12 * mesh_cache_nodes * m
+  const float* mesh_cache;
+  int mesh_cache_nodes, mesh_cache_n;
   // mesh BVHs (global)
   const float* blas_box;
   const float* blas_tri;
@@ -62,15 +66,55 @@ struct SceneView {
 
 MJ_DEV float cut_of(float dist) { return dist * (1.0f + kRayTolRel) + kRayTolAbs; }
 
+typedef float NativeFloat4 __attribute__((ext_vector_type(4)));
+typedef __attribute__((address_space(3))) const NativeFloat4 LdsFloat4;
+typedef __attribute__((address_space(1))) const NativeFloat4 GlobalFloat4;
+
+MJ_DEV float4 to_float4(NativeFloat4 v) { return make_float4(v.x, v.y, v.z, v.w); }
+
+// Inner-node box storage of one BVH: load(i, q) fetches node i's 12 floats.
+struct PlainBoxes {
+  const float* p;
+  MJ_DEV void load(int i, float4 (&q)[3]) const {
+    const float4* b = reinterpret_cast<const float4*>(p) + 3 * i;
+    q[0] = b[0];
+    q[1] = b[1];
+    q[2] = b[2];
+  }
+};
+// A mesh BVH whose first `ncached` inner nodes (its top levels) are also in LDS: those are read
+// with LDS loads, the others with global loads (explicit address spaces, so neither becomes a
+// generic flat load).
+struct CachedBoxes {
+  const float* lds;
+  const float* glob;
+  int ncached;
+  MJ_DEV void load(int i, float4 (&q)[3]) const {
+    if (i < ncached) {
+      LdsFloat4* b = (LdsFloat4*)(lds) + 3 * i;
+      q[0] = to_float4(b[0]);
+      q[1] = to_float4(b[1]);
+      q[2] = to_float4(b[2]);
+    } else {
+      GlobalFloat4* b = (GlobalFloat4*)(glob) + 3 * i;
+      q[0] = to_float4(b[0]);
+      q[1] = to_float4(b[1]);
+      q[2] = to_float4(b[2]);
+    }
+  }
+};
+
 // Visits the leaves of an implicit BVH nearest-first. leaf(slot) returns true to stop.
-template <bool kRetest, class CutFn, class LeafFn>
-MJ_DEV void traverse(const float* boxes, int depth, V3 o, V3 rcp, CutFn cut, LeafFn leaf) {
+template <bool kRetest, class Boxes, class CutFn, class LeafFn>
+MJ_DEV void traverse(Boxes boxes, int depth, V3 o, V3 rcp, CutFn cut, LeafFn leaf) {
   const int ninner = (1 << depth) - 1;
   int node = 0, level = 0;
   uint32_t trail = 0;
   for (;;) {
     if (level < depth) {
-      const float* b = boxes + 12 * node;
+      float4 q[3];
+      boxes.load(node, q);
+      const float* b = reinterpret_cast<const float*>(q);
       float c = cut();
       float ta = ray_aabb(o, rcp, b, b + 3, c);
       float tb = ray_aabb(o, rcp, b + 6, b + 9, c);
@@ -92,7 +136,9 @@ MJ_DEV void traverse(const float* boxes, int depth, V3 o, V3 rcp, CutFn cut, Lea
       node = (anc1 ^ 1) - 1;
       level = L + 1;
       if (!kRetest) break;
-      const float* pb = boxes + 12 * ((node - 1) >> 1) + 6 * ((node - 1) & 1);
+      float4 q[3];
+      boxes.load((node - 1) >> 1, q);
+      const float* pb = reinterpret_cast<const float*>(q) + 6 * ((node - 1) & 1);
       if (ray_aabb(o, rcp, pb, pb + 3, cut()) >= 0.0f) break;
     }
   }
@@ -101,9 +147,16 @@ MJ_DEV void traverse(const float* boxes, int depth, V3 o, V3 rcp, CutFn cut, Lea
 MJ_DEV V3 safe_rcp(V3 d) { return V3{1.0f / d.x, 1.0f / d.y, 1.0f / d.z}; }
 
 // Closest triangle of mesh `mesh` (mesh-frame ray) with t < max_t; ties -> lower face id.
+// Box storage of mesh `mesh`: its top levels from LDS when cached.
+MJ_DEV CachedBoxes mesh_boxes(const SceneView& sv, int mesh) {
+  const float* glob = sv.blas_box + 12 * (int64_t)sv.blas_node_adr[mesh];
+  if (mesh < sv.mesh_cache_n) return CachedBoxes{sv.mesh_cache + 12 * sv.mesh_cache_nodes * mesh, glob, sv.mesh_cache_nodes};
+  return CachedBoxes{glob, glob, 0};
+}
+
 MJ_DEV bool mesh_closest(const SceneView& sv, int mesh, V3 lo, V3 ld, float max_t, float* t_out,
                          int* face_out, float* b1_out, float* b2_out, V3* e1_out, V3* e2_out) {
-  const float* boxes = sv.blas_box + 12 * (int64_t)sv.blas_node_adr[mesh];
+  const CachedBoxes boxes = mesh_boxes(sv, mesh);
   const float* tris = sv.blas_tri + 12 * (int64_t)sv.blas_tri_adr[mesh];
   int depth = sv.blas_depth[mesh];
   float best = INFINITY;
@@ -146,7 +199,7 @@ MJ_DEV bool mesh_closest(const SceneView& sv, int mesh, V3 lo, V3 ld, float max_
 }
 
 MJ_DEV bool mesh_any(const SceneView& sv, int mesh, V3 lo, V3 ld, float max_t) {
-  const float* boxes = sv.blas_box + 12 * (int64_t)sv.blas_node_adr[mesh];
+  const CachedBoxes boxes = mesh_boxes(sv, mesh);
   const float* tris = sv.blas_tri + 12 * (int64_t)sv.blas_tri_adr[mesh];
   int depth = sv.blas_depth[mesh];
   bool hit = false;
@@ -260,7 +313,7 @@ MJ_DEV Hit trace_closest(const SceneView& sv, V3 o, V3 d, float max_dist, bool c
       };
   // unbounded planes first (their hit, if any, becomes the cut of the BVH traversal)
   for (int k = 0; k < sv.ndirect; ++k) visit(sv.direct[k], std::true_type{});
-  traverse<true>(sv.tlas_box, sv.depth, o, rcp, [&]() { return cut_of(h.dist); },
+  traverse<true>(PlainBoxes{sv.tlas_box}, sv.depth, o, rcp, [&]() { return cut_of(h.dist); },
                  [&](int slot) { return visit(sv.tlas_leaf[slot], std::false_type{}); });
   return h;
 }
@@ -288,7 +341,7 @@ MJ_DEV bool trace_any(const SceneView& sv, V3 o, V3 d, float max_dist, bool cull
       };
   for (int k = 0; k < sv.ndirect; ++k)
     if (visit(sv.direct[k], std::true_type{})) return true;
-  traverse<false>(sv.tlas_box, sv.depth, o, rcp, [&]() { return cut; },
+  traverse<false>(PlainBoxes{sv.tlas_box}, sv.depth, o, rcp, [&]() { return cut; },
                   [&](int slot) { return visit(sv.tlas_leaf[slot], std::false_type{}); });
   return hit;
 }
