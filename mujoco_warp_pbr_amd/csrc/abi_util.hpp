@@ -19,6 +19,18 @@ inline int check_launch(const char* what) {
 
 inline size_t host_stage_bytes(const MjpbrScene* sc) { return (size_t)stage_floats(*sc) * 4; }
 
+// LDS a render workgroup may use without lowering occupancy below 8 workgroups a compute unit.
+constexpr size_t kRenderLdsPerGroup = 20 * 1024;
+
+// Sets tm's mesh-BVH cache (the top levels the render kernels stage in LDS after the world's
+// records) for a scene whose records take `stage_bytes`; returns the bytes the cache adds.
+inline size_t plan_mesh_cache(const MjpbrScene* sc, bool meshes, size_t stage_bytes, TileMap* tm) {
+  tm->mesh_cache_nodes = tm->mesh_cache_n = 0;
+  if (!meshes || stage_bytes >= kRenderLdsPerGroup) return 0;
+  mesh_cache_shape(*sc, (int)(kRenderLdsPerGroup - stage_bytes), &tm->mesh_cache_nodes, &tm->mesh_cache_n);
+  return (size_t)tm->mesh_cache_nodes * tm->mesh_cache_n * 48;
+}
+
 // Enqueues the per-world record resolution into sc->geo.world_stage (no-op when it is NULL);
 // the render kernels launched after it on the stream copy the records into LDS.
 int launch_stage_prep(const MjpbrScene* sc, hipStream_t stream);

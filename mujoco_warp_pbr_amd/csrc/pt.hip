@@ -28,7 +28,7 @@ pt_kernel(MjpbrScene sc, MjpbrSettings s, MjpbrOutputs out, int seed, TileMap tm
   int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   int px = tx * kTile + (wave & 1) * 8 + (lane & 7);
   int py = ty * kTile + (wave >> 1) * 8 + (lane >> 3);
-  SceneView sv = stage_world(sc, w, lds);
+  SceneView sv = stage_world(sc, w, lds, tm.mesh_cache_nodes, tm.mesh_cache_n);
   Pixel pix;
   if (!setup_pixel(sc.cams, out, s, w, cidx, px, py, tm.ray_offset[cidx], &pix)) return;
   float exposure = sc.env.exposure[w % sc.env.exposure_n];
@@ -110,6 +110,7 @@ extern "C" int mjpbr_pt_render(const MjpbrScene* scene, const MjpbrSettings* s, 
     return set_error(MJPBR_EINVAL, "max_depth + 4 * max_layers must be below %d", kJitterDepth);
   size_t lds = host_stage_bytes(scene);
   if (lds > 64 * 1024) return set_error(MJPBR_EINVAL, "scene too large for LDS staging (%d geoms, %d lights)", scene->geo.ngeom, scene->lit.nlight);
+  lds += plan_mesh_cache(scene, s->has_meshes != 0, lds, &tm);
   if ((rc = launch_stage_prep(scene, (hipStream_t)stream))) return rc;
   hipLaunchKernelGGL(pt_kernel, dim3(nlaunch), dim3(256), lds, (hipStream_t)stream, *scene, *s, *out, seed, tm);
   return check_launch("pt_kernel");

@@ -186,9 +186,6 @@ const RtInstance kRtInstances[] = {
 };
 #undef MJPBR_RT_INSTANCE
 
-// LDS a render workgroup may use without lowering occupancy below 8 workgroups a compute unit.
-constexpr size_t kRtLdsPerGroup = 20 * 1024;
-
 unsigned required_features(const MjpbrSettings& s) {
   unsigned f = 0;
   if (s.has_meshes) f |= kFMesh;
@@ -225,11 +222,8 @@ extern "C" int mjpbr_rt_render(const MjpbrScene* scene, const MjpbrSettings* s, 
   size_t lds = host_stage_bytes(scene);
   if (lds > 64 * 1024) return set_error(MJPBR_EINVAL, "scene too large for LDS staging (%d geoms, %d lights)", scene->geo.ngeom, scene->lit.nlight);
   const RtInstance& k = pick_instance(required_features(*s));
-  // mesh-BVH top levels in the LDS left to a workgroup at 8 per compute unit (20 KB each)
-  tm.mesh_cache_nodes = tm.mesh_cache_n = 0;
-  if ((k.mask & kFMesh) && s->has_meshes && lds < kRtLdsPerGroup)
-    mesh_cache_shape(*scene, (int)(kRtLdsPerGroup - lds), &tm.mesh_cache_nodes, &tm.mesh_cache_n);
-  lds += (size_t)tm.mesh_cache_nodes * tm.mesh_cache_n * 48;
+  // mesh-BVH top levels in the LDS left to a workgroup at 8 per compute unit
+  lds += plan_mesh_cache(scene, (k.mask & kFMesh) && s->has_meshes, lds, &tm);
   if ((rc = launch_stage_prep(scene, (hipStream_t)stream))) return rc;
   hipLaunchKernelGGL(k.fn, dim3(nlaunch), dim3(256), lds, (hipStream_t)stream, *scene, *s, *out, seed, tm);
   return check_launch("rt_kernel");

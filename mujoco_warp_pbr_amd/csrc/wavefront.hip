@@ -86,7 +86,7 @@ wf_kernel(const WfHeader* __restrict__ hdr, int* counter, float* slots, MjpbrSet
     while (cidx + 1 < G.tm.nrender && t >= G.tm.tile_base[cidx + 1]) ++cidx;
     const int tl = t - G.tm.tile_base[cidx];
     const int tx = tl % G.tm.tiles_x[cidx], ty = tl / G.tm.tiles_x[cidx];
-    SceneView sv = stage_world(sc, w, lds);
+    SceneView sv = stage_world(sc, w, lds, G.tm.mesh_cache_nodes, G.tm.mesh_cache_n);
     const float exposure = sc.env.exposure[w % sc.env.exposure_n];
     PtCtx c{sc, s, sv, w, exposure};
     const int total_rays = sc.cams.total_rays;
@@ -246,6 +246,7 @@ extern "C" int mjpbr_pt_wavefront_prepare(const MjpbrScene* scenes, const MjpbrO
       delete h;
       return set_error(MJPBR_EINVAL, "wavefront: group %d scene too large for LDS staging (%d geoms)", g, scenes[g].geo.ngeom);
     }
+    b += plan_mesh_cache(&scenes[g], s->has_meshes != 0, b, &G.tm);
     lds = b > lds ? b : lds;
     G.item_base = items;
     G.pixel_base = pixels;
