@@ -228,4 +228,26 @@ MJ_DEV float ray_aabb(V3 o, V3 rcp, const float* lo, const float* hi, float max_
   return (tmax >= entry && entry < max_t) ? entry : -1.0f;
 }
 
+// ray_aabb with the slab distances as fma(bound, rcp, -o * rcp): one instruction a slab plane
+// instead of two. The caller passes rcp clamped to +-kSlabRcpMax (an infinite reciprocal of a
+// zero direction component would make o * rcp - bound * rcp an inf - inf NaN) and orc = o * rcp,
+// once per ray. The rounding differs from (bound - o) * rcp by about an ulp of o * rcp, far inside
+// the boxes' padding (1e-5 of the extent plus 1e-6 of the coordinates); a clamped reciprocal keeps
+// every slab distance's sign and puts a slab the ray does not cross at least 1e30 * (distance to
+// the slab) away, beyond every cut.
+constexpr float kSlabRcpMax = 1.0e30f;
+MJ_DEV V3 slab_rcp(V3 rcp) {
+  return V3{fminf(fmaxf(rcp.x, -kSlabRcpMax), kSlabRcpMax), fminf(fmaxf(rcp.y, -kSlabRcpMax), kSlabRcpMax),
+            fminf(fmaxf(rcp.z, -kSlabRcpMax), kSlabRcpMax)};
+}
+MJ_DEV float ray_aabb_fma(V3 orc, V3 rcp, const float* lo, const float* hi, float max_t) {
+  float t1x = fmaf(lo[0], rcp.x, -orc.x), t2x = fmaf(hi[0], rcp.x, -orc.x);
+  float t1y = fmaf(lo[1], rcp.y, -orc.y), t2y = fmaf(hi[1], rcp.y, -orc.y);
+  float t1z = fmaf(lo[2], rcp.z, -orc.z), t2z = fmaf(hi[2], rcp.z, -orc.z);
+  float tmin = fmaxf(fmaxf(fminf(t1x, t2x), fminf(t1y, t2y)), fminf(t1z, t2z));
+  float tmax = fminf(fminf(fmaxf(t1x, t2x), fmaxf(t1y, t2y)), fmaxf(t1z, t2z));
+  float entry = fmaxf(tmin, 0.0f);
+  return (tmax >= entry && entry < max_t) ? entry : -1.0f;
+}
+
 }  // namespace mjpbr

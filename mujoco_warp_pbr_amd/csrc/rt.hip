@@ -131,7 +131,7 @@ rt_kernel(MjpbrScene sc, MjpbrSettings s, MjpbrOutputs out, int seed, TileMap tm
     V3 o, d, dc;
     float md;
     primary_ray(pix, s, 1, 0, 0, &o, &d, &md, &dc);
-    Hit hit = trace_closest<(F & kFMesh) != 0>(sv, o, d, md, cull);
+    Hit hit = trace_closest<(F & kFMesh) != 0, F == 0u>(sv, o, d, md, cull);
     write_geometry(out, w, pix, hit, dc);
     return;
   }

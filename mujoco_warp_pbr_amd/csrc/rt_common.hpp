@@ -50,7 +50,7 @@ MJ_DEV float light_visibility(const RtCtx& c, const Light& L, V3 point, V3 norma
   const bool directional = L.type == MJPBR_LIGHT_DIRECTIONAL;
   const bool soft = (F & kFSoft) && !directional && s.shadow_samples > 1;
   if (!soft) {
-    const bool hit = trace_any<kMesh>(c.sv, origin, l, directional ? kUnboundedShadow : dist - kShadowGap, cull);
+    const bool hit = trace_any<kMesh, F == 0u>(c.sv, origin, l, directional ? kUnboundedShadow : dist - kShadowGap, cull);
     return hit ? s.shadow_light_fraction : 1.0f;
   }
   const int n = s.shadow_samples;

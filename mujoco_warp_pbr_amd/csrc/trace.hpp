@@ -105,9 +105,16 @@ struct CachedBoxes {
 };
 
 // Visits the leaves of an implicit BVH nearest-first. leaf(slot) returns true to stop.
-template <bool kRetest, class Boxes, class CutFn, class LeafFn>
+// kFmaSlabs: box tests as ray_aabb_fma (faster in the leanest real-time instance; slower where its
+// six extra live registers spill).
+template <bool kRetest, bool kFmaSlabs = false, class Boxes, class CutFn, class LeafFn>
 MJ_DEV void traverse(Boxes boxes, int depth, V3 o, V3 rcp, CutFn cut, LeafFn leaf) {
   const int ninner = (1 << depth) - 1;
+  const V3 rc = slab_rcp(rcp);
+  const V3 orc = V3{o.x * rc.x, o.y * rc.y, o.z * rc.z};
+  auto slab = [&](const float* lo, const float* hi, float max_t) {
+    return kFmaSlabs ? ray_aabb_fma(orc, rc, lo, hi, max_t) : ray_aabb(o, rcp, lo, hi, max_t);
+  };
   int node = 0, level = 0;
   uint32_t trail = 0;
   for (;;) {
@@ -116,8 +123,8 @@ MJ_DEV void traverse(Boxes boxes, int depth, V3 o, V3 rcp, CutFn cut, LeafFn lea
       boxes.load(node, q);
       const float* b = reinterpret_cast<const float*>(q);
       float c = cut();
-      float ta = ray_aabb(o, rcp, b, b + 3, c);
-      float tb = ray_aabb(o, rcp, b + 6, b + 9, c);
+      float ta = slab(b, b + 3, c);
+      float tb = slab(b + 6, b + 9, c);
       if (ta >= 0.0f || tb >= 0.0f) {
         int nearside = (tb >= 0.0f && (ta < 0.0f || tb < ta)) ? 1 : 0;
         if (ta >= 0.0f && tb >= 0.0f) trail |= 1u << level;
@@ -139,7 +146,7 @@ MJ_DEV void traverse(Boxes boxes, int depth, V3 o, V3 rcp, CutFn cut, LeafFn lea
       float4 q[3];
       boxes.load((node - 1) >> 1, q);
       const float* pb = reinterpret_cast<const float*>(q) + 6 * ((node - 1) & 1);
-      if (ray_aabb(o, rcp, pb, pb + 3, cut()) >= 0.0f) break;
+      if (slab(pb, pb + 3, cut()) >= 0.0f) break;
     }
   }
 }
@@ -275,7 +282,8 @@ MJ_DEV float intersect_geom(const SceneView& sv, const float* g, V3 o, V3 d, flo
   }
 }
 
-template <bool kMesh = true>
+// kFmaSlabs: the scene BVH's box tests as ray_aabb_fma (instances pick it where it measured faster).
+template <bool kMesh = true, bool kFmaSlabs = false>
 MJ_DEV Hit trace_closest(const SceneView& sv, V3 o, V3 d, float max_dist, bool cull) {
   Hit h;
   h.geom_id = -1;
@@ -313,12 +321,12 @@ MJ_DEV Hit trace_closest(const SceneView& sv, V3 o, V3 d, float max_dist, bool c
       };
   // unbounded planes first (their hit, if any, becomes the cut of the BVH traversal)
   for (int k = 0; k < sv.ndirect; ++k) visit(sv.direct[k], std::true_type{});
-  traverse<true>(PlainBoxes{sv.tlas_box}, sv.depth, o, rcp, [&]() { return cut_of(h.dist); },
+  traverse<true, kFmaSlabs>(PlainBoxes{sv.tlas_box}, sv.depth, o, rcp, [&]() { return cut_of(h.dist); },
                  [&](int slot) { return visit(sv.tlas_leaf[slot], std::false_type{}); });
   return h;
 }
 
-template <bool kMesh = true>
+template <bool kMesh = true, bool kFmaSlabs = false>
 MJ_DEV bool trace_any(const SceneView& sv, V3 o, V3 d, float max_dist, bool cull) {
   V3 rcp = safe_rcp(d);
   bool hit = false;
@@ -341,7 +349,7 @@ MJ_DEV bool trace_any(const SceneView& sv, V3 o, V3 d, float max_dist, bool cull
       };
   for (int k = 0; k < sv.ndirect; ++k)
     if (visit(sv.direct[k], std::true_type{})) return true;
-  traverse<false>(PlainBoxes{sv.tlas_box}, sv.depth, o, rcp, [&]() { return cut; },
+  traverse<false, kFmaSlabs>(PlainBoxes{sv.tlas_box}, sv.depth, o, rcp, [&]() { return cut; },
                   [&](int slot) { return visit(sv.tlas_leaf[slot], std::false_type{}); });
   return hit;
 }
