@@ -19,8 +19,9 @@ RES = 64
 PATH = {"tier": mj.Tier.PATH, "render_hdr": True, "ev100": EV100_UNIT}
 
 
-def _scene():
-  """tests/test_denoise.py:24-40 restated: checkered rough floor, emissive sphere, random sky."""
+def _scene(second_camera=False):
+  """tests/test_denoise.py:24-40 restated: checkered rough floor, emissive sphere, random sky.
+  second_camera: a second, side-on camera (for per-camera image layouts)."""
   from mujoco_warp_pbr_amd.scenes import procedural_textures
 
   b = _Builder()
@@ -33,6 +34,8 @@ def _scene():
   b.geom(mdl.GEOM_PLANE, (2, 2, 0.1), (0, 0, 0), matid=floor)
   b.geom(mdl.GEOM_SPHERE, (0.3, 0, 0), (0, 0, 0.3), matid=glow)
   b.camera((0, -2.5, 1.5), (0, 0, 0.2), 45.0, (RES, RES))
+  if second_camera:
+    b.camera((2.2, -1.0, 1.0), (0, 0, 0.3), 50.0, (RES, RES))
   return b.build()
 
 
@@ -70,9 +73,14 @@ def test_reduces_error_and_keeps_energy():
 
 
 @pytest.mark.gpu
-def test_gpu_atrous_matches_oracle_on_same_inputs(cuda):
-  m, d, rc = make(_scene(), nworld=2, device="cuda", cam_res=(RES, RES), envmap=_env(), envmap_intensity=1.0, path_samples=4,
-                  denoise=True, **PATH)
+@pytest.mark.parametrize("layout", ["one_camera", "two_ragged_cameras"])
+def test_gpu_atrous_matches_oracle_on_same_inputs(cuda, layout):
+  """Same noisy inputs through the GPU filter and the oracle. two_ragged_cameras: two images per
+  world whose sizes are not multiples of the 16-pixel tiles of the small-step passes (37 x 21 and
+  20 x 45), so partial tiles, halos past the image edge and the second camera's offset are covered."""
+  two = layout == "two_ragged_cameras"
+  m, d, rc = make(_scene(two), nworld=2, device="cuda", cam_res=[(37, 21), (20, 45)] if two else (RES, RES), envmap=_env(),
+                  envmap_intensity=1.0, path_samples=4, denoise=True, **PATH)
   rc.render_hdr = False  # keep the noisy HDR in hdr_data as the filter's input
   mj.refit_bvh(m, d, rc)
   mj.render(m, d, rc, seed=0)
