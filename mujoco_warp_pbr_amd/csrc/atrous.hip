@@ -9,9 +9,10 @@
 // per-world images are small enough that the 5x5 neighbourhood reads are served by L2. The
 // demodulation packs each pixel's guides (normal and albedo as their f16 bits, depth) into one
 // 16-byte record, so a tap reads its guides with one load beside its colour. The edge weight's
-// power and exponential use the hardware exp2 / log2 (exp2(s log2 x), exp2(-d log2 e)); they
-// differ from correctly rounded powf / expf by a few ulp of the weight, far inside the 1e-4
-// relative agreement the filtered image is held to.
+// power, exponential and albedo distance use the hardware v_log_f32 / v_exp_f32 / v_sqrt_f32
+// (exp2(s log2 x), exp2(-d log2 e), sqrt without the correctly rounded fix-up); they differ from
+// correctly rounded powf / expf / sqrtf by a few ulp of the weight, far inside the 1e-4 relative
+// agreement the filtered image is held to.
 #pragma clang fp contract(off)
 #include "abi_util.hpp"
 
@@ -122,6 +123,7 @@ __global__ void atrous_kernel(MjpbrAtrousParams p, MjpbrOutputs out, int step, c
   V3 a0 = guide_albedo(g0);
   float inv_sigma_a = 1.0f / p.sigma_albedo;
   float inv_sigma_z = 1.0f / fmaxf(p.sigma_depth * z0 * (float)step, 1.0e-6f);
+  const float wn_zero = powf(0.0f, p.sigma_normal);  // the normal weight of a tap facing away
   V3 total = v3s(0.0f);
   float total_var = 0.0f, wsum = 0.0f;
   for (int j = 0; j < 5; ++j) {
@@ -135,9 +137,9 @@ __global__ void atrous_kernel(MjpbrAtrousParams p, MjpbrOutputs out, int step, c
       const uint4 gq = gw[q];
       V3 c = V3{s.x, s.y, s.z};
       const float nd = fmaxf(dot(n0, guide_normal(gq)), 0.0f);
-      float wn = nd > 0.0f ? __powf(nd, p.sigma_normal) : powf(0.0f, p.sigma_normal);
+      float wn = nd > 0.0f ? __builtin_amdgcn_exp2f(p.sigma_normal * __builtin_amdgcn_logf(nd)) : wn_zero;
       float dist = fabsf(guide_depth(gq) - z0) * inv_sigma_z + fabsf(luminance(c) - l0) * inv_sigma_l +
-                   len(guide_albedo(gq) - a0) * inv_sigma_a;
+                   __builtin_amdgcn_sqrtf(len_sq(guide_albedo(gq) - a0)) * inv_sigma_a;
       float weight = b3(k) * b3(j) * wn * __expf(-dist);
       total += c * weight;
       total_var += weight * weight * s.w;
