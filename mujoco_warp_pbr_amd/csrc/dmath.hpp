@@ -56,6 +56,11 @@ MJ_DEV V3 normalize(V3 a) {
 MJ_DEV V3 vmax(V3 a, float s) { return V3{fmaxf(a.x, s), fmaxf(a.y, s), fmaxf(a.z, s)}; }
 MJ_DEV V3 vmin(V3 a, float s) { return V3{fminf(a.x, s), fminf(a.y, s), fminf(a.z, s)}; }
 MJ_DEV float clampf(float x, float lo, float hi) { return fminf(fmaxf(x, lo), hi); }
+// log2 and x^y (x > 0) on the hardware v_log_f32 / v_exp_f32: a few ulp from the correctly
+// rounded log2f / powf, whose full sequences cost some 40 instructions each. For shading and
+// display values only (texture and sky LODs, specular occlusion, the sRGB curve), never for hits.
+MJ_DEV float log2_hw(float x) { return __builtin_amdgcn_logf(x); }
+MJ_DEV float pow_hw(float x, float y) { return __builtin_amdgcn_exp2f(y * __builtin_amdgcn_logf(x)); }
 
 // Row-major 3x3: m[3 i + j].
 MJ_DEV V3 mat_mul(const float* m, V3 v) {

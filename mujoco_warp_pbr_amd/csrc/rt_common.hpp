@@ -132,7 +132,7 @@ MJ_DEV V3 ambient_radiance(const RtCtx& c, const Surface& sf, V3 normal, V3 view
     prefiltered = ambient + sample_specular(env, e, dominant, yaw, lb.perceptual_roughness) * scale;
   }
   float ao = sf.occlusion;
-  float spec_ao = clampf(powf(nov + ao, exp2f(-16.0f * alpha - 1.0f)) - 1.0f + ao, 0.0f, 1.0f);
+  float spec_ao = clampf(pow_hw(nov + ao, exp2f(-16.0f * alpha - 1.0f)) - 1.0f + ao, 0.0f, 1.0f);
   *lobes = lb;
   *nov_out = nov;
   return cw(irradiance, lb.diffuse_color) * ao + cw(prefiltered, cw(lb.spec_albedo, lb.energy_comp)) * spec_ao;
@@ -170,7 +170,7 @@ MJ_DEV V3 background(const RtCtx& c, V3 dir, float spread) {
     int e;
     float yaw, scale;
     env_lookup(env, c.w, &e, &yaw, &scale);
-    float lod = log2f(fmaxf(spread * c.s.inv_texel_angle, kMinTexels));
+    float lod = log2_hw(fmaxf(spread * c.s.inv_texel_angle, kMinTexels));
     return sample_radiance(env, e, dir, yaw, lod) * scale;
   }
   return ld3(env.background) / c.exposure;

@@ -115,7 +115,7 @@ MJ_DEV float4 sample_texture(const MjpbrMaterials& m, int tex, V2 uv, float lod)
 MJ_DEV float texture_lod(const MjpbrMaterials& m, int tex, float footprint, float uv_per_length) {
   float size = (float)max(m.tex_desc[4 * tex], m.tex_desc[4 * tex + 1]);
   float texels = footprint * uv_per_length * size;
-  return log2f(fmaxf(texels, 1.0e-8f));
+  return log2_hw(fmaxf(texels, 1.0e-8f));
 }
 
 // ---------------------------------------------------------------- environment
@@ -408,7 +408,7 @@ MJ_DEV V3 display_transform(V3 c, int tone_mapper) {
   return vmin(vmax(c, 0.0f), 1.0f);
 }
 
-MJ_DEV float srgb_oetf(float x) { return x <= 0.0031308f ? 12.92f * x : 1.055f * powf(x, 1.0f / 2.4f) - 0.055f; }
+MJ_DEV float srgb_oetf(float x) { return x <= 0.0031308f ? 12.92f * x : 1.055f * pow_hw(x, 1.0f / 2.4f) - 0.055f; }
 
 MJ_DEV uint32_t encode_display(V3 c) {
   uint32_t r = (uint32_t)(srgb_oetf(c.x) * 255.0f + 0.5f);
