@@ -249,7 +249,7 @@ typedef struct MjpbrOutputs {
   int32_t depth_adr[MJPBR_MAX_CAMS];
   int32_t seg_adr[MJPBR_MAX_CAMS];
   int32_t rgb_stride, depth_stride, seg_stride, hdr_stride, guide_stride, aov_stride;
-  uint32_t* rgb;     /* [nworld, rgb_stride] packed sRGB8 R | G<<8 | B<<16 | 255<<24 */
+  uint32_t* rgb;     /* [nworld, rgb_stride] packed sRGB8 0xAARRGGBB: B | G<<8 | R<<16 | 255<<24 */
   float* depth;      /* [nworld, depth_stride] */
   int32_t* seg;      /* [nworld, seg_stride, 2] (id, object type) */
   float* hdr;        /* [nworld, hdr_stride, 3] linear radiance (optional) */

@@ -421,7 +421,7 @@ MJ_DEV uint32_t encode_display(V3 c) {
   uint32_t r = (uint32_t)(srgb_oetf(c.x) * 255.0f + 0.5f);
   uint32_t g = (uint32_t)(srgb_oetf(c.y) * 255.0f + 0.5f);
   uint32_t b = (uint32_t)(srgb_oetf(c.z) * 255.0f + 0.5f);
-  return r | (g << 8) | (b << 16) | (255u << 24);
+  return (255u << 24) | (r << 16) | (g << 8) | b;  // MuJoCo Warp's 0xAARRGGBB
 }
 
 // ---------------------------------------------------------------- surfaces (material.py)

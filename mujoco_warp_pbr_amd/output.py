@@ -36,19 +36,22 @@ def _check_out(out: torch.Tensor, rc: PbrRenderContext, h: int, w: int, tail: tu
 
 
 def get_rgb(rc: PbrRenderContext, camera_index: int, rgb_out: torch.Tensor) -> None:
-  """Display RGB in [0, 1] of one camera, (n, h, w, 3) float32 (packed sRGB8 / 255)."""
+  """Display RGB in [0, 1] of one camera, (n, h, w, 3) float32: the packed sRGB8 bytes / 255
+  (rgb_data holds 0xAARRGGBB words, MuJoCo Warp's layout)."""
   adr, w, h = _camera(rc, camera_index, "rgb_adr", "rgb")
   n = _check_out(rgb_out, rc, h, w, (3,), torch.float32, "rgb_out")
   bits = rc.base.rgb_data[:n, adr : adr + w * h]
-  for c in range(3):
-    rgb_out[..., c] = ((bits >> (8 * c)) & 0xFF).reshape(n, h, w).to(torch.float32) / 255.0
+  for c, shift in enumerate((16, 8, 0)):
+    rgb_out[..., c] = ((bits >> shift) & 0xFF).reshape(n, h, w).to(torch.float32) / 255.0
 
 
 def get_depth(rc: PbrRenderContext, camera_index: int, depth_scale: float, depth_out: torch.Tensor) -> None:
-  """Planar depth times depth_scale of one camera, (n, h, w) float32."""
+  """Planar depth / depth_scale clamped to [0, 1] of one camera, (n, h, w) float32 (reference
+  output.py:41-45; background pixels hold depth 0)."""
   adr, w, h = _camera(rc, camera_index, "depth_adr", "depth")
   n = _check_out(depth_out, rc, h, w, (), torch.float32, "depth_out")
-  depth_out.copy_(rc.base.depth_data[:n, adr : adr + w * h].reshape(n, h, w) * depth_scale)
+  depth = rc.base.depth_data[:n, adr : adr + w * h].reshape(n, h, w)
+  torch.clamp(depth / float(depth_scale), 0.0, 1.0, out=depth_out)
 
 
 def get_segmentation(rc: PbrRenderContext, camera_index: int, segmentation_out: torch.Tensor) -> None:

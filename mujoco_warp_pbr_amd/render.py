@@ -40,9 +40,13 @@ def _check(t: torch.Tensor, dtype, name: str):
 
 
 def geometry_struct(m, d, rc: PbrRenderContext):
+  """The MjpbrGeometry of the current frame. Every entry point builds it, so the check that the
+  data's world count matches the context's buffers (sized for rc.base.nworld) sits here."""
+  base = rc.base
+  if d.nworld != base.nworld:
+    raise ValueError(f"data has {d.nworld} worlds, the context {base.nworld}")
   S = _abi.structs()
   g = S["MjpbrGeometry"]()
-  base = rc.base
   g.nworld = d.nworld
   g.ngeom = m.ngeom
   g.nmesh = m.nmesh
@@ -324,10 +328,10 @@ def render(m, d, rc: PbrRenderContext, seed: int = 0) -> None:
 
   Call `refit_bvh(m, d, rc)` first whenever geometry moved.
   """
-  check_env_id(rc)
-  warn_ignored_overrides(m, rc)
   if d.nworld != rc.base.nworld:
     raise ValueError(f"data has {d.nworld} worlds, the context {rc.base.nworld}")
+  check_env_id(rc)
+  warn_ignored_overrides(m, rc)
   sc = scene_struct(m, d, rc)
   s = settings_struct(rc)
   out = outputs_struct(rc)

@@ -174,11 +174,11 @@ def train(net: learned.UNet, X: torch.Tensor, T: torch.Tensor, steps: int, batch
 
 def _rgb(t: torch.Tensor) -> torch.Tensor:
   t = t.to(torch.int64)
-  return torch.stack([(t >> s) & 255 for s in (0, 8, 16)], -1).to(torch.float32) / 255.0
+  return torch.stack([(t >> s) & 255 for s in (16, 8, 0)], -1).to(torch.float32) / 255.0
 
 
 def psnr(a: torch.Tensor, b: torch.Tensor) -> float:
-  """PSNR in dB of two packed sRGB8 images (int32 bits R | G << 8 | B << 16), over RGB."""
+  """PSNR in dB of two packed sRGB8 images (int32 bits 0xAARRGGBB), over RGB."""
   mse = torch.mean((_rgb(a).double() - _rgb(b).double()) ** 2).item()
   return 10.0 * math.log10(1.0 / max(mse, 1e-12))
 

@@ -68,7 +68,7 @@ class RenderBase:
     rgb_adr / depth_adr / seg_adr: first pixel of each camera in the flat per-world buffers
       (-1 when the camera does not render that output)                (nrender,)
     total_rays: pixels per world over all rendering cameras.
-    rgb_data: packed sRGB8 (R | G << 8 | B << 16 | 255 << 24)          (nworld, nrgb) uint32 bits in int32
+    rgb_data: packed sRGB8 0xAARRGGBB (B | G << 8 | R << 16 | 255 << 24) (nworld, nrgb) uint32 bits in int32
     depth_data: planar depth                                          (nworld, ndepth)
     seg_data: (geom id, object type), (-1, -1) for the background     (nworld, nseg, 2)
     znear / zfar: clip distances (zfar <= 0: unbounded).

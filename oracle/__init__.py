@@ -50,6 +50,22 @@ def library() -> ctypes.CDLL:
       for name in ("oracle_pt_render", "oracle_atrous"):
         if hasattr(lib, name):
           getattr(lib, name).restype = ctypes.c_int
+      f = ctypes.c_float
+      lib.oracle_ray_primitive.argtypes = [ctypes.c_int] + [vp] * 6
+      lib.oracle_ray_primitive.restype = f
+      lib.oracle_ray_triangle.argtypes = [vp] * 5 + [f, vp]
+      lib.oracle_ray_triangle.restype = ctypes.c_int
+      lib.oracle_display.argtypes = [vp, ctypes.c_int, vp]
+      lib.oracle_encode.argtypes = [vp]
+      lib.oracle_encode.restype = ctypes.c_uint32
+      lib.oracle_spec_ao.argtypes = [f, f, f]
+      lib.oracle_spec_ao.restype = f
+      lib.oracle_eval_light.argtypes = [ctypes.c_int] * 3 + [vp, vp, f, f, vp, f, f, vp, vp]
+      lib.oracle_atrous_pass.argtypes = [vp, vp, vp, vp, ctypes.c_int]
+      lib.oracle_spec_lobe.argtypes = [vp, f, f, f, f, f, vp]
+      lib.oracle_dfg_lookup.argtypes = [vp, ctypes.c_int, f, f, vp]
+      lib.oracle_lobes.argtypes = [vp, f, f, f, f, vp, ctypes.c_int, vp]
+      lib.oracle_atrous_pass.restype = ctypes.c_int
       _lib = lib
     return _lib
 
@@ -242,3 +258,106 @@ def prefilter_texel(env, e: int, level: int, y: int, x: int, nsample: int) -> np
   out = np.zeros(3, np.float32)
   library().oracle_prefilter_texel(rad.data_ptr(), adr.data_ptr(), desc.data_ptr(), L, H, W, nsample, level, y, x, out.ctypes.data)
   return out
+
+
+# ----------------------------------------------------------------------------- scalar building blocks
+def _f32(x, n: int | None = None) -> np.ndarray:
+  a = np.ascontiguousarray(np.asarray(x, np.float32).reshape(-1))
+  if n is not None:
+    assert a.size == n, a.size
+  return a
+
+
+def ray_primitive(geom_type: int, pos, mat, size, origin, direction) -> tuple[float, np.ndarray]:
+  """Distance (-1: miss) and world normal of one ray against one MuJoCo primitive."""
+  n = np.zeros(3, np.float32)
+  args = [_f32(pos, 3), _f32(mat, 9), _f32(size, 3), _f32(origin, 3), _f32(direction, 3)]
+  x = library().oracle_ray_primitive(int(geom_type), *(a.ctypes.data for a in args), n.ctypes.data)
+  return float(x), n
+
+
+def ray_triangle(origin, direction, v0, v1, v2, tmax: float) -> tuple[bool, float, float, float]:
+  """(hit, t, b1, b2) of the oracle's ray-triangle test (b1, b2: weights of v1, v2)."""
+  out = np.zeros(3, np.float32)
+  args = [_f32(origin, 3), _f32(direction, 3), _f32(v0, 3), _f32(v1, 3), _f32(v2, 3)]
+  hit = library().oracle_ray_triangle(*(a.ctypes.data for a in args), float(tmax), out.ctypes.data)
+  return bool(hit), float(out[0]), float(out[1]), float(out[2])
+
+
+def display(color, tone_mapper: int) -> np.ndarray:
+  """Exposed linear HDR -> tone-mapped display-linear colour in [0, 1]."""
+  c, out = _f32(color, 3), np.zeros(3, np.float32)
+  library().oracle_display(c.ctypes.data, int(tone_mapper), out.ctypes.data)
+  return out
+
+
+def encode(color) -> int:
+  """Display-linear colour -> packed sRGB8 word."""
+  c = _f32(color, 3)
+  return int(library().oracle_encode(c.ctypes.data))
+
+
+def spec_ao(nov: float, ao: float, alpha: float) -> float:
+  return float(library().oracle_spec_ao(float(nov), float(ao), float(alpha)))
+
+
+def eval_light(light_type: int, model: int, has_spot: bool, pos, direction, cutoff: float, exponent: float, attenuation,
+               intensity: float, light_range: float, point) -> dict:
+  """One light evaluated at `point` (white light colour): direction to the light, distance and
+  the diffuse / specular scale."""
+  out = np.zeros(10, np.float32)
+  a = [_f32(pos, 3), _f32(direction, 3), _f32(attenuation, 3), _f32(point, 3)]
+  library().oracle_eval_light(int(light_type), int(model), int(bool(has_spot)), a[0].ctypes.data, a[1].ctypes.data, float(cutoff),
+                              float(exponent), a[2].ctypes.data, float(intensity), float(light_range), a[3].ctypes.data,
+                              out.ctypes.data)
+  return {"l": out[0:3].copy(), "dist": float(out[3]), "diffuse": out[4:7].copy(), "specular": out[7:10].copy()}
+
+
+def atrous_pass(src: np.ndarray, normal: np.ndarray, depth: np.ndarray, albedo: np.ndarray, step: int,
+                sigma=(4.0, 128.0, 0.02, 0.05)) -> np.ndarray:
+  """One a-trous pass over one (h, w) image: src (h, w, 4) demodulated colour + variance,
+  normal / albedo (h, w, 3) (stored as f16 like the renderer's features), depth (h, w)."""
+  from mujoco_warp_pbr_amd import _abi
+
+  h, w = depth.shape
+  S = _abi.structs()
+  p = S["MjpbrAtrousParams"]()
+  p.nworld, p.nrender, p.stride, p.iterations = 1, 1, h * w, 1
+  p.cam_res[0][0], p.cam_res[0][1] = w, h
+  p.rgb_adr[0] = 0
+  p.sigma_color, p.sigma_normal, p.sigma_depth, p.sigma_albedo = (float(x) for x in sigma)
+  o = S["MjpbrOutputs"]()
+  o.aov_stride = h * w
+  nrm = np.ascontiguousarray(np.asarray(normal, np.float16).reshape(-1, 3))
+  alb = np.ascontiguousarray(np.asarray(albedo, np.float16).reshape(-1, 3))
+  dep = np.ascontiguousarray(np.asarray(depth, np.float32).reshape(-1))
+  o.aov_normal = nrm.ctypes.data
+  o.aov_albedo = alb.ctypes.data
+  o.aov_depth = dep.ctypes.data
+  s = np.ascontiguousarray(np.asarray(src, np.float32).reshape(-1, 4))
+  dst = np.zeros_like(s)
+  vp = ctypes.c_void_p
+  library().oracle_atrous_pass(vp(ctypes.addressof(p)), vp(ctypes.addressof(o)), s.ctypes.data, dst.ctypes.data, int(step))
+  return dst.reshape(h, w, 4)
+
+
+def spec_lobe(f0, alpha: float, noh: float, nov: float, nol: float, loh: float) -> np.ndarray:
+  """F D V of the specular lobe (no NoL cosine)."""
+  a, out = _f32(f0, 3), np.zeros(3, np.float32)
+  library().oracle_spec_lobe(a.ctypes.data, float(alpha), float(noh), float(nov), float(nol), float(loh), out.ctypes.data)
+  return out
+
+
+def dfg_lookup(lut: np.ndarray, nov: float, pr: float) -> np.ndarray:
+  t = np.ascontiguousarray(lut, np.float32)
+  out = np.zeros(2, np.float32)
+  library().oracle_dfg_lookup(t.ctypes.data, int(t.shape[0]), float(nov), float(pr), out.ctypes.data)
+  return out
+
+
+def lobes(base_color, metallic: float, reflectance: float, pr: float, nov: float, lut: np.ndarray) -> dict:
+  t = np.ascontiguousarray(lut, np.float32)
+  b, out = _f32(base_color, 3), np.zeros(14, np.float32)
+  library().oracle_lobes(b.ctypes.data, float(metallic), float(reflectance), float(pr), float(nov), t.ctypes.data, int(t.shape[0]),
+                         out.ctypes.data)
+  return {"pr": out[0], "alpha": out[1], "f0": out[2:5], "energy_comp": out[5:8], "spec_albedo": out[8:11], "diffuse": out[11:14]}

@@ -6,6 +6,69 @@ static v3 half3_o(const uint16_t* p) { return mk3(half2f(p[0]), half2f(p[1]), ha
 
 static float b3_o(int i) { return i == 2 ? 0.375f : ((i == 1 || i == 3) ? 0.25f : 0.0625f); }
 
+/* One edge-avoiding pass of footprint `step` (reference denoise.py:105-178): src -> dst, both
+ * [nworld, stride, 4] (demodulated colour, variance). */
+static void atrous_pass_o(const MjpbrAtrousParams* p, const MjpbrOutputs* out, const float* src, float* dst, int step) {
+  int64_t n = (int64_t)p->nworld * p->stride;
+  for (int64_t i = 0; i < n; ++i) {
+    int w = (int)(i / p->stride), index = (int)(i - (int64_t)w * p->stride);
+    int adr = -1, W = 0, H = 0, pid = 0;
+    for (int c = 0; c < p->nrender; ++c) {
+      int ca = p->rgb_adr[c], np_ = p->cam_res[c][0] * p->cam_res[c][1];
+      if (ca >= 0 && index >= ca && index < ca + np_) { adr = ca; W = p->cam_res[c][0]; H = p->cam_res[c][1]; pid = index - ca; break; }
+    }
+    if (adr < 0) continue;
+    const float* sw = src + 4 * (int64_t)w * p->stride;
+    int64_t fa = (int64_t)w * out->aov_stride;
+    v3 n0 = half3_o(out->aov_normal + 3 * (fa + index));
+    if (dot3(n0, n0) == 0.0f) {
+      for (int k = 0; k < 4; ++k) dst[4 * i + k] = src[4 * i + k];
+      continue;
+    }
+    int px = pid % W, py = pid / W;
+    float variance = 0.0f, vw = 0.0f;
+    for (int j = 0; j < 3; ++j) {
+      int y = clampi(py + j - 1, 0, H - 1);
+      for (int k = 0; k < 3; ++k) {
+        int x = clampi(px + k - 1, 0, W - 1);
+        float g = (k == 1 ? 0.5f : 0.25f) * (j == 1 ? 0.5f : 0.25f);
+        variance += g * sw[4 * (adr + y * W + x) + 3];
+        vw += g;
+      }
+    }
+    float inv_l = 1.0f / (p->sigma_color * sqrtf(fmaxf(variance / vw, 0.0f)) + 1.0e-6f);
+    const float* s0 = sw + 4 * index;
+    float l0 = lum3(mk3(s0[0], s0[1], s0[2]));
+    float z0 = out->aov_depth[fa + index];
+    v3 a0 = half3_o(out->aov_albedo + 3 * (fa + index));
+    float inv_a = 1.0f / p->sigma_albedo;
+    float inv_z = 1.0f / fmaxf(p->sigma_depth * z0 * (float)step, 1.0e-6f);
+    v3 total = mk3(0, 0, 0);
+    float tv = 0.0f, ws = 0.0f;
+    for (int j = 0; j < 5; ++j) {
+      int y = py + (j - 2) * step;
+      if (y < 0 || y >= H) continue;
+      for (int k = 0; k < 5; ++k) {
+        int x = px + (k - 2) * step;
+        if (x < 0 || x >= W) continue;
+        int q = adr + y * W + x;
+        const float* s = sw + 4 * q;
+        v3 c = mk3(s[0], s[1], s[2]);
+        float wn = powf(fmaxf(dot3(n0, half3_o(out->aov_normal + 3 * (fa + q))), 0.0f), p->sigma_normal);
+        float dist = fabsf(out->aov_depth[fa + q] - z0) * inv_z + fabsf(lum3(c) - l0) * inv_l +
+                     length3(sub3(half3_o(out->aov_albedo + 3 * (fa + q)), a0)) * inv_a;
+        float weight = b3_o(k) * b3_o(j) * wn * expf(-dist);
+        total = add3(total, mul3(c, weight));
+        tv += weight * weight * s[3];
+        ws += weight;
+      }
+    }
+    v3 c = div3(total, ws);
+    dst[4 * i] = c.x; dst[4 * i + 1] = c.y; dst[4 * i + 2] = c.z;
+    dst[4 * i + 3] = tv / (ws * ws);
+  }
+}
+
 int oracle_atrous(const MjpbrAtrousParams* p, MjpbrOutputs* out, float* work0, float* work1) {
   int64_t n = (int64_t)p->nworld * p->stride;
   float* src = work0;
@@ -25,64 +88,7 @@ int oracle_atrous(const MjpbrAtrousParams* p, MjpbrOutputs* out, float* work0, f
   }
   /* _atrous */
   for (int it = 0; it < p->iterations; ++it) {
-    int step = 1 << it;
-    for (int64_t i = 0; i < n; ++i) {
-      int w = (int)(i / p->stride), index = (int)(i - (int64_t)w * p->stride);
-      int adr = -1, W = 0, H = 0, pid = 0;
-      for (int c = 0; c < p->nrender; ++c) {
-        int ca = p->rgb_adr[c], np_ = p->cam_res[c][0] * p->cam_res[c][1];
-        if (ca >= 0 && index >= ca && index < ca + np_) { adr = ca; W = p->cam_res[c][0]; H = p->cam_res[c][1]; pid = index - ca; break; }
-      }
-      if (adr < 0) continue;
-      const float* sw = src + 4 * (int64_t)w * p->stride;
-      int64_t fa = (int64_t)w * out->aov_stride;
-      v3 n0 = half3_o(out->aov_normal + 3 * (fa + index));
-      if (dot3(n0, n0) == 0.0f) {
-        for (int k = 0; k < 4; ++k) dst[4 * i + k] = src[4 * i + k];
-        continue;
-      }
-      int px = pid % W, py = pid / W;
-      float variance = 0.0f, vw = 0.0f;
-      for (int j = 0; j < 3; ++j) {
-        int y = clampi(py + j - 1, 0, H - 1);
-        for (int k = 0; k < 3; ++k) {
-          int x = clampi(px + k - 1, 0, W - 1);
-          float g = (k == 1 ? 0.5f : 0.25f) * (j == 1 ? 0.5f : 0.25f);
-          variance += g * sw[4 * (adr + y * W + x) + 3];
-          vw += g;
-        }
-      }
-      float inv_l = 1.0f / (p->sigma_color * sqrtf(fmaxf(variance / vw, 0.0f)) + 1.0e-6f);
-      const float* s0 = sw + 4 * index;
-      float l0 = lum3(mk3(s0[0], s0[1], s0[2]));
-      float z0 = out->aov_depth[fa + index];
-      v3 a0 = half3_o(out->aov_albedo + 3 * (fa + index));
-      float inv_a = 1.0f / p->sigma_albedo;
-      float inv_z = 1.0f / fmaxf(p->sigma_depth * z0 * (float)step, 1.0e-6f);
-      v3 total = mk3(0, 0, 0);
-      float tv = 0.0f, ws = 0.0f;
-      for (int j = 0; j < 5; ++j) {
-        int y = py + (j - 2) * step;
-        if (y < 0 || y >= H) continue;
-        for (int k = 0; k < 5; ++k) {
-          int x = px + (k - 2) * step;
-          if (x < 0 || x >= W) continue;
-          int q = adr + y * W + x;
-          const float* s = sw + 4 * q;
-          v3 c = mk3(s[0], s[1], s[2]);
-          float wn = powf(fmaxf(dot3(n0, half3_o(out->aov_normal + 3 * (fa + q))), 0.0f), p->sigma_normal);
-          float dist = fabsf(out->aov_depth[fa + q] - z0) * inv_z + fabsf(lum3(c) - l0) * inv_l +
-                       length3(sub3(half3_o(out->aov_albedo + 3 * (fa + q)), a0)) * inv_a;
-          float weight = b3_o(k) * b3_o(j) * wn * expf(-dist);
-          total = add3(total, mul3(c, weight));
-          tv += weight * weight * s[3];
-          ws += weight;
-        }
-      }
-      v3 c = div3(total, ws);
-      dst[4 * i] = c.x; dst[4 * i + 1] = c.y; dst[4 * i + 2] = c.z;
-      dst[4 * i + 3] = tv / (ws * ws);
-    }
+    atrous_pass_o(p, out, src, dst, 1 << it);
     float* t = src;
     src = dst;
     dst = t;
@@ -102,5 +108,11 @@ int oracle_atrous(const MjpbrAtrousParams* p, MjpbrOutputs* out, float* work0, f
     }
     out->rgb[(int64_t)w * out->rgb_stride + index] = encode_o(display_o(exposed, p->tone_mapper));
   }
+  return 0;
+}
+
+/* One pass on its own, for the tests that pin it. */
+int oracle_atrous_pass(const MjpbrAtrousParams* p, const MjpbrOutputs* out, const float* src, float* dst, int step) {
+  atrous_pass_o(p, out, src, dst, step);
   return 0;
 }

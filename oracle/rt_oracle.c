@@ -133,24 +133,23 @@ static float box_o(v3 pos, const float* mat, v3 size, v3 pnt, v3 vec, v3* n) {
   return best;
 }
 
-/* Moller-Trumbore on (v0, e1, e2); 0 <= t < tmax; b1, b2 weights of v1, v2. */
+/* Moller & Trumbore (1997), "Fast, minimum storage ray-triangle intersection", as published:
+ * solve o + t d = v0 + b1 e1 + b2 e2 by Cramer's rule with one division by the determinant.
+ * Double-sided (culling is the caller's); edges and vertices count as hits; 0 <= t < tmax. */
 static int tri_o(v3 o, v3 d, v3 v0, v3 e1, v3 e2, float tmax, float* t, float* b1, float* b2) {
-  /* Moller-Trumbore with the edge tests on the numerators, sign-normalised by det. */
   v3 p = cross3(d, e2);
   float det = dot3(e1, p);
-  if (fabsf(det) < 1.0e-20f) return 0;
+  if (fabsf(det) < 1.0e-20f) return 0; /* ray parallel to the triangle's plane */
+  float inv_det = 1.0f / det;
   v3 s = sub3(o, v0);
-  float us = dot3(s, p), adet = fabsf(det);
-  if (det < 0.0f) us = -us;
-  if (us < 0.0f || us > adet) return 0;
+  float u = dot3(s, p) * inv_det;
+  if (u < 0.0f || u > 1.0f) return 0;
   v3 q = cross3(s, e1);
-  float vs = dot3(d, q), ts = dot3(e2, q);
-  if (det < 0.0f) { vs = -vs; ts = -ts; }
-  if (vs < 0.0f || us + vs > adet || ts < 0.0f) return 0;
-  float inv = 1.0f / adet;
-  float tt = ts * inv;
-  if (tt >= tmax) return 0;
-  *t = tt; *b1 = us * inv; *b2 = vs * inv;
+  float v = dot3(d, q) * inv_det;
+  if (v < 0.0f || u + v > 1.0f) return 0;
+  float tt = dot3(e2, q) * inv_det;
+  if (tt < 0.0f || tt >= tmax) return 0;
+  *t = tt; *b1 = u; *b2 = v;
   return 1;
 }
 
@@ -616,7 +615,7 @@ static float oetf_o(float x) { return x <= 0.0031308f ? 12.92f * x : 1.055f * po
 static uint32_t encode_o(v3 c) {
   uint32_t r = (uint32_t)(oetf_o(c.x) * 255.0f + 0.5f), g = (uint32_t)(oetf_o(c.y) * 255.0f + 0.5f),
            b = (uint32_t)(oetf_o(c.z) * 255.0f + 0.5f);
-  return r | (g << 8) | (b << 16) | (255u << 24);
+  return (255u << 24) | (r << 16) | (g << 8) | b; /* 0xAARRGGBB (MuJoCo Warp pack_rgba_to_uint32) */
 }
 
 /* ------------------------------------------------------------------ surfaces (material.py:300-501) */
@@ -890,6 +889,12 @@ static v3 shade_light_o(const Ctx* c, const LightO* L, const LobesO* lb, v3 poin
   return mul3(add3(cwm(fd, ds), cwm(fr, ss)), nol * vis);
 }
 
+/* Lagarde & de Rousiers, "Moving Frostbite to PBR" (2014), specular occlusion from ambient
+ * occlusion (reference render.py:217-219). */
+static float spec_ao_o(float nov, float ao, float alpha) {
+  return clampf_(powf(nov + ao, powf(2.0f, -16.0f * alpha - 1.0f)) - 1.0f + ao, 0.0f, 1.0f);
+}
+
 static v3 shade_surface_o(const Ctx* c, const SurfO* sf, v3 point, v3 normal, v3 view, v3 head_dir, int stream) {
   const MjpbrScene* sc = c->sc;
   const MjpbrSettings* s = c->s;
@@ -909,7 +914,7 @@ static v3 shade_surface_o(const Ctx* c, const SurfO* sf, v3 point, v3 normal, v3
     prefiltered = add3(ambient, mul3(specular_env_o(env, e, dominant, yaw, lb.pr), scale));
   }
   float ao = sf->occlusion;
-  float spec_ao = clampf_(powf(nov + ao, powf(2.0f, -16.0f * alpha - 1.0f)) - 1.0f + ao, 0.0f, 1.0f);
+  float spec_ao = spec_ao_o(nov, ao, alpha);
   v3 radiance = add3(mul3(cwm(irradiance, lb.diffuse_color), ao), mul3(cwm(prefiltered, cwm(lb.spec_albedo, lb.energy_comp)), spec_ao));
   const MjpbrLights* lt = &sc->lit;
   for (int li = 0; li < lt->nlight; ++li) {
@@ -1337,4 +1342,100 @@ int oracle_closest(const MjpbrScene* sc, const MjpbrSettings* s, int w, const fl
   *geom = h.geom_id;
   *dist = h.dist;
   return 0;
+}
+
+/* ------------------------------------------------------------------ test entry points
+ * The oracle's scalar building blocks, exported so CPU tests can pin each one against closed
+ * forms and published values (the renderer paths above call the same static functions). */
+float oracle_ray_primitive(int type, const float* pos, const float* mat, const float* size, const float* o, const float* d,
+                           float* normal) {
+  v3 n = mk3(0, 0, 0), p = load3(pos), sz = load3(size), po = load3(o), pd = load3(d);
+  float x = -1.0f;
+  switch (type) {
+    case MJPBR_GEOM_PLANE: x = plane_o(p, mat, sz, po, pd, &n); break;
+    case MJPBR_GEOM_SPHERE: x = sphere_o(p, sz.x * sz.x, po, pd, &n); break;
+    case MJPBR_GEOM_ELLIPSOID: x = ellipsoid_o(p, mat, sz, po, pd, &n); break;
+    case MJPBR_GEOM_CAPSULE: x = capsule_o(p, mat, sz, po, pd, &n); break;
+    case MJPBR_GEOM_CYLINDER: x = cylinder_o(p, mat, sz, po, pd, &n); break;
+    case MJPBR_GEOM_BOX: x = box_o(p, mat, sz, po, pd, &n); break;
+    default: break;
+  }
+  normal[0] = n.x; normal[1] = n.y; normal[2] = n.z;
+  return x;
+}
+
+/* out = {t, b1, b2}; returns 1 on a hit. */
+int oracle_ray_triangle(const float* o, const float* d, const float* v0, const float* v1, const float* v2, float tmax, float* out) {
+  v3 a = load3(v0);
+  float t = 0, b1 = 0, b2 = 0;
+  int hit = tri_o(load3(o), load3(d), a, sub3(load3(v1), a), sub3(load3(v2), a), tmax, &t, &b1, &b2);
+  out[0] = t; out[1] = b1; out[2] = b2;
+  return hit;
+}
+
+void oracle_display(const float* c, int tone_mapper, float* out) {
+  v3 r = display_o(load3(c), tone_mapper);
+  out[0] = r.x; out[1] = r.y; out[2] = r.z;
+}
+
+uint32_t oracle_encode(const float* c) { return encode_o(load3(c)); }
+
+float oracle_spec_ao(float nov, float ao, float alpha) { return spec_ao_o(nov, ao, alpha); }
+
+/* One light at `point`: out = {l.xyz, dist, diffuse scale.xyz, specular scale.xyz}. */
+void oracle_eval_light(int type, int model, int has_spot, const float* pos, const float* dir, float cutoff, float exponent,
+                       const float* attenuation, float intensity, float range, const float* point, float* out) {
+  LightO L;
+  memset(&L, 0, sizeof(L));
+  L.type = type;
+  L.pos = load3(pos);
+  L.dir = load3(dir);
+  L.diffuse = mk3(1, 1, 1);
+  L.specular = mk3(1, 1, 1);
+  L.attenuation = load3(attenuation);
+  L.cutoff = cutoff;
+  L.exponent = exponent;
+  L.intensity = intensity;
+  L.range = range;
+  v3 l, ds, ss;
+  float dist;
+  eval_light_o(&L, model, has_spot, load3(point), &l, &dist, &ds, &ss);
+  out[0] = l.x; out[1] = l.y; out[2] = l.z; out[3] = dist;
+  out[4] = ds.x; out[5] = ds.y; out[6] = ds.z;
+  out[7] = ss.x; out[8] = ss.y; out[9] = ss.z;
+}
+
+/* out = F D V of the specular lobe (brdf.py:32-72), no NoL cosine. */
+void oracle_spec_lobe(const float* f0, float alpha, float noh, float nov, float nol, float loh, float* out) {
+  v3 r = spec_lobe_o(load3(f0), alpha, noh, nov, nol, loh);
+  out[0] = r.x; out[1] = r.y; out[2] = r.z;
+}
+
+/* Bilinear DFG lookup (ibl.py:117-119) in an n x n x 2 table. */
+void oracle_dfg_lookup(const float* lut, int n, float nov, float pr, float* out) {
+  MjpbrEnvironment env;
+  memset(&env, 0, sizeof(env));
+  env.dfg_size = n;
+  env.dfg_lut = lut;
+  v2 r = dfg_o(&env, nov, pr);
+  out[0] = r.x; out[1] = r.y;
+}
+
+/* Shading lobes (material.py:506-539) of a surface against an n x n x 2 DFG table:
+ * out = {pr, alpha, f0.xyz, energy_comp.xyz, spec_albedo.xyz, diffuse_color.xyz}. */
+void oracle_lobes(const float* base_color, float metallic, float reflectance, float pr, float nov, const float* lut, int n, float* out) {
+  MjpbrEnvironment env;
+  memset(&env, 0, sizeof(env));
+  env.dfg_size = n;
+  env.dfg_lut = lut;
+  SurfO sf;
+  memset(&sf, 0, sizeof(sf));
+  sf.base_color = load3(base_color);
+  sf.metallic = metallic;
+  sf.reflectance = reflectance;
+  sf.pr = pr;
+  LobesO L = lobes_o(&env, &sf, nov);
+  float v[14] = {L.pr, L.alpha, L.f0.x, L.f0.y, L.f0.z, L.energy_comp.x, L.energy_comp.y, L.energy_comp.z,
+                 L.spec_albedo.x, L.spec_albedo.y, L.spec_albedo.z, L.diffuse_color.x, L.diffuse_color.y, L.diffuse_color.z};
+  memcpy(out, v, sizeof(v));
 }

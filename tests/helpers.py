@@ -58,7 +58,7 @@ def compare_rt(gpu: dict, ref: dict, hdr_atol=1e-3, depth_rtol=1e-5):
   np.testing.assert_allclose(gpu["hdr"], ref["hdr"], atol=hdr_atol, rtol=0)
   a = gpu["rgb"].view(np.uint32)
   b = ref["rgb"].view(np.uint32)
-  for sh in (0, 8, 16):
+  for sh in (16, 8, 0):
     ca = ((a >> sh) & 0xFF).astype(np.int32)
     cb = ((b >> sh) & 0xFF).astype(np.int32)
     assert np.abs(ca - cb).max() <= 1
@@ -87,9 +87,9 @@ def xyaxes(x, y) -> np.ndarray:
 
 
 def channels(rgb: np.ndarray) -> np.ndarray:
-  """Packed RGBA8 (uint32) -> (..., 3) floats in [0, 1], like the reference's get_rgb."""
+  """Packed 0xAARRGGBB (uint32) -> (..., 3) floats in [0, 1], like the reference's get_rgb."""
   rgb = np.asarray(rgb).view(np.uint32)
-  return np.stack([(rgb >> s) & 0xFF for s in (0, 8, 16)], axis=-1).astype(np.float32) / 255.0
+  return np.stack([(rgb >> s) & 0xFF for s in (16, 8, 0)], axis=-1).astype(np.float32) / 255.0
 
 
 class Shot:

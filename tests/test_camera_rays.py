@@ -78,7 +78,7 @@ def _ray_scale(offset, fovys=(50.0, 70.0)) -> np.ndarray:
 
 def _channels(rgb):
   rgb = rgb.view(np.uint32)
-  return np.stack([(rgb >> s) & 0xFF for s in (0, 8, 16)], axis=-1).astype(np.int32)
+  return np.stack([(rgb >> s) & 0xFF for s in (16, 8, 0)], axis=-1).astype(np.int32)
 
 
 def _oracle(mjm, rays=None, nworld=2, **kw):

@@ -177,10 +177,10 @@ def test_psnr_and_ssim_scores():
 
   rng = np.random.default_rng(0)
   img = rng.integers(0, 255, (2, 32 * 32, 3))
-  packed = torch.from_numpy((img[..., 0] | (img[..., 1] << 8) | (img[..., 2] << 16) | (255 << 24)).astype(np.int64).astype(np.int32))
+  packed = torch.from_numpy(((img[..., 0] << 16) | (img[..., 1] << 8) | img[..., 2] | (255 << 24)).astype(np.int64).astype(np.int32))
   assert td.ssim(packed, packed, 32, 32) == pytest.approx(1.0, abs=1e-6)
   noisy = np.clip(img + rng.integers(-20, 21, img.shape), 0, 255)
-  packed2 = torch.from_numpy((noisy[..., 0] | (noisy[..., 1] << 8) | (noisy[..., 2] << 16) | (255 << 24)).astype(np.int64).astype(np.int32))
+  packed2 = torch.from_numpy(((noisy[..., 0] << 16) | (noisy[..., 1] << 8) | noisy[..., 2] | (255 << 24)).astype(np.int64).astype(np.int32))
   mse = np.mean(((noisy - img) / 255.0) ** 2)
   assert td.psnr(packed2, packed) == pytest.approx(10 * np.log10(1 / mse), rel=1e-6)
   assert 0.0 < td.ssim(packed2, packed, 32, 32) < 0.99

@@ -55,7 +55,7 @@ def test_fallback_directional_matches_diffuse_colour():
   m, d, rc = make(mjm, cam_res=(8, 8), tone_mapper=mj.ToneMapper.CLAMP, ev100=14.96578428466209)
   out = oracle.rt_render(m, d, rc)
   px = out["rgb"][0].reshape(8, 8)[4, 4]
-  rgb = np.array([(px >> s) & 0xFF for s in (0, 8, 16)]) / 255.0
+  rgb = np.array([(px >> s) & 0xFF for s in (16, 8, 0)]) / 255.0
   np.testing.assert_allclose(rgb, _srgb(np.array([0.5, 0.25, 0.125])), atol=2.5 / 255)
 
 
@@ -67,7 +67,7 @@ def test_light_models_agree_for_default_lights(light_model):
   m, d, rc = make(mjm, cam_res=(8, 8), tone_mapper=mj.ToneMapper.CLAMP, light_model=light_model, ev100=14.96578428466209)
   out = oracle.rt_render(m, d, rc)
   px = out["rgb"][0].reshape(8, 8)[4, 4]
-  rgb = np.array([(px >> s) & 0xFF for s in (0, 8, 16)]) / 255.0
+  rgb = np.array([(px >> s) & 0xFF for s in (16, 8, 0)]) / 255.0
   np.testing.assert_allclose(rgb, _srgb(np.full(3, 0.7 * 0.6)), atol=2.5 / 255)
 
 
