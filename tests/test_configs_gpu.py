@@ -131,7 +131,7 @@ def test_gpu_spot_and_aces_match_oracle(cuda, name, kw):
   ref = oracle.rt_render(m, d, rc, seed=0, threads=16)
   compare_rt(gpu, ref)
   if name == "spot_physical":
-    # the cone is visible: lit and unlit floor inside the frame
+    # the cone edge is in the frame: floor inside it is lit by the spot, floor outside is not
     lum = gpu["hdr"][0].reshape(48, 48, 3).sum(-1)
     floor = gpu["seg"][0].reshape(48, 48, 2)[..., 0] == 0
-    assert lum[floor].max() > 4 * np.percentile(lum[floor], 10)
+    assert lum[floor].max() - np.percentile(lum[floor], 10) > 0.5
