@@ -331,6 +331,15 @@ int mjpbr_scene_bvh(const MjpbrGeometry* geo, int refit, void* stream);
 int mjpbr_rt_render(const MjpbrScene* scene, const MjpbrSettings* s, MjpbrOutputs* out, int32_t seed,
                     void* stream);
 
+/* mjpbr_rt_render without a caller-provided world_stage and without any allocation: the
+ * per-world staging lives in a static device array of MJPBR_NOALLOC_STAGE_BYTES the library holds
+ * on each GPU it runs on (scene->geo.world_stage is ignored). No counterpart in the reference; it
+ * is for callers that cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops). Returns MJPBR_EINVAL when nworld x mjpbr_world_stage_floats x 4 exceeds the maximum.
+ * Calls share one array per GPU: calls that overlap on one GPU (different streams) race. */
+#define MJPBR_NOALLOC_STAGE_BYTES 33554432 /* 32 MiB */
+int mjpbr_rt_render_noalloc(const MjpbrScene* scene, const MjpbrSettings* s, MjpbrOutputs* out, int32_t seed, void* stream);
+
 /* Feature mask of the real-time kernel instance mjpbr_rt_render launches for these settings
  * (bits: 1 meshes, 2 supersampling, 4 soft shadows, 8 mirror, 16 layers, 32 textures): the
  * reference's kernel specialisation key (kernel_inputs.py:321-337) as compiled instances. */

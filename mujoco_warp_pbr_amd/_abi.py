@@ -41,6 +41,7 @@ FUNCTIONS = {
   "mjpbr_struct_sizes": (ctypes.c_int, [_VP, ctypes.c_int]),
   "mjpbr_scene_bvh": (ctypes.c_int, [_VP, ctypes.c_int, _VP]),
   "mjpbr_rt_render": (ctypes.c_int, [_VP, _VP, _VP, ctypes.c_int32, _VP]),
+  "mjpbr_rt_render_noalloc": (ctypes.c_int, [_VP, _VP, _VP, ctypes.c_int32, _VP]),
   "mjpbr_rt_kernel_features": (ctypes.c_int, [_VP]),
   "mjpbr_world_stage_floats": (ctypes.c_int64, [_VP]),
   "mjpbr_pt_render": (ctypes.c_int, [_VP, _VP, _VP, ctypes.c_int32, _VP]),

@@ -5,6 +5,7 @@
 // of its pixel: primary ray, split-sum IBL + direct lights with ray-traced (soft) shadows +
 // emission, one mirror bounce, translucent layers, tone mapping and sRGB encoding.
 #pragma clang fp contract(off)
+#include <mutex>
 #include "abi_util.hpp"
 #include "rt_common.hpp"
 
@@ -227,4 +228,57 @@ extern "C" int mjpbr_rt_render(const MjpbrScene* scene, const MjpbrSettings* s, 
   if ((rc = launch_stage_prep(scene, (hipStream_t)stream))) return rc;
   hipLaunchKernelGGL(k.fn, dim3(nlaunch), dim3(256), lds, (hipStream_t)stream, *scene, *s, *out, seed, tm);
   return check_launch("rt_kernel");
+}
+
+// ---------------------------------------------------------------- allocation-free variant
+// The per-world staging buffer of mjpbr_rt_render_noalloc: a file-scope device array, so the
+// entry point needs no world_stage from the caller and allocates nothing. Each GPU the library
+// runs on holds its own copy (MJPBR_NOALLOC_STAGE_BYTES) until the process ends. In host code
+// the name `scratch` is not a device address: the address of this device's copy comes from
+// hipGetSymbolAddress, once per device, and only that address is handed to kernels.
+static __device__ __align__(256) float scratch[(size_t)MJPBR_NOALLOC_STAGE_BYTES / sizeof(float)];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+float* g_scratch_ptr[kMaxDevices] = {};
+
+// Address of the current device's scratch (looked up on the first call on that device).
+int scratch_address(float** ptr) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return set_error(MJPBR_ELAUNCH, "hipGetDevice: %s", hipGetErrorString(e));
+  if (dev < 0 || dev >= kMaxDevices) return set_error(MJPBR_EINVAL, "device %d beyond the %d the library tracks", dev, kMaxDevices);
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) return set_error(MJPBR_ELAUNCH, "hipGetSymbolAddress(scratch): %s", hipGetErrorString(e));
+    g_scratch_ptr[dev] = static_cast<float*>(p);
+  }
+  *ptr = g_scratch_ptr[dev];
+  return MJPBR_OK;
+}
+
+}  // namespace
+
+extern "C" int mjpbr_rt_render_noalloc(const MjpbrScene* scene, const MjpbrSettings* s, MjpbrOutputs* out, int32_t seed,
+                                       void* stream) {
+  if (!scene || !s || !out) return set_error(MJPBR_EINVAL, "scene, settings and outputs must not be NULL");
+  const int64_t floats = mjpbr_world_stage_floats(scene);
+  if (floats < 0) return (int)floats;
+  if (scene->geo.nworld < 1) return set_error(MJPBR_EINVAL, "nworld must be at least 1, got %d", scene->geo.nworld);
+  const int64_t bytes = floats * (int64_t)sizeof(float) * scene->geo.nworld;
+  if (bytes > (int64_t)MJPBR_NOALLOC_STAGE_BYTES)
+    return set_error(MJPBR_EINVAL, "%d worlds need %lld bytes of staging; the allocation-free entry point holds %lld (use mjpbr_rt_render with a world_stage buffer)",
+                     scene->geo.nworld, (long long)bytes, (long long)MJPBR_NOALLOC_STAGE_BYTES);
+  float* ptr = nullptr;
+  int rc = scratch_address(&ptr);
+  if (rc) return rc;
+  // Every world's block is rewritten by the staging launch before the render kernel reads it on
+  // the same stream, so nothing needs clearing between calls.
+  MjpbrScene sc = *scene;
+  sc.geo.world_stage = ptr;
+  return mjpbr_rt_render(&sc, s, out, seed, stream);
 }
