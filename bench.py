@@ -9,8 +9,12 @@ resident in HBM before the timed region.
 
 Prints one JSON line: Mpixels/s, the roofline of the render kernel (algorithmic HBM bytes of
 one launch / its average duration measured with HIP events on the launch stream; `traffic` from
-the committed rocprofv3 counter runs) and the CPU oracle (scalar C restatement, threads over
-worlds) timed on a bounded sample of the same workload on this host. Two path-tracing sections
+the committed rocprofv3 counter runs), the CPU oracle (scalar C restatement, threads over
+worlds) timed on a bounded sample of the same workload on this host, `parity` (the sampled
+worlds of the timed render against the oracle's images: segmentation bit-exact, depth 1e-5
+relative, sRGB8 within 1), `small_case` (16 worlds x 64^2 on the GPU and on every host core the
+way the reference's CPU benchmark runs), and `reference_suite` / `reference_suite_path_traced`
+(the reference suite's procedural scenes next to its published rates). Path-tracing sections
 follow in the same line (skip with --no-pt):
   pt_wavefront      4096 worlds at mixed resolutions (2048 x 64^2, 1536 x 128^2, 512 x 256^2),
                     4 spp, depth 8: the persistent wavefront tracer (autotuned, graph replay)
@@ -103,8 +107,9 @@ def measured_traffic(nworld: int, res: int):
   return None
 
 
-def cpu_baseline(mjm, m, d, rc, target_s: float = 12.0) -> dict:
-  """The CPU oracle on a sample of the workload's worlds, one thread per available core."""
+def cpu_baseline(mjm, m, d, rc, target_s: float = 12.0) -> tuple[dict, dict, int]:
+  """The CPU oracle on a sample of the workload's worlds, one thread per available core.
+  Returns the baseline record, the oracle's output images and the number of sampled worlds."""
   import oracle
 
   cores = len(os.sched_getaffinity(0))
@@ -116,7 +121,7 @@ def cpu_baseline(mjm, m, d, rc, target_s: float = 12.0) -> dict:
   nsample = max(cores, min(d.nworld, int(target_s * cores / max(one, 1e-3))))
   nsample = min(nsample, d.nworld)
   t0 = time.perf_counter()
-  oracle.rt_render(m, d, rc, seed=0, threads=cores, worlds=(0, nsample))
+  images = oracle.rt_render(m, d, rc, seed=0, threads=cores, worlds=(0, nsample))
   dt = time.perf_counter() - t0
   mpx = nsample * res * res / dt / 1e6
   return {
@@ -124,8 +129,33 @@ def cpu_baseline(mjm, m, d, rc, target_s: float = 12.0) -> dict:
     "unit": "Mpixels/s",
     "cores": cores,
     "kind": "port",
-    "sample": f"{nsample} of the {d.nworld} worlds ({res}x{res}, same scene and settings), {dt:.1f} s on {cores} threads",
+    "sample": f"{nsample} of the {d.nworld} worlds ({res}x{res}, same scene and settings), {dt:.1f} s on {cores} threads; "
+    "CPU oracle restatement (scalar C, brute-force ray queries, not Warp)",
+  }, images, nsample
+
+
+def full_size_parity(gpu: dict, ref: dict, nworld: int) -> dict:
+  """The bench render's first `nworld` worlds against the oracle's images of the same worlds:
+  segmentation bit-exact, depth within 1e-5 relative, packed sRGB8 within 1 per channel."""
+  import numpy as np
+
+  seg_g, seg_r = gpu["seg"][:nworld], ref["seg"][:nworld]
+  dg, dr = gpu["depth"][:nworld].astype(np.float64), ref["depth"][:nworld].astype(np.float64)
+  rel = np.abs(dg - dr) / np.maximum(np.abs(dr), 1e-30)
+  rel[(dg == 0) & (dr == 0)] = 0.0
+  a = gpu["rgb"][:nworld].view(np.uint32)
+  b = ref["rgb"][:nworld].view(np.uint32)
+  lsb = max(int(np.abs(((a >> sh) & 0xFF).astype(np.int32) - ((b >> sh) & 0xFF).astype(np.int32)).max()) for sh in (16, 8, 0))
+  seg_bad = int(np.sum(np.any(seg_g != seg_r, axis=-1)))
+  out = {
+    "worlds": nworld,
+    "pixels": int(seg_g.shape[0] * seg_g.shape[1]),
+    "seg_mismatched_pixels": seg_bad,
+    "depth_max_rel": float(rel.max()),
+    "rgb_max_lsb": lsb,
   }
+  out["pass"] = seg_bad == 0 and out["depth_max_rel"] <= 1e-5 and lsb <= 1
+  return out
 
 
 # The reference's published real-time rates for its suite at 1024 worlds x 128^2 (frames/s x
@@ -181,6 +211,153 @@ def reference_suite_section(device, nworld: int = 1024, res: int = 128, reps: in
     del rc, m, d
     torch.cuda.empty_cache()
   out["workload"] = f"reference suite scenes, {nworld} worlds x {res}^2, rgb only, 1 spp, shadows, 1024x512 HDR environment"
+  return out
+
+
+# The reference's published Warp CPU rates (results/cpu.jsonl:201-208; AMD EPYC 7R13, 16 cores /
+# 32 threads, real-time tier at 128^2): Mpixels/s on 1 thread and on 32 threads.
+WARP_CPU_PUBLISHED_MPX = {
+  "material_grid": {"1_thread": 1.44, "32_threads": 27.5},
+  "primitives": {"1_thread": 0.67, "32_threads": 12.6},
+}
+
+
+def small_case_section(device, nworld: int = 16, res: int = 64, seconds: float = 3.0) -> dict:
+  """The small real-time case (plane + 8 primitives, a directional and a point light, constant grey
+  sky, 16 worlds x 64^2, 1 spp): the GPU render timed with HIP events, and the CPU oracle run the
+  way the reference's CPU benchmark runs Warp's CPU device (benchmarks/cpu_bench.py:16-46,
+  :308-333): one worker thread per available core, each rendering its own copy of the 16 worlds
+  back to back for `seconds` after a common start, aggregate frames/s."""
+  import ctypes
+  import threading
+
+  import torch
+
+  import mujoco_warp_pbr_amd as mj
+  import oracle
+  from mujoco_warp_pbr_amd import scenes
+  from mujoco_warp_pbr_amd.render import outputs_struct
+  from mujoco_warp_pbr_amd.render import scene_struct
+  from mujoco_warp_pbr_amd.render import settings_struct
+
+  mjm = scenes.primitives_scene(0, res)
+  m = mj.put_model(mjm, device)
+  d = mj.put_data(mjm, nworld, device)
+  rc = mj.create_render_context(mjm, nworld=nworld, cam_res=(res, res), envmap=scenes.constant_sky(), render_rgb=True,
+                                render_depth=True, render_seg=True, device=device)
+  mj.refit_bvh(m, d, rc)
+  for _ in range(5):
+    mj.render(m, d, rc)
+  torch.cuda.synchronize()
+  stream = torch.cuda.current_stream()
+  reps = 200
+  a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+  a.record(stream)
+  for _ in range(reps):
+    mj.render(m, d, rc)
+  b.record(stream)
+  b.synchronize()
+  gpu_ms = a.elapsed_time(b) / reps
+
+  cores = len(os.sched_getaffinity(0))
+  lib = oracle.library()
+  workers = []
+  for _ in range(cores):
+    hm, hd, hrc = oracle.to_host(m, d, rc)
+    structs = (scene_struct(hm, hd, hrc), settings_struct(hrc), outputs_struct(hrc))
+    workers.append((structs, (hm, hd, hrc)))
+  frames = [0] * cores
+  start = threading.Barrier(cores + 1)
+  stop = [0.0]
+
+  def run(i):
+    (sc, st, out), _keep = workers[i]
+    args = (ctypes.c_void_p(ctypes.addressof(sc)), ctypes.c_void_p(ctypes.addressof(st)), ctypes.c_void_p(ctypes.addressof(out)))
+    start.wait()
+    while time.perf_counter() < stop[0]:
+      lib.oracle_rt_render(*args, 0, 0, nworld)
+      frames[i] += nworld
+
+  pool = [threading.Thread(target=run, args=(i,)) for i in range(cores)]
+  for t in pool:
+    t.start()
+  t0 = time.perf_counter()
+  stop[0] = t0 + seconds
+  start.wait()
+  for t in pool:
+    t.join()
+  dt = time.perf_counter() - t0
+  cpu_fps = sum(frames) / dt
+  gpu_fps = nworld / gpu_ms * 1e3
+  log(f"small case: GPU {gpu_fps:.0f} frames/s, CPU oracle {cpu_fps:.0f} frames/s on {cores} threads")
+  return {
+    "workload": f"real-time tier, plane + 8 primitives, 2 lights, constant grey sky, {nworld} worlds x {res}^2, 1 spp, rgb + depth + seg",
+    "gpu_ms": round(gpu_ms, 4),
+    "gpu_frames_s": round(gpu_fps, 1),
+    "gpu_Mpixels_s": round(gpu_fps * res * res / 1e6, 2),
+    "cpu_oracle": {
+      "label": "CPU oracle restatement (scalar C, brute-force ray queries, not Warp)",
+      "threads": cores,
+      "frames_s": round(cpu_fps, 1),
+      "Mpixels_s": round(cpu_fps * res * res / 1e6, 3),
+      "Mpixels_s_per_thread": round(cpu_fps * res * res / 1e6 / cores, 4),
+      "seconds": round(dt, 2),
+    },
+    "gpu_vs_cpu_oracle": round(gpu_fps / cpu_fps, 1),
+    "reference_warp_cpu_published_Mpixels_s_128px": WARP_CPU_PUBLISHED_MPX,
+  }
+
+
+# The reference's published path-traced suite rates (16 spp + a-trous, 1024 worlds x 128^2;
+# results/suite/suite-v2/README.md:54-55, :129-130, :204-205), frames/s x 16384 = Mpixels/s.
+PT_SUITE_PUBLISHED_MPX = {
+  "material_grid": {"rtx_pro_6000": 3.6e3 * 16384 / 1e6, "h200": 1.4e3 * 16384 / 1e6, "a100": 1.0e3 * 16384 / 1e6},
+  "primitives": {"rtx_pro_6000": 2.9e3 * 16384 / 1e6, "h200": 658 * 16384 / 1e6, "a100": 500 * 16384 / 1e6},
+}
+
+
+def pt_suite_section(device, nworld: int = 1024, res: int = 128, reps: int = 3) -> dict:
+  """The reference suite's path-traced setting on its two procedural scenes: 16 paths a pixel,
+  depth 4, Russian roulette from 3, a-trous denoising (3 iterations), 1024 worlds x 128^2 (the
+  reference suite's `pbr_path`: tier=1, path_samples=16, denoise=true, other settings default)."""
+  import torch
+
+  import mujoco_warp_pbr_amd as mj
+  from mujoco_warp_pbr_amd import scenes
+
+  out = {}
+  stream = torch.cuda.current_stream()
+  for name, fn in (("material_grid", scenes.suite_material_grid), ("primitives", scenes.suite_primitives)):
+    mjm = fn(res)
+    m = mj.put_model(mjm, device)
+    d = mj.put_data(mjm, nworld, device)
+    rc = mj.create_render_context(mjm, nworld=nworld, cam_res=(res, res), render_rgb=True, render_depth=False, render_seg=False,
+                                  envmap=scenes.procedural_sky(3, 512, 1024), tier=mj.Tier.PATH, path_samples=16, denoise=True,
+                                  device=device)
+    mj.refit_bvh(m, d, rc)
+    mj.render(m, d, rc)
+    torch.cuda.synchronize()
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record(stream)
+    for _ in range(reps):
+      mj.render(m, d, rc)
+    b.record(stream)
+    b.synchronize()
+    ms = a.elapsed_time(b) / reps
+    mpx = nworld * res * res / ms / 1e3
+    pub = PT_SUITE_PUBLISHED_MPX[name]
+    log(f"suite PT {name}: {ms:.1f} ms = {mpx:.1f} Mpx/s (reference RTX PRO 6000 {pub['rtx_pro_6000']:.1f})")
+    out[name] = {
+      "ms": round(ms, 3),
+      "Mpixels_s": round(mpx, 2),
+      "Mpaths_s": round(mpx * 16, 1),
+      "reference_published_Mpixels_s": {k: round(v, 2) for k, v in pub.items()},
+      "vs_rtx_pro_6000": round(mpx / pub["rtx_pro_6000"], 3),
+      "vs_h200": round(mpx / pub["h200"], 3),
+    }
+    del rc, m, d
+    torch.cuda.empty_cache()
+  out["workload"] = f"reference suite scenes, path-traced tier, {nworld} worlds x {res}^2, 16 spp, depth 4, a-trous (3 iterations), rgb"
   return out
 
 
@@ -282,11 +459,11 @@ def learned_section(device, time_worlds: int = 1024, quality_worlds: int = 128, 
     "fallback_worlds": res["fallback_worlds"],
     "unet_roofline": {
       "bound": "mfma",
-      "achieved": round(res["unet_tflops"], 2),
-      "achieved_algorithmic": round(res["unet_tflops_algorithmic"], 2),
+      "achieved": round(res["unet_tflops_algorithmic"], 2),
+      "achieved_as_executed": round(res["unet_tflops"], 2),
       "peak": peak,
       "unit": "TFLOP/s",
-      "frac": round(res["unet_tflops"] / peak, 4),
+      "frac": round(res["unet_tflops_algorithmic"] / peak, 4),
       "ms": round(res["unet_ms"], 3),
     },
   }
@@ -381,15 +558,30 @@ def main(argv=None) -> int:
   achieved = total_bytes / (kernel_ms * 1e-3) / 1e9
   log(f"{value:.1f} Mpx/s, render kernel {kernel_ms:.3f} ms")
 
-  baseline = None
+  baseline = parity = configs0 = None
   if not args.no_cpu_baseline:
+    gpu_images = {
+      "rgb": rc.base.rgb_data.cpu().numpy(),
+      "depth": rc.base.depth_data.cpu().numpy(),
+      "seg": rc.base.seg_data.cpu().numpy(),
+    }
     log("cpu baseline (oracle)")
-    baseline = cpu_baseline(mjm, m, d, rc)
+    baseline, ref_images, nsample = cpu_baseline(mjm, m, d, rc)
+    parity = full_size_parity(gpu_images, ref_images, nsample)
+    log(f"full-size parity on {nsample} worlds: {parity}")
+    del ref_images, gpu_images
 
   del rc, m, d
   torch.cuda.empty_cache()
+  if not args.no_cpu_baseline:
+    log("small real-time case (16 worlds x 64^2): GPU and CPU oracle")
+    configs0 = small_case_section(device)
   log("reference suite scenes (material_grid, primitives)")
   suite = reference_suite_section(device)
+  pt_suite = None
+  if not args.no_pt:
+    log("reference suite scenes, path-traced tier (16 spp + a-trous)")
+    pt_suite = pt_suite_section(device)
 
   pt_wavefront = learned_denoiser = learned_denoiser_mesh = None
   if not args.no_pt:
@@ -431,7 +623,10 @@ def main(argv=None) -> int:
       "bytes_per_pixel": round(bpp, 3),
     },
     "cpu_baseline": baseline,
+    "parity": parity,
+    "small_case": configs0,
     "reference_suite": suite,
+    "reference_suite_path_traced": pt_suite,
     "pt_wavefront": pt_wavefront,
     "learned_denoiser": learned_denoiser,
     "learned_denoiser_mesh_256": learned_denoiser_mesh,

@@ -151,3 +151,37 @@ def test_world_count_mismatch_is_refused_before_any_launch():
   m, d, prc = make(mjm, nworld=2, cam_res=(8, 8), tier=mj.Tier.PATH, path_samples=1)
   with pytest.raises(ValueError, match="data has 3 worlds, the context 2"):
     render_path_samples(m, d3, prc)
+
+
+@pytest.mark.gpu
+def test_gpu_readback_matches_oracle_buffers(cuda):
+  """Every get_* on GPU tensors after a GPU render of 2 worlds x 2 cameras (16 x 16 with RGB,
+  24 x 12 depth and segmentation only), against the oracle's buffers read with the reference's
+  semantics (output.py:34-152)."""
+  mjm, _ = _two_camera_scene()
+  m, d, rc = make(mjm, nworld=2, device=cuda, cam_res=None, render_rgb=[True, False], render_depth=[True, True],
+                  render_seg=[True, True], render_guides=True)
+  mj.refit_bvh(m, d, rc)
+  mj.render(m, d, rc)
+  ref = oracle.rt_render(m, d, rc)
+  w0, h0 = 16, 16
+  rgb = torch.zeros((2, h0, w0, 3), device=cuda)
+  mj.get_rgb(rc, 0, rgb)
+  words = ref["rgb"][:, : w0 * h0].reshape(2, h0, w0)
+  want = np.stack([((words >> s) & 0xFF) for s in (16, 8, 0)], -1).astype(np.float32) / 255.0
+  assert np.abs(rgb.cpu().numpy() - want).max() <= 1.0 / 255.0 + 1e-7
+  for name, key in (("get_hdr", "hdr"), ("get_albedo", "albedo"), ("get_normal", "normal")):
+    out = torch.zeros((2, h0, w0, 3), device=cuda)
+    getattr(mj, name)(rc, 0, out)
+    np.testing.assert_allclose(out.cpu().numpy(), ref[key][:, : w0 * h0].reshape(2, h0, w0, 3), atol=1e-3 if key == "hdr" else 1e-5)
+  for cam in (0, 1):
+    w, h = (int(x) for x in rc.base.cam_res[cam])
+    adr = int(rc.base.depth_adr[cam])
+    dep = torch.zeros((2, h, w), device=cuda)
+    mj.get_depth(rc, cam, 2.0, dep)
+    raw = ref["depth"][:, adr : adr + w * h].reshape(2, h, w)
+    np.testing.assert_allclose(dep.cpu().numpy(), np.clip(raw / 2.0, 0, 1), rtol=1e-5, atol=1e-7)
+    seg = torch.zeros((2, h, w, 2), dtype=torch.int32, device=cuda)
+    mj.get_segmentation(rc, cam, seg)
+    sadr = int(rc.base.seg_adr[cam])
+    np.testing.assert_array_equal(seg.cpu().numpy(), ref["seg"][:, sadr : sadr + w * h].reshape(2, h, w, 2))
