@@ -238,7 +238,24 @@ def settings_struct(rc: PbrRenderContext):
   return s
 
 
+def _check_output_buffers(rc: PbrRenderContext) -> None:
+  """The kernels write HDR, guides and denoiser features for every RGB pixel when the matching
+  flag is set; a flag switched on after the context was created (its buffer then holds one
+  pixel a world) is refused here instead of being written past."""
+  base = rc.base
+  npix = int(base.rgb_data.shape[1])
+  need = [("render_hdr" if rc.render_hdr else "denoise", bool(rc.render_hdr) or bool(rc.denoise), rc.hdr_data)]
+  need.append(("render_guides", bool(rc.render_guides), rc.guide_albedo))
+  need.append(("render_guides", bool(rc.render_guides), rc.guide_normal))
+  if rc.tier == Tier.PATH and rc.denoise:
+    need += [("denoise", True, t) for t in (rc.aov_albedo, rc.aov_normal, rc.aov_depth, rc.aov_emission, rc.aov_variance)]
+  for flag, on, t in need:
+    if on and (int(t.shape[0]) < base.nworld or int(t.shape[1]) < npix):
+      raise ValueError(f"rc.{flag} is set but its output buffers were not allocated for it: create the context with {flag}=True")
+
+
 def outputs_struct(rc: PbrRenderContext):
+  _check_output_buffers(rc)
   S = _abi.structs()
   o = S["MjpbrOutputs"]()
   base = rc.base

@@ -185,3 +185,17 @@ def test_gpu_readback_matches_oracle_buffers(cuda):
     mj.get_segmentation(rc, cam, seg)
     sadr = int(rc.base.seg_adr[cam])
     np.testing.assert_array_equal(seg.cpu().numpy(), ref["seg"][:, sadr : sadr + w * h].reshape(2, h, w, 2))
+
+
+@pytest.mark.parametrize("flag", ["render_hdr", "render_guides", "denoise"])
+def test_flags_switched_on_after_creation_are_refused(flag):
+  """Output buffers are sized when the context is created; switching a flag on afterwards would
+  make the kernels write past them, so every entry point refuses it."""
+  mjm = scenes.primitives_scene(0, 8)
+  kw = {"tier": mj.Tier.PATH, "path_samples": 1} if flag == "denoise" else {}
+  m, d, rc = make(mjm, nworld=2, cam_res=(8, 8), render_hdr=False, **kw)
+  setattr(rc, flag, True)
+  with pytest.raises(ValueError, match=f"rc.{flag} is set but"):
+    oracle.rt_render(m, d, rc) if flag != "denoise" else oracle.pt_render(m, d, rc)
+  with pytest.raises(ValueError, match=f"rc.{flag} is set but"):
+    mj.render(m, d, rc)
