@@ -53,11 +53,6 @@ MJ_DEV V3 normalize(V3 a) {
   float l = sqrtf(dot(a, a));
   return l > 0.0f ? a / l : a;
 }
-// normalize on v_sqrt_f32 / v_rcp_f32 (about 1 ulp each): shading vectors only (half vectors).
-MJ_DEV V3 normalize_hw(V3 a) {
-  float l = __builtin_amdgcn_sqrtf(dot(a, a));
-  return l > 0.0f ? a * __builtin_amdgcn_rcpf(l) : a;
-}
 MJ_DEV V3 vmax(V3 a, float s) { return V3{fmaxf(a.x, s), fmaxf(a.y, s), fmaxf(a.z, s)}; }
 MJ_DEV V3 vmin(V3 a, float s) { return V3{fminf(a.x, s), fminf(a.y, s), fminf(a.z, s)}; }
 MJ_DEV float clampf(float x, float lo, float hi) { return fminf(fmaxf(x, lo), hi); }
@@ -66,9 +61,6 @@ MJ_DEV float clampf(float x, float lo, float hi) { return fminf(fmaxf(x, lo), hi
 // display values only (texture and sky LODs, specular occlusion, the sRGB curve), never for hits.
 MJ_DEV float log2_hw(float x) { return __builtin_amdgcn_logf(x); }
 MJ_DEV float pow_hw(float x, float y) { return __builtin_amdgcn_exp2f(y * __builtin_amdgcn_logf(x)); }
-// Square root on v_sqrt_f32 (about 1 ulp, without the correctly rounded fix-up): shading terms
-// only (BRDF visibility, roughness widening).
-MJ_DEV float sqrt_hw(float x) { return __builtin_amdgcn_sqrtf(x); }
 
 // Row-major 3x3: m[3 i + j].
 MJ_DEV V3 mat_mul(const float* m, V3 v) {

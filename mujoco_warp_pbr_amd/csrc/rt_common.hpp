@@ -88,7 +88,7 @@ MJ_DEV bool light_term(const RtCtx& c, const Light& L, const Lobes& lb, V3 point
   eval_light(L, c.s.light_model, c.s.has_spot != 0, point, l, dist, &ds, &ss);
   *nol = dot(normal, *l);
   if (*nol <= 0.0f || (len_sq(ds) == 0.0f && len_sq(ss) == 0.0f)) return false;
-  V3 h = normalize_hw(*l + view);
+  V3 h = normalize(*l + view);
   float noh = fmaxf(dot(normal, h), 0.0f);
   float loh = fmaxf(dot(*l, h), 0.0f);
   V3 fr = cw(specular_lobe(lb.f0, lb.alpha, noh, nov, *nol, loh), lb.energy_comp);

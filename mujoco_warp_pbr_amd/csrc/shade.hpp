@@ -220,13 +220,6 @@ MJ_DEV float v_smith_ggx_correlated(float nov, float nol, float alpha) {
   float gl = nov * sqrtf(nol * nol * (1.0f - a2) + a2);
   return 0.5f / (gv + gl);
 }
-// The same on the hardware square root, for shading (the DFG table keeps the exact one).
-MJ_DEV float v_smith_ggx_correlated_hw(float nov, float nol, float alpha) {
-  float a2 = alpha * alpha;
-  float gv = nol * sqrt_hw(nov * nov * (1.0f - a2) + a2);
-  float gl = nov * sqrt_hw(nol * nol * (1.0f - a2) + a2);
-  return 0.5f / (gv + gl);
-}
 MJ_DEV V3 f_schlick(V3 f0, float f90, float voh) {
   float x = 1.0f - voh;
   float x2 = x * x;
@@ -234,7 +227,7 @@ MJ_DEV V3 f_schlick(V3 f0, float f90, float voh) {
   return f0 * (1.0f - fc) + v3s(f90 * fc);
 }
 MJ_DEV V3 specular_lobe(V3 f0, float alpha, float noh, float nov, float nol, float loh) {
-  return f_schlick(f0, 1.0f, loh) * (d_ggx(noh, alpha) * v_smith_ggx_correlated_hw(nov, nol, alpha));
+  return f_schlick(f0, 1.0f, loh) * (d_ggx(noh, alpha) * v_smith_ggx_correlated(nov, nol, alpha));
 }
 
 MJ_DEV Lobes shading_lobes(const MjpbrEnvironment& env, const Surface& s, float nov) {
@@ -428,7 +421,7 @@ MJ_DEV uint32_t encode_display(V3 c) {
 MJ_DEV float widen_roughness(float pr, float a2_extra) {
   pr = clampf(pr, 0.0f, 1.0f);
   float a2 = fminf(pr * pr * pr * pr + a2_extra, 1.0f);
-  return sqrt_hw(sqrt_hw(a2));
+  return sqrtf(sqrtf(a2));
 }
 
 // MuJoCo Warp's _geom_rgba: a geom with a material and MuJoCo's default rgba shows the
