@@ -379,6 +379,33 @@ int mjpbr_dfg_lut(float* lut, int size, int nsample, void* stream);
 int mjpbr_env_prefilter(const uint16_t* radiance, const int64_t* level_adr, const int32_t* desc,
                         uint16_t* stack, int levels, int height, int width, int nsample, void* stream);
 
+/* Context tables built on the GPU from uploaded images. Every table is a pure function of its
+ * image: reductions run in double precision in a fixed order, mip levels cascade in double from
+ * the unrounded level above (columns averaged first, then rows; an axis of one texel is kept) and
+ * each level is rounded once into its format (RGBA8: floor(v + 0.5); RGBA16F: round to nearest
+ * even). `work` is caller-provided device memory of `work_doubles` doubles.
+ *
+ * Texture mip chain (reference texture.py:63-101): level 0 from RGBA8 texels, kept RGBA8, or
+ * (format RGBA16F) colour decoded through srgb_table (256 f16 values) with alpha / 255; levels are
+ * written at pool + level_byte_adr[l] (host array of nlevels byte offsets). work: 8 w h. */
+int mjpbr_texture_mips(const uint8_t* rgba8, int width, int height, int format, const uint16_t* srgb_table, uint8_t* pool,
+                       const int64_t* level_byte_adr, int nlevels, double* work, int64_t work_doubles, void* stream);
+/* Radiance mip chain of an environment map (reference envmap.py:366-399): RGB float32 (height x
+ * width x 3) -> RGBA16F levels (alpha 1) at pool + level_half_adr[l] halves (host array). work: 8 w h. */
+int mjpbr_radiance_mips(const float* rgb, int width, int height, uint16_t* pool, const int64_t* level_half_adr, int nlevels,
+                        double* work, int64_t work_doubles, void* stream);
+/* sh[9][3]: the radiance projected on the real SH basis (solid-angle weighted), each band times
+ * its Lambertian convolution over pi (1, 2/3, 1/4), so that evaluating it gives the cosine-weighted
+ * mean radiance around a normal (reference envmap.py:224-249, Ramamoorthi & Hanrahan 2001).
+ * work: 27 height. */
+int mjpbr_env_sh9(const float* rgb, int width, int height, double* work, int64_t work_doubles, float* sh, void* stream);
+/* Importance-sampling tables of an environment map (reference envmap.py:402-430): the map box
+ * filtered to at most cdf_width texels wide (w x h), p proportional to luminance sin(theta) +
+ * 1e-6 mean luminance; prob [h][w], marginal [h + 1] and conditional [h][w + 1] CDFs (last entries
+ * exactly 1). work: w h + h. */
+int mjpbr_env_sampling_tables(const float* rgb, int width, int height, int cdf_width, double* work, int64_t work_doubles, float* prob,
+                              float* marginal, float* conditional, void* stream);
+
 /* get_hdr / get_albedo / get_normal extraction of one camera into (n, h, w, 3)
  * (reference output.py:55-67). */
 int mjpbr_extract_rgb_image(const float* src, int src_stride, int adr, int width, int height, int nout,

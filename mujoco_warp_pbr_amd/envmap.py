@@ -1,16 +1,20 @@
-"""Environment maps: RGBE loading, radiance mip chains, GGX-prefiltered stacks (on the GPU),
-SH9 irradiance and importance-sampling tables (reference envmap.py:57-576).
+"""Environment maps (reference envmap.py:57-576): Radiance RGBE reading, and the device tables the
+renderer samples, all built on the GPU from the uploaded radiance image:
 
-Per map we keep the radiance as an RGBA16F mip chain (background, sharp reflections), an
-8-level GGX-prefiltered stack whose depth axis is perceptual roughness, and 9 SH coefficients
-of the Lambertian radiance. All maps share the prefiltered stack and sampling-table shapes.
+- an RGBA16F radiance mip chain (background, sharp reflections);
+- an 8-level GGX-prefiltered stack whose depth axis is perceptual roughness (split-sum IBL);
+- nine SH coefficients of the Lambertian-convolved radiance (diffuse IBL);
+- piecewise-constant importance-sampling tables (path tracer).
+
+All maps of a context share the prefiltered-stack and sampling-table shapes. A context created
+with precompute=False (host-side harnesses on device="cpu") gets the buffers zero-filled.
 """
 
 from __future__ import annotations
 
+import ctypes
 import dataclasses
 import math
-import re
 from pathlib import Path
 
 import numpy as np
@@ -18,7 +22,6 @@ import torch
 
 from mujoco_warp_pbr_amd import _abi
 from mujoco_warp_pbr_amd import model as mdl
-from mujoco_warp_pbr_amd.material import LUMINANCE_WEIGHTS
 
 PREFILTER_WIDTH = 256
 PREFILTER_LEVELS = 8
@@ -28,67 +31,95 @@ CDF_WIDTH = 256
 MAX_LEVELS = 16
 
 
-def _decode_rle_scanline(data: bytes, pos: int, width: int, path: str, y: int):
-  """Adaptive run-length encoded RGBE scanline at `pos` -> ((width, 4) uint8, end)."""
-  line = bytearray(4 * width)
-  n = len(data)
-  for c in range(4):
-    x = c * width
-    end = x + width
-    while x < end:
-      if pos >= n:
-        raise ValueError(f"{path}: truncated run-length data in scanline {y}")
-      count = data[pos]
-      pos += 1
-      if count > 128:
-        count -= 128
-        if count > end - x or pos >= n:
-          raise ValueError(f"{path}: bad run-length data in scanline {y}")
-        line[x : x + count] = bytes((data[pos],)) * count
-        pos += 1
+# ----------------------------------------------------------------------------- RGBE files
+class _RgbeFile:
+  """A Radiance picture held in memory: the '#?' signature, header lines up to a blank line, a
+  '-Y H +X W' resolution line, then H scanlines of W RGBE pixels, each either flat (4 bytes a
+  pixel) or in the adaptive run-length form (2, 2, W >> 8, W & 255, then the R, G, B and E planes
+  as packets: n > 128 repeats the next byte n - 128 times, 0 < n <= 128 copies n literal bytes).
+  Error messages are those of the reference's loader (its tests pin them)."""
+
+  def __init__(self, path):
+    self.name = str(path)
+    self.buf = memoryview(Path(path).read_bytes())
+    self.pos = 0
+
+  def _error(self, what: str) -> ValueError:
+    return ValueError(f"{self.name}: {what}")
+
+  def resolution(self) -> tuple[int, int]:
+    raw = self.buf.tobytes()
+    if not raw.startswith(b"#?"):
+      raise self._error("not a Radiance .hdr file (missing '#?' signature)")
+    blank = raw.find(b"\n\n")
+    eol = raw.find(b"\n", blank + 2) if blank >= 0 else -1
+    if blank < 0 or eol < 0:
+      raise self._error("truncated .hdr header")
+    fields = raw[blank + 2 : eol].split()
+    if len(fields) != 4 or fields[0] != b"-Y" or fields[2] != b"+X" or not (fields[1].isdigit() and fields[3].isdigit()):
+      raise self._error("only '-Y H +X W' oriented .hdr files are supported")
+    self.pos = eol + 1
+    return int(fields[1]), int(fields[3])
+
+  def _run_length_start(self, width: int) -> bool:
+    head = self.buf[self.pos : self.pos + 3]
+    return 8 <= width <= 0x7FFF and len(head) == 3 and head[0] == 2 and head[1] == 2 and not head[2] & 0x80
+
+  def _flat(self, y: int, width: int) -> np.ndarray:
+    end = self.pos + 4 * width
+    if end > len(self.buf):
+      raise self._error(f"truncated pixel data in scanline {y}")
+    px = np.frombuffer(self.buf[self.pos : end], np.uint8).reshape(width, 4)
+    self.pos = end
+    return px
+
+  def _plane(self, y: int, width: int) -> np.ndarray:
+    plane = np.empty(width, np.uint8)
+    filled, buf, size = 0, self.buf, len(self.buf)
+    while filled < width:
+      if self.pos >= size:
+        raise self._error(f"truncated run-length data in scanline {y}")
+      code = buf[self.pos]
+      self.pos += 1
+      run = code - 128 if code > 128 else code
+      repeat = code > 128
+      if run == 0 or run > width - filled or self.pos + (1 if repeat else run) > size:
+        raise self._error(f"bad run-length data in scanline {y}")
+      if repeat:
+        plane[filled : filled + run] = buf[self.pos]
+        self.pos += 1
       else:
-        if count == 0 or count > end - x or pos + count > n:
-          raise ValueError(f"{path}: bad run-length data in scanline {y}")
-        line[x : x + count] = data[pos : pos + count]
-        pos += count
-      x += count
-  return np.frombuffer(bytes(line), dtype=np.uint8).reshape(4, width).T, pos
+        plane[filled : filled + run] = np.frombuffer(buf[self.pos : self.pos + run], np.uint8)
+        self.pos += run
+      filled += run
+    return plane
+
+  def _run_length(self, y: int, width: int) -> np.ndarray:
+    stated = (self.buf[self.pos + 2] << 8) | self.buf[self.pos + 3] if self.pos + 3 < len(self.buf) else -1
+    if stated != width:
+      raise self._error(f"scanline {y} width does not match the header")
+    self.pos += 4
+    return np.stack([self._plane(y, width) for _ in range(4)], axis=1)
+
+  def pixels(self) -> np.ndarray:
+    height, width = self.resolution()
+    out = np.empty((height, width, 4), np.uint8)
+    for y in range(height):
+      out[y] = self._run_length(y, width) if self._run_length_start(width) else self._flat(y, width)
+    return out
 
 
 def load_hdr(path) -> np.ndarray:
-  """Reads a Radiance RGBE (.hdr) file as float32 (height, width, 3) (reference envmap.py:106-142).
+  """Reads a Radiance RGBE (.hdr) file as float32 (height, width, 3) radiance (reference
+  envmap.py:106-142): channel value = byte * 2^(E - 136), zero where E = 0.
 
   Raises:
     ValueError: not a '-Y H +X W' RGBE file, or truncated / malformed pixel data.
   """
-  data = Path(path).read_bytes()
-  if not data.startswith(b"#?"):
-    raise ValueError(f"{path}: not a Radiance .hdr file (missing '#?' signature)")
-  try:
-    header_end = data.index(b"\n\n") + 2
-    resolution_end = data.index(b"\n", header_end)
-  except ValueError:
-    raise ValueError(f"{path}: truncated .hdr header") from None
-  match = re.match(rb"-Y (\d+) \+X (\d+)", data[header_end:resolution_end])
-  if match is None:
-    raise ValueError(f"{path}: only '-Y H +X W' oriented .hdr files are supported")
-  height, width = int(match.group(1)), int(match.group(2))
-  pos = resolution_end + 1
-  rgbe = np.empty((height, width, 4), dtype=np.uint8)
-  for y in range(height):
-    rle = 8 <= width <= 0x7FFF and data[pos : pos + 2] == b"\x02\x02" and pos + 2 < len(data) and not data[pos + 2] & 0x80
-    if not rle:
-      if pos + 4 * width > len(data):
-        raise ValueError(f"{path}: truncated pixel data in scanline {y}")
-      rgbe[y] = np.frombuffer(data, dtype=np.uint8, count=4 * width, offset=pos).reshape(width, 4)
-      pos += 4 * width
-      continue
-    if (data[pos + 2] << 8 | data[pos + 3]) != width:
-      raise ValueError(f"{path}: scanline {y} width does not match the header")
-    rgbe[y], pos = _decode_rle_scanline(data, pos + 4, width, str(path), y)
-  exponent = rgbe[..., 3].astype(np.int32)
-  scale = np.where(exponent > 0, np.ldexp(1.0, exponent - 136), 0.0).astype(np.float32)
-  return rgbe[..., :3].astype(np.float32) * scale[..., None]
+  rgbe = _RgbeFile(path).pixels()
+  e = rgbe[..., 3:4].astype(np.int32)
+  scale = np.ldexp(np.float32(1.0), e - 136).astype(np.float32) * (e > 0)
+  return rgbe[..., :3].astype(np.float32) * scale
 
 
 def save_hdr(path, image: np.ndarray) -> None:
@@ -105,84 +136,32 @@ def save_hdr(path, image: np.ndarray) -> None:
   Path(path).write_bytes(header + rgbe.tobytes())
 
 
-def downsample(image: np.ndarray, max_width: int) -> np.ndarray:
-  """Box-filter halvings until the width is at most max_width (envmap.py:145-151)."""
-  while image.shape[1] > max_width:
-    h, w = image.shape[0] // 2 * 2, image.shape[1] // 2 * 2
-    image = image[:h, :w].reshape(h // 2, 2, w // 2, 2, -1).mean(axis=(1, 3))
-  return image
+# ----------------------------------------------------------------------------- device tables
+def halve_to_width(image: np.ndarray, max_width: int) -> np.ndarray:
+  """2x2 block means (odd trailing rows / columns dropped) until the width is at most max_width
+  (the reference loads maps wider than 2048 texels this way, envmap.py:145-151)."""
+  img = np.asarray(image)
+  while img.shape[1] > max_width:
+    h2, w2 = img.shape[0] // 2, img.shape[1] // 2
+    quads = img[: 2 * h2, : 2 * w2].astype(np.float64)
+    img = 0.25 * (quads[0::2, 0::2] + quads[0::2, 1::2] + quads[1::2, 0::2] + quads[1::2, 1::2])
+  return img
 
 
-def sh9_irradiance(image: np.ndarray) -> np.ndarray:
-  """SH9 coefficients (9, 3) of the Lambertian radiance of an equirect image (envmap.py:224-249)."""
-  h, w, _ = image.shape
-  theta = (np.arange(h) + 0.5) / h * np.pi
-  phi = ((np.arange(w) + 0.5) / w - 0.5) * 2.0 * np.pi
-  st, ct = np.sin(theta)[:, None], np.cos(theta)[:, None]
-  x, y, z = st * np.cos(phi)[None], st * np.sin(phi)[None], np.broadcast_to(ct, (h, w))
-  basis = np.stack(
-    [
-      np.full((h, w), 0.282095),
-      0.488603 * y,
-      0.488603 * z,
-      0.488603 * x,
-      1.092548 * x * y,
-      1.092548 * y * z,
-      0.315392 * (3.0 * z * z - 1.0),
-      1.092548 * x * z,
-      0.546274 * (x * x - y * y),
-    ]
-  )
-  solid_angle = (2.0 * np.pi / w) * (np.pi / h) * np.broadcast_to(st, (h, w))
-  coeffs = np.einsum("khw,hwc->kc", basis * solid_angle, image.astype(np.float64))
-  band = np.array([np.pi, 2 * np.pi / 3, 2 * np.pi / 3, 2 * np.pi / 3] + [np.pi / 4] * 5)
-  return (coeffs * band[:, None] / np.pi).astype(np.float32)
+def mip_shapes(width: int, height: int) -> list[tuple[int, int]]:
+  """(width, height) of every level of a mip chain down to 1 x 1 (an axis of one texel stays)."""
+  shapes = [(width, height)]
+  while shapes[-1] != (1, 1) and len(shapes) < MAX_LEVELS:
+    w, h = shapes[-1]
+    shapes.append((max(w // 2, 1), max(h // 2, 1)))
+  return shapes
 
 
-def sampling_tables(image: np.ndarray, width: int = CDF_WIDTH):
-  """(prob (h, w), marginal (h + 1,), conditional (h, w + 1)) of the distribution over equirect
-  cells proportional to luminance * sin(theta) (envmap.py:402-430)."""
-  small = downsample(np.asarray(image, dtype=np.float64)[..., :3], width)
-  h = small.shape[0]
-  lum = small @ np.array(LUMINANCE_WEIGHTS)
-  sin_theta = np.sin((np.arange(h) + 0.5) / h * np.pi)[:, None]
-  weight = lum * sin_theta + 1e-6 * max(float(lum.mean()), 1e-12)
-  prob = weight / weight.sum()
-  row = prob.sum(axis=1)
-  marginal = np.concatenate([[0.0], np.cumsum(row)])
-  conditional = np.concatenate([np.zeros((h, 1)), np.cumsum(prob, axis=1) / np.maximum(row[:, None], 1e-30)], axis=1)
-  marginal[-1] = 1.0
-  conditional[:, -1] = 1.0
-  return prob.astype(np.float32), marginal.astype(np.float32), conditional.astype(np.float32)
-
-
-def _downsampled_shape(shape, max_width: int):
-  h, w = shape[0], shape[1]
-  while w > max_width:
-    h, w = h // 2, w // 2
-  return h, w
-
-
-def _resize_nearest(image: np.ndarray, shape) -> np.ndarray:
-  h, w = shape
-  ys = (np.arange(h) + 0.5) / h * image.shape[0]
-  xs = (np.arange(w) + 0.5) / w * image.shape[1]
-  return image[ys.astype(int)][:, xs.astype(int)]
-
-
-def radiance_levels(image: np.ndarray) -> list[np.ndarray]:
-  """RGBA16F mip chain of a radiance image (alpha 1), 2x2 box filtered down to 1x1."""
-  from mujoco_warp_pbr_amd.texture import downsample2
-
-  h, w, _ = image.shape
-  cur = np.concatenate([image.astype(np.float64), np.ones((h, w, 1))], axis=-1).astype(np.float16).astype(np.float64)
-  levels = []
-  while True:
-    levels.append(cur.astype(np.float16))
-    if (cur.shape[0] == 1 and cur.shape[1] == 1) or len(levels) == MAX_LEVELS:
-      break
-    cur = downsample2(cur)
-  return levels
+def table_shape(width: int, height: int, cdf_width: int = CDF_WIDTH) -> tuple[int, int]:
+  """(height, width) of the sampling tables of a map: halved while wider than cdf_width."""
+  while width > cdf_width:
+    width, height = width // 2, height // 2
+  return height, width
 
 
 @dataclasses.dataclass
@@ -190,105 +169,135 @@ class Environments:
   """Device resources of the environment maps (reference envmap.py:493-515)."""
 
   nenv: int
-  radiance: torch.Tensor  # uint16 (f16 bits) pool
+  radiance: torch.Tensor  # uint16 (f16 bits) pool of every map's mip chain
   level_adr: torch.Tensor  # (nenv, MAX_LEVELS) int64, offsets in halves
-  desc: torch.Tensor  # (nenv, 4) int32: width, height, nlevels, 0
+  desc: torch.Tensor  # (nenv, 4) int32: width, height, nlevels, format (1: RGBA16F)
   prefiltered: torch.Tensor  # (nenv, L, H, W, 4) f16 bits
   sh: torch.Tensor  # (nenv, 9, 3)
   width: float
-  prob: torch.Tensor
-  marginal: torch.Tensor
-  conditional: torch.Tensor
-  images: list
+  prob: torch.Tensor  # (nenv, h, w)
+  marginal: torch.Tensor  # (nenv, h + 1)
+  conditional: torch.Tensor  # (nenv, h, w + 1)
+  images: list  # the float32 radiance images the tables were built from
+  table_images: list  # the images the sampling tables were built from (resized to one shape)
 
   @property
   def pf_shape(self):
     return tuple(self.prefiltered.shape[1:4])
 
 
-def _upload_radiance(images: list[np.ndarray], device):
-  chunks, offset = [], 0
-  level_adr = np.zeros((len(images), MAX_LEVELS), np.int64)
-  desc = np.zeros((len(images), 4), np.int32)
-  for e, im in enumerate(images):
-    levels = radiance_levels(im)
-    desc[e] = (im.shape[1], im.shape[0], len(levels), 1)  # format 1: RGBA16F
-    for lv, img in enumerate(levels):
-      raw = np.ascontiguousarray(img).view(np.uint16).ravel()
+def _resample_nearest(image: np.ndarray, height: int, width: int) -> np.ndarray:
+  rows = np.minimum(((np.arange(height) + 0.5) * image.shape[0] / height).astype(int), image.shape[0] - 1)
+  cols = np.minimum(((np.arange(width) + 0.5) * image.shape[1] / width).astype(int), image.shape[1] - 1)
+  return image[rows][:, cols]
+
+
+def build_environments(images: list[np.ndarray], device, nsample: int = PREFILTER_SAMPLES, precompute: bool = True) -> Environments:
+  """Device resources of equirect radiance maps. With precompute (a GPU device) the radiance mip
+  chains, SH9 coefficients, sampling tables and prefiltered stacks are built by HIP kernels from
+  the uploaded images; otherwise they are left zero for the caller to fill."""
+  dev = torch.device(device)
+  imgs = [np.ascontiguousarray(halve_to_width(np.asarray(im)[..., :3], MAX_RADIANCE_WIDTH), np.float32) for im in images]
+  nenv = len(imgs)
+  # sampling tables share the first map's shape; other maps are resampled to it first
+  th, tw = table_shape(imgs[0].shape[1], imgs[0].shape[0])
+  table_imgs = [im if table_shape(im.shape[1], im.shape[0]) == (th, tw) else _resample_nearest(im, th, tw) for im in imgs]
+  # radiance pool layout
+  level_adr = np.zeros((nenv, MAX_LEVELS), np.int64)
+  desc = np.zeros((nenv, 4), np.int32)
+  offset = 0
+  for e, im in enumerate(imgs):
+    shapes = mip_shapes(im.shape[1], im.shape[0])
+    desc[e] = (im.shape[1], im.shape[0], len(shapes), 1)
+    for lv, (w, h) in enumerate(shapes):
       level_adr[e, lv] = offset
-      chunks.append(raw)
-      offset += raw.size
-  pool = np.concatenate(chunks)
-  return (
-    torch.as_tensor(pool.view(np.int16)).to(device),
-    torch.as_tensor(level_adr).to(device),
-    torch.as_tensor(desc).to(device),
+      offset += 4 * w * h
+  pw = max(min(PREFILTER_WIDTH, min(im.shape[1] for im in imgs)), 2)
+  env = Environments(
+    nenv=nenv,
+    radiance=torch.zeros(offset, dtype=torch.int16, device=dev),
+    level_adr=torch.as_tensor(level_adr).to(dev),
+    desc=torch.as_tensor(desc).to(dev),
+    prefiltered=torch.zeros((nenv, PREFILTER_LEVELS, pw // 2, pw, 4), dtype=torch.int16, device=dev),
+    sh=torch.zeros((nenv, 9, 3), dtype=torch.float32, device=dev),
+    width=float(max(im.shape[1] for im in imgs)),
+    prob=torch.zeros((nenv, th, tw), dtype=torch.float32, device=dev),
+    marginal=torch.zeros((nenv, th + 1), dtype=torch.float32, device=dev),
+    conditional=torch.zeros((nenv, th, tw + 1), dtype=torch.float32, device=dev),
+    images=imgs,
+    table_images=table_imgs,
   )
+  if precompute:
+    _build_on_device(env, level_adr, nsample)
+  return env
 
 
-def build_environments(images: list[np.ndarray], device, nsample: int = PREFILTER_SAMPLES, prefilter: bool = True) -> Environments:
-  """GPU resources of a list of equirect radiance maps; prefiltering runs on the GPU
-  (prefilter=False leaves the stacks zero for a caller that fills them)."""
-  imgs = [downsample(np.asarray(im, dtype=np.float32)[..., :3], MAX_RADIANCE_WIDTH) for im in images]
-  radiance, level_adr, desc = _upload_radiance(imgs, device)
-  pw = min(PREFILTER_WIDTH, min(im.shape[1] for im in imgs))
-  pw = max(pw, 2)
-  stack = torch.zeros((len(imgs), PREFILTER_LEVELS, pw // 2, pw, 4), dtype=torch.int16, device=device)
-  stream = _stream(device)
-  for e in range(len(imgs) if prefilter else 0):
+def _build_on_device(env: Environments, level_adr: np.ndarray, nsample: int) -> None:
+  dev = env.radiance.device
+  stream = _stream(dev)
+  largest = max(im.shape[0] * im.shape[1] for im in env.images + env.table_images)
+  maxh = max(im.shape[0] for im in env.images)
+  work = torch.empty(max(8 * largest, 27 * maxh, 1), dtype=torch.float64, device=dev)
+  base = env.radiance.data_ptr()
+  for e, (im, tim) in enumerate(zip(env.images, env.table_images)):
+    h, w = im.shape[:2]
+    nlevels = int(env.desc[e, 2])
+    src = torch.as_tensor(im).to(dev)
+    adr = (ctypes.c_int64 * nlevels)(*(int(x) for x in level_adr[e, :nlevels]))  # read on the host by the launcher
+    _abi.call("mjpbr_radiance_mips", _abi.ptr(src), w, h, base, ctypes.cast(adr, ctypes.c_void_p), nlevels, _abi.ptr(work), work.numel(),
+              stream)
+    _abi.call("mjpbr_env_sh9", _abi.ptr(src), w, h, _abi.ptr(work), work.numel(), _abi.ptr(env.sh[e]), stream)
+    tsrc = src if tim is im else torch.as_tensor(np.ascontiguousarray(tim, np.float32)).to(dev)
+    _abi.call(
+      "mjpbr_env_sampling_tables",
+      _abi.ptr(tsrc),
+      int(tim.shape[1]),
+      int(tim.shape[0]),
+      CDF_WIDTH,
+      _abi.ptr(work),
+      work.numel(),
+      _abi.ptr(env.prob[e]),
+      _abi.ptr(env.marginal[e]),
+      _abi.ptr(env.conditional[e]),
+      stream,
+    )
+  L, H, W = env.pf_shape
+  for e in range(env.nenv):
     _abi.call(
       "mjpbr_env_prefilter",
-      _abi.ptr(radiance),
-      _abi.ptr(level_adr[e]),
-      _abi.ptr(desc[e]),
-      _abi.ptr(stack[e]),
-      PREFILTER_LEVELS,
-      pw // 2,
-      pw,
+      _abi.ptr(env.radiance),
+      _abi.ptr(env.level_adr[e]),
+      _abi.ptr(env.desc[e]),
+      _abi.ptr(env.prefiltered[e]),
+      L,
+      H,
+      W,
       nsample,
       stream,
     )
-  shape = None
-  tables = []
-  for im in imgs:
-    if shape is not None and _downsampled_shape(im.shape, CDF_WIDTH) != shape:
-      im = _resize_nearest(im, shape)
-    t = sampling_tables(im)
-    shape = shape or t[0].shape
-    tables.append(t)
-  return Environments(
-    nenv=len(imgs),
-    radiance=radiance,
-    level_adr=level_adr,
-    desc=desc,
-    prefiltered=stack,
-    sh=torch.as_tensor(np.stack([sh9_irradiance(im) for im in imgs])).to(device),
-    width=float(max(im.shape[1] for im in imgs)),
-    prob=torch.as_tensor(np.stack([t[0] for t in tables])).to(device),
-    marginal=torch.as_tensor(np.stack([t[1] for t in tables])).to(device),
-    conditional=torch.as_tensor(np.stack([t[2] for t in tables])).to(device),
-    images=imgs,
-  )
 
 
 def empty_environments(device) -> Environments:
   """Placeholder resources of a context without environment map (constant ambient)."""
-  radiance, level_adr, desc = _upload_radiance([np.zeros((1, 1, 3), np.float32)], device)
+  dev = torch.device(device)
+  one = np.zeros((1, 1, 3), np.float32)
   return Environments(
     nenv=1,
-    radiance=radiance,
-    level_adr=level_adr,
-    desc=desc,
-    prefiltered=torch.zeros((1, 2, 1, 2, 4), dtype=torch.int16, device=device),
-    sh=torch.zeros((1, 9, 3), dtype=torch.float32, device=device),
+    radiance=torch.tensor([0, 0, 0, 15360], dtype=torch.int16, device=dev),  # one texel (0, 0, 0, 1)
+    level_adr=torch.zeros((1, MAX_LEVELS), dtype=torch.int64, device=dev),
+    desc=torch.tensor([[1, 1, 1, 1]], dtype=torch.int32, device=dev),
+    prefiltered=torch.zeros((1, 2, 1, 2, 4), dtype=torch.int16, device=dev),
+    sh=torch.zeros((1, 9, 3), dtype=torch.float32, device=dev),
     width=1.0,
-    prob=torch.ones((1, 1, 1), dtype=torch.float32, device=device),
-    marginal=torch.tensor([[0.0, 1.0]], dtype=torch.float32, device=device),
-    conditional=torch.tensor([[[0.0, 1.0]]], dtype=torch.float32, device=device),
+    prob=torch.ones((1, 1, 1), dtype=torch.float32, device=dev),
+    marginal=torch.tensor([[0.0, 1.0]], dtype=torch.float32, device=dev),
+    conditional=torch.tensor([[[0.0, 1.0]]], dtype=torch.float32, device=dev),
     images=[],
+    table_images=[],
   )
 
 
+# ----------------------------------------------------------------------------- skybox
 SKYBOX_WIDTH = 1024
 
 

@@ -148,9 +148,9 @@ def create_render_context(
   with the a-trous filter as fallback for worlds outside its training range. The learned
   denoiser needs camera sides that are multiples of 64.
 
-  `precompute=False` leaves the GPU-built tables (DFG LUT, prefiltered environment stacks)
-  zero; it exists so host-side test harnesses can create contexts on device="cpu" and fill the
-  tables themselves.
+  `precompute=False` leaves the GPU-built tables (DFG LUT, texture and radiance mip chains,
+  prefiltered environment stacks, SH9 coefficients, sampling tables) zero; it exists so host-side
+  test harnesses can create contexts on device="cpu" and fill the tables themselves.
 
   Raises:
     ValueError: for invalid arguments, or a model with nothing to render.
@@ -318,7 +318,7 @@ def create_render_context(
   if features.textures:
     ids = mjm.mat_texid[rendered_mats].ravel()
     used_tex[ids[ids >= 0]] = True
-  textures = texture.upload_textures(mjm, used_tex, dev)
+  textures = texture.upload_textures(mjm, used_tex, dev, precompute=precompute)
   alpha = material.effective_geom_rgba(mjm)[rendered, 3]
   has_transparency = bool(np.any((alpha > 0.0) & (alpha < 1.0))) or features.alpha_map
   has_reflection = enable_reflections and bool(mjm.nmat and np.any(mjm.mat_reflectance[rendered_mats] > 0.0))
@@ -334,7 +334,7 @@ def create_render_context(
       images.append(sky)
       skybox_only = not physical
   if images:
-    env = envmap_lib.build_environments(images, dev, prefilter=precompute)
+    env = envmap_lib.build_environments(images, dev, precompute=precompute)
     has_env, any_env = not skybox_only, True
   else:
     env = envmap_lib.empty_environments(dev)

@@ -1,19 +1,21 @@
-"""Material textures: decoding and mip chains in one device byte pool (reference
-texture.py:41-101).
+"""Material textures (reference texture.py:41-101): one device byte pool holding every used 2D
+texture's full mip chain, built on the GPU from the uploaded RGBA8 texels.
 
-sRGB (colour-role) textures become linear float16 RGBA (filtering and mips are then linear);
-data textures (normal, ORM, ...) stay normalised RGBA8. Each texture gets a full mip chain
-made by 2x2 box filtering. Level l of texture t starts at byte `level_adr[t, l]`;
-`desc[t] = (width, height, nlevels, format)` with format 0 = RGBA8, 1 = RGBA16F.
+Colour-role (sRGB) textures become linear RGBA16F, so filtering and mips are linear; data textures
+(normal, ORM, ...) stay normalised RGBA8. Level l of texture t starts at byte `level_adr[t, l]`
+(16-byte aligned); `desc[t] = (width, height, nlevels, format)` with format 0 = RGBA8,
+1 = RGBA16F. Unused and non-2D textures hold one white RGBA8 texel.
 """
 
 from __future__ import annotations
 
+import ctypes
 import dataclasses
 
 import numpy as np
 import torch
 
+from mujoco_warp_pbr_amd import _abi
 from mujoco_warp_pbr_amd import model as mdl
 from mujoco_warp_pbr_amd.material import srgb_to_linear
 
@@ -21,6 +23,7 @@ MAX_LEVELS = 16
 FORMAT_RGBA8 = 0
 FORMAT_RGBA16F = 1
 _COLOR_ROLES = [mdl.ROLE_RGB, mdl.ROLE_RGBA, mdl.ROLE_EMISSIVE]
+# sRGB byte -> linear f16 (the decode the GPU applies to colour textures' level 0)
 SRGB_TABLE = srgb_to_linear(np.arange(256) / 255.0).astype(np.float16)
 
 
@@ -34,33 +37,22 @@ def is_srgb(mjm: mdl.HostModel, tex_id: int) -> bool:
   return bool(np.isin(tex_id, mjm.mat_texid[:, _COLOR_ROLES])) if mjm.nmat else False
 
 
-def downsample2(image: np.ndarray) -> np.ndarray:
-  """One mip step: 2x2 box filter (a 1-texel axis is kept), computed in float64."""
-  h, w = image.shape[:2]
-  img = image.astype(np.float64)
-  if w > 1:
-    w2 = w // 2
-    img = 0.5 * (img[:, 0 : 2 * w2 : 2] + img[:, 1 : 2 * w2 : 2])
-  if h > 1:
-    h2 = h // 2
-    img = 0.5 * (img[0 : 2 * h2 : 2] + img[1 : 2 * h2 : 2])
-  return img
+def level_shapes(width: int, height: int) -> list[tuple[int, int]]:
+  """(width, height) of each mip level down to 1 x 1; an axis of one texel stays one texel."""
+  shapes = [(width, height)]
+  while shapes[-1] != (1, 1) and len(shapes) < MAX_LEVELS:
+    w, h = shapes[-1]
+    shapes.append((max(w // 2, 1), max(h // 2, 1)))
+  return shapes
 
 
-def mip_chain(image: np.ndarray, fmt: int) -> list[np.ndarray]:
-  """Full mip chain down to 1x1, each level stored in `fmt` (uint8 RGBA or float16 RGBA)."""
-  levels = []
-  cur = image.astype(np.float64)
-  while True:
-    if fmt == FORMAT_RGBA8:
-      levels.append(np.clip(np.floor(cur + 0.5), 0, 255).astype(np.uint8))
-    else:
-      levels.append(cur.astype(np.float16))
-    h, w = cur.shape[:2]
-    if (h == 1 and w == 1) or len(levels) == MAX_LEVELS:
-      break
-    cur = downsample2(cur)
-  return levels
+@dataclasses.dataclass
+class TextureSource:
+  """Level-0 texels of one pooled texture (what the mip chain is built from)."""
+
+  tex_id: int
+  rgba8: np.ndarray  # (h, w, 4) uint8
+  fmt: int
 
 
 @dataclasses.dataclass
@@ -70,55 +62,85 @@ class TexturePool:
   level_adr: torch.Tensor  # (ntex, MAX_LEVELS) int64 byte offsets
   desc: torch.Tensor  # (ntex, 4) int32
   size: np.ndarray  # (ntex, 2) float32 (width, height)
+  sources: list  # TextureSource of every pooled 2D texture
+  host_level_adr: np.ndarray  # level_adr on the host
 
 
 def _rgba8(mjm: mdl.HostModel, t: int) -> np.ndarray:
   w, h, nc = int(mjm.tex_width[t]), int(mjm.tex_height[t]), int(mjm.tex_nchannel[t])
-  data = mjm.tex_data[mjm.tex_adr[t] : mjm.tex_adr[t] + w * h * nc].reshape(h, w, nc)
-  rgba = np.full((h, w, 4), 255, dtype=np.uint8)
+  texels = np.asarray(mjm.tex_data[mjm.tex_adr[t] : mjm.tex_adr[t] + w * h * nc], np.uint8).reshape(h, w, nc)
+  out = np.full((h, w, 4), 255, np.uint8)
+  out[..., : min(nc, 4)] = texels[..., :4]
   if nc == 1:
-    rgba[..., :3] = data
-  else:
-    rgba[..., :nc] = data
-  return rgba
+    out[..., 1:3] = texels
+  return out
 
 
-def upload_textures(mjm: mdl.HostModel, used: np.ndarray, device) -> TexturePool:
-  """Pool of mipmapped textures for the 2D textures in `used`; others get a 1x1 white texel."""
-  chunks: list[np.ndarray] = []
-  offset = 0
+def upload_textures(mjm: mdl.HostModel, used: np.ndarray, device, precompute: bool = True) -> TexturePool:
+  """Pool of mipmapped textures for the 2D textures in `used` (GPU-built mips with precompute;
+  otherwise the pooled textures' levels are left zero for the caller to fill)."""
   ntex = max(mjm.ntex, 1)
   level_adr = np.zeros((ntex, MAX_LEVELS), np.int64)
   desc = np.zeros((ntex, 4), np.int32)
   size = np.ones((ntex, 2), np.float32)
+  sources: list[TextureSource] = []
+  white: list[int] = []
+  offset = 0
   for t in range(ntex):
-    if t >= mjm.ntex or not used[t] or mjm.tex_type[t] != mdl.TEXTURE_2D:
-      levels, fmt = [np.full((1, 1, 4), 255, np.uint8)], FORMAT_RGBA8
-    else:
+    pooled = t < mjm.ntex and bool(used[t]) and mjm.tex_type[t] == mdl.TEXTURE_2D
+    if pooled:
       rgba = _rgba8(mjm, t)
-      if is_srgb(mjm, t):
-        texels = np.concatenate([SRGB_TABLE[rgba[..., :3]].astype(np.float64), rgba[..., 3:] / 255.0], axis=-1)
-        fmt = FORMAT_RGBA16F
-        texels = texels.astype(np.float16).astype(np.float64)
-      else:
-        texels, fmt = rgba.astype(np.float64), FORMAT_RGBA8
-      levels = mip_chain(texels, fmt)
+      fmt = FORMAT_RGBA16F if is_srgb(mjm, t) else FORMAT_RGBA8
+      sources.append(TextureSource(t, rgba, fmt))
+      shapes = level_shapes(rgba.shape[1], rgba.shape[0])
       size[t] = (rgba.shape[1], rgba.shape[0])
-    desc[t] = (levels[0].shape[1], levels[0].shape[0], len(levels), fmt)
-    for lv, img in enumerate(levels):
-      raw = np.ascontiguousarray(img).view(np.uint8).ravel()
+    else:
+      fmt, shapes = FORMAT_RGBA8, [(1, 1)]
+      white.append(t)
+    desc[t] = (shapes[0][0], shapes[0][1], len(shapes), fmt)
+    texel = 8 if fmt == FORMAT_RGBA16F else 4
+    for lv, (w, h) in enumerate(shapes):
       level_adr[t, lv] = offset
-      chunks.append(raw)
-      offset += raw.size
-      pad = (-offset) % 16
-      if pad:
-        chunks.append(np.zeros(pad, np.uint8))
-        offset += pad
-  pool = np.concatenate(chunks) if chunks else np.zeros(16, np.uint8)
-  return TexturePool(
+      offset += -(-w * h * texel // 16) * 16
+  pool = torch.zeros(max(offset, 16), dtype=torch.uint8, device=device)
+  for t in white:
+    pool[int(level_adr[t, 0]) : int(level_adr[t, 0]) + 4] = 255
+  tp = TexturePool(
     ntex=ntex,
-    data=torch.as_tensor(pool).to(device),
+    data=pool,
     level_adr=torch.as_tensor(level_adr).to(device),
     desc=torch.as_tensor(desc).to(device),
     size=size,
+    sources=sources,
+    host_level_adr=level_adr,
   )
+  if precompute and sources:
+    _build_mips(tp)
+  return tp
+
+
+def _build_mips(tp: TexturePool) -> None:
+  dev = tp.data.device
+  stream = int(torch.cuda.current_stream(dev).cuda_stream)
+  table = torch.as_tensor(SRGB_TABLE.view(np.int16)).to(dev)
+  largest = max(s.rgba8.shape[0] * s.rgba8.shape[1] for s in tp.sources)
+  work = torch.empty(8 * largest, dtype=torch.float64, device=dev)
+  for s in tp.sources:
+    h, w = s.rgba8.shape[:2]
+    n = int(tp.desc[s.tex_id, 2])
+    src = torch.as_tensor(np.ascontiguousarray(s.rgba8)).to(dev)
+    adr = (ctypes.c_int64 * n)(*(int(x) for x in tp.host_level_adr[s.tex_id, :n]))  # read on the host by the launcher
+    _abi.call(
+      "mjpbr_texture_mips",
+      _abi.ptr(src),
+      w,
+      h,
+      s.fmt,
+      _abi.ptr(table),
+      _abi.ptr(tp.data),
+      ctypes.cast(adr, ctypes.c_void_p),
+      n,
+      _abi.ptr(work),
+      work.numel(),
+      stream,
+    )

@@ -232,12 +232,17 @@ def dfg_lut(size: int = 128, nsample: int = 1024) -> np.ndarray:
 
 
 def fill_tables(rc, dfg_samples: int = 1024, prefilter_samples: int = 512) -> None:
-  """Fills a host context made with precompute=False: DFG LUT and prefiltered stacks (oracle)."""
+  """Fills a host context made with precompute=False: DFG LUT, texture mip chains, radiance mip
+  chains, SH9 coefficients, sampling tables and prefiltered stacks (oracle)."""
+  from oracle import tables
+
   n = int(rc.dfg_lut.shape[0])
   rc.dfg_lut.copy_(torch.from_numpy(dfg_lut(n, dfg_samples)))
+  tables.fill_textures(rc.textures)
   env = rc.env
   if not env.images:
     return
+  tables.fill_environments(env)
   L, H, W = (int(v) for v in env.prefiltered.shape[1:4])
   stack = np.zeros((env.nenv, L, H, W, 4), np.float16)
   for e in range(env.nenv):

