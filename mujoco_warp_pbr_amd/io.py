@@ -2,6 +2,7 @@
 
 from __future__ import annotations
 
+import dataclasses
 import math
 import os
 import warnings
@@ -34,25 +35,72 @@ _MESH_BVH_FRAMES = ("auto", "mujoco")
 _MESH_BVH_CONSTRUCTORS = ("cubql", "sah", "lbvh")
 
 
-def _enum(cls, value, name):
-  try:
-    return cls(value)
-  except ValueError:
-    options = ", ".join(f"{cls.__name__}.{e.name}" for e in cls)
-    raise ValueError(f"{name} must be one of {options}, got {value!r}") from None
+def _as_enum(cls, value, name: str):
+  """`value` as a member of the IntEnum `cls` (members or their integer values)."""
+  members = {int(m): m for m in cls}
+  if isinstance(value, (int, np.integer)) and int(value) in members:
+    return members[int(value)]
+  listing = ", ".join(f"{cls.__name__}.{m.name}" for m in cls)
+  raise ValueError(f"{name} must be one of {listing}, got {value!r}")
 
 
-def _check_at_least(name: str, value: int, low: int):
-  if value < low:
-    raise ValueError(f"{name} must be at least {low}, got {value}")
+# Argument rules of create_render_context, checked in this order: (argument, test, message). The
+# messages are the reference's (io.py:471-509; its tests/test_context_validation.py pins them).
+def _at_least(low):
+  return (lambda v: v >= low), f"must be at least {low}, got {{v}}"
 
 
-def _check_default_material(default_material):
-  for key, value in (default_material or {}).items():
+_ARGUMENT_RULES = (
+  ("samples_per_pixel",) + _at_least(1),
+  ("shadow_samples",) + _at_least(1),
+  ("max_layers",) + _at_least(1),
+  ("path_samples",) + _at_least(1),
+  ("max_depth",) + _at_least(0),
+  ("rr_depth",) + _at_least(0),
+  ("denoise_iterations",) + _at_least(0),
+  ("shadow_light_fraction", lambda v: 0.0 <= v <= 1.0, "must be in [0, 1], got {v}"),
+  ("denoiser", lambda v: v in ("atrous", "learned"), 'must be "atrous" or "learned", got {v!r}'),
+  ("mesh_bvh_frame", lambda v: v in _MESH_BVH_FRAMES, 'must be "auto" or "mujoco", got {v!r}'),
+  ("mesh_bvh_constructor", lambda v: v in _MESH_BVH_CONSTRUCTORS, f"must be one of {_MESH_BVH_CONSTRUCTORS}, got {{v!r}}"),
+  ("traversal", lambda v: v in _TRAVERSALS, 'must be "software", "optix" or "auto", got {v!r}'),
+  ("block_dim", lambda v: v is None or v >= 1, "must be at least 1, got {v}"),
+  ("max_registers", lambda v: v is None or v >= 0, "must be at least 0, got {v}"),
+  ("nworld", lambda v: v >= 1, "must be at least 1, got {v}"),
+)
+
+
+def _check_arguments(args: dict) -> None:
+  for name, ok, message in _ARGUMENT_RULES:
+    value = args[name]
+    if not ok(value):
+      raise ValueError(f"{name} " + message.format(v=value))
+
+
+def _check_default_material(default_material) -> None:
+  """Keys among metallic / roughness, values in [0, 1]."""
+  for key, value in dict(default_material or {}).items():
     if key not in _MATERIAL_KEYS:
       raise ValueError(f"default_material keys must be among {_MATERIAL_KEYS}, got {key!r}")
     if not 0.0 <= value <= 1.0:
       raise ValueError(f"default_material[{key!r}] must be in [0, 1], got {value}")
+
+
+def _envmap_images(envmap) -> list[np.ndarray]:
+  """The radiance images of `envmap`: None, one map or a list of maps, each a path to an RGBE
+  file or a (height, width, 3) array (a fourth channel is ignored)."""
+  if envmap is None:
+    return []
+  maps = list(envmap) if isinstance(envmap, (list, tuple)) else [envmap]
+
+  def image(i, item):
+    if isinstance(item, (str, bytes, os.PathLike)):
+      return envmap_lib.load_hdr(os.fsdecode(item))
+    arr = np.asarray(item)
+    if arr.ndim == 3 and arr.shape[2] in (3, 4) and min(arr.shape[:2]) >= 1:
+      return arr
+    raise ValueError(f"envmap[{i}] must be a path or a (height, width, 3) radiance array, got shape {arr.shape}")
+
+  return [image(i, item) for i, item in enumerate(maps)]
 
 
 def _per_camera(value, n: int, name: str, default: bool) -> np.ndarray:
@@ -157,44 +205,20 @@ def create_render_context(
   """
   if precompute and torch.device(device).type != "cuda":
     raise ValueError("create_render_context builds its tables with HIP kernels: use a GPU device (or precompute=False)")
-  light_model = _enum(LightModel, light_model, "light_model")
-  tone_mapper = _enum(ToneMapper, tone_mapper, "tone_mapper")
-  unset_material = _enum(UnsetMaterial, unset_material, "unset_material")
-  render_order = _enum(RenderOrder, render_order, "render_order")
-  tier = _enum(Tier, tier, "tier")
-  for name, value in (
-    ("samples_per_pixel", samples_per_pixel),
-    ("shadow_samples", shadow_samples),
-    ("max_layers", max_layers),
-    ("path_samples", path_samples),
-  ):
-    _check_at_least(name, value, 1)
-  for name, value in (("max_depth", max_depth), ("rr_depth", rr_depth), ("denoise_iterations", denoise_iterations)):
-    _check_at_least(name, value, 0)
+  light_model = _as_enum(LightModel, light_model, "light_model")
+  tone_mapper = _as_enum(ToneMapper, tone_mapper, "tone_mapper")
+  unset_material = _as_enum(UnsetMaterial, unset_material, "unset_material")
+  render_order = _as_enum(RenderOrder, render_order, "render_order")
+  tier = _as_enum(Tier, tier, "tier")
+  _check_arguments(locals())
   if tier == Tier.PATH and max_depth + 4 * max_layers >= MAX_PATH_VERTICES:
     raise ValueError(
       f"max_depth + 4 * max_layers must be below {MAX_PATH_VERTICES} for the path tracer's random "
       f"number layout, got {max_depth} + 4 * {max_layers}"
     )
-  if not 0.0 <= shadow_light_fraction <= 1.0:
-    raise ValueError(f"shadow_light_fraction must be in [0, 1], got {shadow_light_fraction}")
   _check_default_material(default_material)
   if denoise and tier != Tier.PATH:
     raise ValueError("denoise applies to the path-traced tier (Tier.PATH)")
-  if denoiser not in ("atrous", "learned"):
-    raise ValueError(f'denoiser must be "atrous" or "learned", got {denoiser!r}')
-  if mesh_bvh_frame not in _MESH_BVH_FRAMES:
-    raise ValueError(f'mesh_bvh_frame must be "auto" or "mujoco", got {mesh_bvh_frame!r}')
-  if mesh_bvh_constructor not in _MESH_BVH_CONSTRUCTORS:
-    raise ValueError(f"mesh_bvh_constructor must be one of {_MESH_BVH_CONSTRUCTORS}, got {mesh_bvh_constructor!r}")
-  if traversal not in _TRAVERSALS:
-    raise ValueError(f'traversal must be "software", "optix" or "auto", got {traversal!r}')
-  if block_dim is not None:
-    _check_at_least("block_dim", block_dim, 1)
-  if max_registers is not None:
-    _check_at_least("max_registers", max_registers, 0)
-  if nworld < 1:
-    raise ValueError(f"nworld must be at least 1, got {nworld}")
   if traversal == "optix":
     warnings.warn('traversal="optix" falls back to "software": no OptiX on this device', stacklevel=2)
   traversal = "software"
@@ -311,38 +335,17 @@ def create_render_context(
 
   physical = light_model == LightModel.PHYSICAL
   lighting = material.resolve_lighting(mjm, light_model, ev100)
-  metallic, roughness, reflectance = material.resolve_geom_materials(mjm, unset_material, default_material)
-  rendered_mats = np.unique(mjm.geom_matid[rendered & (mjm.geom_matid >= 0)])
-  features = material.SurfaceFeatures(mjm, use_textures, specular_aa and tier == Tier.REALTIME, rendered_mats)
-  used_tex = np.zeros(max(mjm.ntex, 1), dtype=bool)
-  if features.textures:
-    ids = mjm.mat_texid[rendered_mats].ravel()
-    used_tex[ids[ids >= 0]] = True
-  textures = texture.upload_textures(mjm, used_tex, dev, precompute=precompute)
-  alpha = material.effective_geom_rgba(mjm)[rendered, 3]
-  has_transparency = bool(np.any((alpha > 0.0) & (alpha < 1.0))) or features.alpha_map
-  has_reflection = enable_reflections and bool(mjm.nmat and np.any(mjm.mat_reflectance[rendered_mats] > 0.0))
+  surf = _resolve_surfaces(mjm, rendered, unset_material, default_material, use_textures, specular_aa and tier == Tier.REALTIME,
+                           enable_reflections)
+  metallic, roughness, reflectance, features = surf.metallic, surf.roughness, surf.reflectance, surf.features
+  has_transparency, has_reflection = surf.has_transparency, surf.has_reflection
+  textures = texture.upload_textures(mjm, surf.used_textures, dev, precompute=precompute)
   background_linear = tuple(float(x) for x in material.srgb_to_linear(background_color[:3]))
-
-  # Explicit maps, else the model's skybox, else constant ambient (io.py:267-285). MuJoCo's own
-  # lights never come from the skybox: outside the physical light model it is background only.
-  images = _envmap_images(envmap)
-  skybox_only = False
-  if not images and use_skybox:
-    sky = envmap_lib.skybox_image(mjm)
-    if sky is not None:
-      images.append(sky)
-      skybox_only = not physical
-  if images:
-    env = envmap_lib.build_environments(images, dev, precompute=precompute)
-    has_env, any_env = not skybox_only, True
-  else:
-    env = envmap_lib.empty_environments(dev)
-    has_env = any_env = False
+  env, has_env, any_env = _resolve_environment(mjm, envmap, use_skybox, physical, dev, precompute)
   if envmap_intensity is None:
     envmap_intensity = 1.0 / lighting.exposure
   if has_env and physical:
-    lighting.ambient[:] = 0.0
+    lighting.ambient[:] = 0.0  # image-based lighting replaces the headlight's ambient term
 
   dfg = torch.zeros((DFG_LUT_SIZE, DFG_LUT_SIZE, 2), dtype=torch.float32, device=dev)
   if precompute:
@@ -446,3 +449,45 @@ def world_stage_floats(tlas_depth: int, ngeom: int, nlight: int, ndirect: int = 
 def build_scene_bvh(m, d, rc) -> None:
   """Builds the per-world scene BVH (leaf assignment and boxes) from the current poses."""
   bvh_lib.refit_bvh(m, d, rc, rebuild=True)
+
+
+@dataclasses.dataclass
+class _Surfaces:
+  metallic: np.ndarray
+  roughness: np.ndarray
+  reflectance: np.ndarray
+  features: material.SurfaceFeatures
+  used_textures: np.ndarray  # (max(ntex, 1),) bool
+  has_transparency: bool
+  has_reflection: bool
+
+
+def _resolve_surfaces(mjm, rendered, unset_material, default_material, use_textures, specular_aa, enable_reflections) -> _Surfaces:
+  """Per-geom PBR parameters, the texture roles and textures the rendered materials use, and
+  whether translucency (an rgba or alpha texture below 1) or mirrors appear at all."""
+  metallic, roughness, reflectance = material.resolve_geom_materials(mjm, unset_material, default_material)
+  mats = np.unique(mjm.geom_matid[rendered & (mjm.geom_matid >= 0)])
+  features = material.SurfaceFeatures(mjm, use_textures, specular_aa, mats)
+  used = np.zeros(max(mjm.ntex, 1), dtype=bool)
+  if features.textures:
+    ids = np.asarray(mjm.mat_texid)[mats].ravel()
+    used[ids[ids >= 0]] = True
+  alpha = material.effective_geom_rgba(mjm)[rendered, 3]
+  translucent = bool(np.any((alpha > 0.0) & (alpha < 1.0))) or features.alpha_map
+  mirrors = enable_reflections and bool(mjm.nmat and np.any(np.asarray(mjm.mat_reflectance)[mats] > 0.0))
+  return _Surfaces(metallic, roughness, reflectance, features, used, translucent, mirrors)
+
+
+def _resolve_environment(mjm, envmap, use_skybox: bool, physical: bool, dev, precompute: bool):
+  """(environments, has_env, any_env) from the explicit maps, else the model's skybox, else
+  none (constant ambient) (reference io.py:267-285). A skybox lights the scene only under the
+  physical light model; with MuJoCo's lights it is background only (has_env False)."""
+  images = _envmap_images(envmap)
+  lights_scene = True
+  if not images and use_skybox:
+    sky = envmap_lib.skybox_image(mjm)
+    if sky is not None:
+      images, lights_scene = [sky], physical
+  if not images:
+    return envmap_lib.empty_environments(dev), False, False
+  return envmap_lib.build_environments(images, dev, precompute=precompute), lights_scene, True

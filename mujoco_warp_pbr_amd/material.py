@@ -30,101 +30,124 @@ def srgb_to_linear(x) -> np.ndarray:
 
 
 def effective_geom_rgba(mjm: mdl.HostModel) -> np.ndarray:
-  """(ngeom, 4) rgba each geom renders with: material colour when the geom keeps the default."""
-  rgba = np.array(mjm.geom_rgba, dtype=np.float64).reshape(-1, 4)
+  """(ngeom, 4) rgba each geom renders with (MuJoCo Warp's _geom_rgba on the host): a geom that
+  has a material and still carries MuJoCo's default rgba (0.5, 0.5, 0.5, 1) shows the
+  material's colour; any other geom its own."""
+  own = np.asarray(mjm.geom_rgba, np.float64).reshape(-1, 4)
   if not mjm.nmat:
-    return rgba
-  inherit = (mjm.geom_matid >= 0) & np.all(np.isclose(rgba, _DEFAULT_GEOM_RGBA), axis=1)
-  rgba[inherit] = mjm.mat_rgba[mjm.geom_matid[inherit]]
-  return rgba
+    return own.copy()
+  default = np.asarray(_DEFAULT_GEOM_RGBA)
+  untouched = np.all(np.abs(own - default) <= 1e-8 + 1e-5 * default, axis=1)
+  matid = np.asarray(mjm.geom_matid)
+  from_material = untouched & (matid >= 0)
+  mat_rgba = np.asarray(mjm.mat_rgba, np.float64).reshape(-1, 4)
+  return np.where(from_material[:, None], mat_rgba[np.maximum(matid, 0)], own)
 
 
-def _heuristic(rgba, shininess: float, has_material: bool) -> tuple[float, float, float]:
-  """Colour-based (metallic, roughness, reflectance) guess (material.py:106-129)."""
-  r, g, b = rgba[:3]
-  if not has_material:
-    brightness = (r + g + b) / 3.0
-    if brightness < 0.25:
-      return 0.4, 0.45, 0.35
-    if brightness > 0.8 or (r > 0.7 and g < 0.3):
-      return 0.0, 0.4, 0.25
-    return 0.0, 0.5, 0.2
-  lum = float(np.dot(rgba[:3], LUMINANCE_WEIGHTS))
-  chroma = max(r, g, b) - min(r, g, b)
-  metallic, roughness, reflectance = 0.0, 0.42, 0.6
-  if lum > 0.75 and chroma < 0.10:
-    roughness, reflectance = 0.38, 0.65
-  elif lum < 0.28 and chroma < 0.12:
-    roughness, reflectance = 0.28, 0.6
-  elif chroma < 0.08 and 0.35 <= lum <= 0.75:
-    metallic, roughness, reflectance = 0.6, 0.40, 0.7
-  else:
-    roughness = 0.40
-  if shininess > 0.0 and shininess != _DEFAULT_SHININESS:
-    roughness = float(np.clip(roughness * (1.3 - shininess), 0.06, 1.0))
-  return metallic, roughness, reflectance
+# The colour heuristics of the HEURISTIC unset-material policy (reference material.py:106-129),
+# as data: the first matching class gives (metallic, perceptual roughness, reflectance).
+# Geoms without a material, by the mean of r, g, b:
+_PLAIN_CLASSES = (
+  ("dark", lambda mean, r, g: mean < 0.25, (0.4, 0.45, 0.35)),
+  ("bright or red", lambda mean, r, g: (mean > 0.8) | ((r > 0.7) & (g < 0.3)), (0.0, 0.40, 0.25)),
+  ("other", lambda mean, r, g: np.ones_like(mean, bool), (0.0, 0.50, 0.20)),
+)
+# Geoms with a material, by luminance and chroma (max - min of r, g, b):
+_MATERIAL_CLASSES = (
+  ("white", lambda lum, chroma: (lum > 0.75) & (chroma < 0.10), (0.0, 0.38, 0.65)),
+  ("black", lambda lum, chroma: (lum < 0.28) & (chroma < 0.12), (0.0, 0.28, 0.60)),
+  ("grey metal", lambda lum, chroma: (chroma < 0.08) & (lum >= 0.35) & (lum <= 0.75), (0.6, 0.40, 0.70)),
+  ("coloured", lambda lum, chroma: np.ones_like(lum, bool), (0.0, 0.40, 0.60)),
+)
+# A material shininess other than MuJoCo's default scales that roughness by (1.3 - shininess),
+# clamped to [0.06, 1].
+_ROUGHNESS_CLAMP = (0.06, 1.0)
+
+
+def _first_match(classes, *keys) -> np.ndarray:
+  """(n, 3) values of the first class whose predicate holds for each entry."""
+  tests = [pred(*keys) for _, pred, _ in classes]
+  table = np.array([vals for _, _, vals in classes], np.float64)
+  return table[np.argmax(np.stack(tests), axis=0)]
+
+
+def heuristic_materials(rgba: np.ndarray, shininess: np.ndarray, has_material: np.ndarray) -> np.ndarray:
+  """(n, 3) (metallic, roughness, reflectance) guessed from flat colours (HEURISTIC policy)."""
+  rgb = np.asarray(rgba, np.float64)[:, :3]
+  r, g = rgb[:, 0], rgb[:, 1]
+  plain = _first_match(_PLAIN_CLASSES, rgb.mean(axis=1), r, g)
+  lum = rgb @ np.asarray(LUMINANCE_WEIGHTS)
+  chroma = rgb.max(axis=1) - rgb.min(axis=1)
+  with_mat = _first_match(_MATERIAL_CLASSES, lum, chroma)
+  glossy = (shininess > 0.0) & (shininess != _DEFAULT_SHININESS)
+  with_mat[:, 1] = np.where(glossy, np.clip(with_mat[:, 1] * (1.3 - shininess), *_ROUGHNESS_CLAMP), with_mat[:, 1])
+  return np.where(np.asarray(has_material)[:, None], with_mat, plain)
 
 
 def resolve_geom_materials(mjm: mdl.HostModel, unset_material: int = UnsetMaterial.GLTF, default_material: dict | None = None):
-  """(metallic, perceptual roughness, reflectance) per geom, float32 (ngeom,) each
-  (material.py:132-182). Authored values win; unset ones (-1) follow `default_material`, else
-  the `unset_material` policy."""
+  """(metallic, perceptual roughness, reflectance) per geom, float32 (ngeom,) each (reference
+  material.py:132-182). Authored material values win. Unset ones (MuJoCo's -1) come from
+  `default_material` when it names them, else from the policy: GLTF gives metallic 0, roughness
+  1 - shininess and reflectance = the material's specular; HEURISTIC (for geoms whose metallic
+  and roughness are both unset) the colour classes above, and matte ground planes."""
   default_material = default_material or {}
-  n = mjm.ngeom
-  metallic = np.zeros(n, np.float32)
-  roughness = np.zeros(n, np.float32)
-  reflectance = np.zeros(n, np.float32)
-  rgba_all = effective_geom_rgba(mjm)
-  for g in range(n):
-    mat = int(mjm.geom_matid[g])
-    has_material = mat >= 0
-    shininess = float(mjm.mat_shininess[mat]) if has_material else _DEFAULT_SHININESS
-    specular = float(mjm.mat_specular[mat]) if has_material else _DEFAULT_SPECULAR
-    authored_m = float(mjm.mat_metallic[mat]) if has_material else -1.0
-    authored_r = float(mjm.mat_roughness[mat]) if has_material else -1.0
-    if unset_material == UnsetMaterial.HEURISTIC and authored_m < 0.0 and authored_r < 0.0:
-      m, r, f = _heuristic(rgba_all[g], shininess, has_material)
-      if mjm.geom_type[g] == mdl.GEOM_PLANE:
-        m, r, f = 0.0, 1.0, 0.0
-    else:
-      m = 0.0
-      r = float(np.clip(1.0 - shininess, 0.0, 1.0))
-      f = specular
-    if authored_m < 0.0 and "metallic" in default_material:
-      m = default_material["metallic"]
-    if authored_r < 0.0 and "roughness" in default_material:
-      r = default_material["roughness"]
-    metallic[g] = authored_m if authored_m >= 0.0 else m
-    roughness[g] = authored_r if authored_r >= 0.0 else r
-    reflectance[g] = f
-  return metallic, roughness, reflectance
+  matid = np.asarray(mjm.geom_matid)
+  has = matid >= 0
+  m_idx = np.maximum(matid, 0)
+
+  def per_material(field, fallback):
+    values = np.asarray(getattr(mjm, field), np.float64) if mjm.nmat else np.zeros(1)
+    return np.where(has, values[m_idx] if mjm.nmat else fallback, fallback)
+
+  shininess = per_material("mat_shininess", _DEFAULT_SHININESS)
+  specular = per_material("mat_specular", _DEFAULT_SPECULAR)
+  authored = np.stack([per_material("mat_metallic", -1.0), per_material("mat_roughness", -1.0)], axis=1)
+  unset = authored < 0.0
+  # the policy's guess for every geom: metallic, roughness, reflectance
+  guess = np.stack([np.zeros_like(shininess), np.clip(1.0 - shininess, 0.0, 1.0), specular], axis=1)
+  if unset_material == UnsetMaterial.HEURISTIC:
+    heuristic = heuristic_materials(effective_geom_rgba(mjm), shininess, has)
+    heuristic[np.asarray(mjm.geom_type) == mdl.GEOM_PLANE] = (0.0, 1.0, 0.0)
+    use = unset.all(axis=1)
+    guess = np.where(use[:, None], heuristic, guess)
+  for k, key in enumerate(("metallic", "roughness")):
+    if key in default_material:
+      guess[:, k] = np.where(unset[:, k], float(default_material[key]), guess[:, k])
+  resolved = np.where(unset, guess[:, :2], authored)
+  return resolved[:, 0].astype(np.float32), resolved[:, 1].astype(np.float32), guess[:, 2].astype(np.float32)
+
+
+def _texels_below_opaque(mjm: mdl.HostModel, tex_ids, channel: int) -> bool:
+  """True when some texture of `tex_ids` has a texel below 255 in `channel` (when it has it)."""
+  for t in set(int(x) for x in np.asarray(tex_ids).ravel() if x >= 0):
+    nc = int(mjm.tex_nchannel[t])
+    if channel >= nc:
+      continue
+    n = int(mjm.tex_width[t]) * int(mjm.tex_height[t]) * nc
+    texels = np.asarray(mjm.tex_data[int(mjm.tex_adr[t]) : int(mjm.tex_adr[t]) + n])
+    if texels[channel::nc].min(initial=255) < 255:
+      return True
+  return False
 
 
 class SurfaceFeatures:
-  """Which optional material inputs the rendered geoms use (material.py:185-221)."""
+  """Which optional material inputs the rendered materials `mats` use (reference
+  material.py:185-221): texture roles (when textures are on), whether any alpha texture is not
+  fully opaque (translucent layers), and specular anti-aliasing."""
 
   def __init__(self, mjm: mdl.HostModel, use_textures: bool, specular_aa: bool, mats: np.ndarray):
-    texid = mjm.mat_texid[mats] if mjm.nmat else np.full((0, mdl.NTEXROLE), -1)
-
-    def has(*roles):
-      return use_textures and bool(np.any(texid[:, list(roles)] >= 0))
-
-    self.base_color_map = has(mdl.ROLE_RGB, mdl.ROLE_RGBA)
-    self.normal_map = has(mdl.ROLE_NORMAL)
-    self.orm_map = has(mdl.ROLE_ORM, mdl.ROLE_OCCLUSION, mdl.ROLE_ROUGHNESS, mdl.ROLE_METALLIC)
-    self.emissive_map = has(mdl.ROLE_EMISSIVE)
-    self.opacity_map = has(mdl.ROLE_OPACITY)
+    texid = np.asarray(mjm.mat_texid)[mats] if mjm.nmat else np.full((0, mdl.NTEXROLE), -1)
+    used = (texid >= 0).any(axis=0) if use_textures and len(texid) else np.zeros(mdl.NTEXROLE, bool)
+    self.base_color_map = bool(used[mdl.ROLE_RGB] or used[mdl.ROLE_RGBA])
+    self.normal_map = bool(used[mdl.ROLE_NORMAL])
+    self.orm_map = bool(used[[mdl.ROLE_ORM, mdl.ROLE_OCCLUSION, mdl.ROLE_ROUGHNESS, mdl.ROLE_METALLIC]].any())
+    self.emissive_map = bool(used[mdl.ROLE_EMISSIVE])
+    self.opacity_map = bool(used[mdl.ROLE_OPACITY])
     self.textures = self.base_color_map or self.normal_map or self.orm_map or self.emissive_map or self.opacity_map
-    self.alpha_map = False
-    if use_textures:
-      for role, channel in ((mdl.ROLE_RGBA, 3), (mdl.ROLE_OPACITY, 0)):
-        for t in np.unique(texid[:, role]):
-          if t < 0:
-            continue
-          w, h, nc = int(mjm.tex_width[t]), int(mjm.tex_height[t]), int(mjm.tex_nchannel[t])
-          data = mjm.tex_data[mjm.tex_adr[t] : mjm.tex_adr[t] + w * h * nc].reshape(-1, nc)
-          if channel < nc and np.any(data[:, channel] < 255):
-            self.alpha_map = True
+    # RGBA textures carry alpha in channel 3, opacity textures in channel 0
+    self.alpha_map = bool(use_textures and len(texid)) and (
+      _texels_below_opaque(mjm, texid[:, mdl.ROLE_RGBA], 3) or _texels_below_opaque(mjm, texid[:, mdl.ROLE_OPACITY], 0)
+    )
     self.specular_aa = specular_aa
 
 
