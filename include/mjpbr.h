@@ -92,6 +92,11 @@ typedef struct MjpbrGeometry {
    * then each workgroup copies its world's block into LDS. NULL: each workgroup resolves its
    * world itself (same results). The wavefront tracer ignores it. */
   float* world_stage;
+  /* Optional [nmesh, 9] row-major rotations R of meshes whose BVH (blas_*, mesh_aabb) and vertex
+   * arrays (MjpbrMaterials.mesh_vert / mesh_normal) are in the frame R (vertices R v): ray queries
+   * and shading of their mesh and SDF geoms use geom_xmat R^T (reference trace.py:20-30,
+   * io.py:138-211). NULL: every mesh in its MuJoCo frame. Height fields are never re-framed. */
+  const float* mesh_frame;
 } MjpbrGeometry;
 
 /* Materials, mesh attributes and textures (reference: material.py:224-249 `Materials`). */
@@ -325,6 +330,16 @@ int64_t mjpbr_world_stage_floats(const MjpbrScene* scene);
  * or refit (boxes only, leaf assignment kept). Replaces `mujoco_warp.refit_bvh` and the BVH of
  * `mujoco_warp.create_render_context` (reference io.py:511-528, README.md:50-52). */
 int mjpbr_scene_bvh(const MjpbrGeometry* geo, int refit, void* stream);
+
+/* Mesh BVH of one triangle mesh on the GPU (replaces the mesh BVHs of MuJoCo Warp's render
+ * context, reference io.py:511-528; re-framed meshes pass their re-framed vertices, io.py:138-211):
+ * verts [nvert][3], faces [nface][3] (device). Fills inner [2^depth - 1][12] (the layout of
+ * MjpbrGeometry.blas_box) and tri [2^depth * MJPBR_BLAS_LEAF][12] (blas_tri). Triangles go to leaves
+ * by recursive median split of their centroids along the widest axis (a stable sort a level), so
+ * the tree is a pure function of the mesh. work: mjpbr_mesh_bvh_workspace bytes. */
+int64_t mjpbr_mesh_bvh_workspace(int nface, int depth);
+int mjpbr_mesh_bvh_build(const float* verts, const int32_t* faces, int nface, int depth, float* inner, float* tri, void* work,
+                         int64_t work_bytes, void* stream);
 
 /* Real-time tier megakernel: one thread per (world, pixel), all samples of a pixel.
  * Replaces `render` with tier REALTIME (reference render.py:430-460, megakernel :400-427). */

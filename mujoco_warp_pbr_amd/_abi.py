@@ -75,6 +75,8 @@ FUNCTIONS.update(
     "mjpbr_unet_unpack_output": (_I, [_VP, _VP, _I, _I, _I, _I, _VP, _I, _I, _VP, _I, _VP, _I, _I, _VP, _VP]),
     "mjpbr_unet_guide_stats": (_I, [_VP, _I, _I, _I, _I, _VP, _VP, _VP, _I, _VP]),
     "mjpbr_unet_mfma_probe": (_I, [_VP, _VP, _VP, _VP]),
+    "mjpbr_mesh_bvh_workspace": (_L, [_I, _I]),
+    "mjpbr_mesh_bvh_build": (_I, [_VP, _VP, _I, _I, _VP, _VP, _VP, _L, _VP]),
     "mjpbr_texture_mips": (_I, [_VP, _I, _I, _I, _VP, _VP, _VP, _I, _VP, _L, _VP]),
     "mjpbr_radiance_mips": (_I, [_VP, _I, _I, _VP, _VP, _I, _VP, _L, _VP]),
     "mjpbr_env_sh9": (_I, [_VP, _I, _I, _VP, _L, _VP, _VP]),

@@ -17,7 +17,9 @@ constexpr float kHugeExtent = 1.0e8f;
 MJ_DEV void geom_aabb(const MjpbrGeometry& g, int w, int gi, float* lo, float* hi) {
   int type = g.geom_type[gi];
   const float* sz = g.geom_size + (int64_t)(w % g.geom_size_n) * 3 * g.ngeom + 3 * gi;
-  const float* R = g.geom_xmat + ((int64_t)w * g.ngeom + gi) * 9;
+  float framed[9];
+  const float* R = traced_rotation(g, type, g.geom_dataid[(int64_t)(w % g.geom_dataid_n) * g.ngeom + gi],
+                                   g.geom_xmat + ((int64_t)w * g.ngeom + gi) * 9, framed);
   V3 pos = ld3(g.geom_xpos + ((int64_t)w * g.ngeom + gi) * 3);
   V3 c = v3s(0.0f), e = v3s(0.0f);
   switch (type) {

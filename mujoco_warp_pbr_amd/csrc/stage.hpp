@@ -7,6 +7,21 @@
 namespace mjpbr {
 
 
+// out = M R^T (row-major 3 x 3; sums in index order): the rotation a mesh geom is traced and
+// shaded with when its mesh BVH is in frame R (MjpbrGeometry.mesh_frame).
+MJ_HOST_DEV inline void framed_rotation(const float* M, const float* R, float* out) {
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) out[3 * i + j] = M[3 * i] * R[3 * j] + M[3 * i + 1] * R[3 * j + 1] + M[3 * i + 2] * R[3 * j + 2];
+}
+
+// The rotation geom gi of world w is traced with: geom_xmat, or geom_xmat R^T for a mesh or SDF
+// geom whose mesh is re-framed. Returns `xm` or `tmp`.
+MJ_HOST_DEV inline const float* traced_rotation(const MjpbrGeometry& g, int type, int dataid, const float* xm, float* tmp) {
+  if (!g.mesh_frame || dataid < 0 || (type != MJPBR_GEOM_MESH && type != MJPBR_GEOM_SDF)) return xm;
+  framed_rotation(xm, g.mesh_frame + 9 * dataid, tmp);
+  return tmp;
+}
+
 // Floats of one world's staged records: scene BVH boxes, geom records, leaf ids and direct geom
 // ids (each padded to a multiple of 4), light records, surface records, the environment record
 // (kEnvRec). A multiple of 4, so records copy as float4.
@@ -49,8 +64,11 @@ MJ_DEV void stage_world_to(const MjpbrScene& sc, int w, float* dst) {
   for (int gi = threadIdx.x; gi < g.ngeom; gi += blockDim.x) {
     float* r = geoms + gi * kGeomRec;
     const float* xp = g.geom_xpos + ((int64_t)w * g.ngeom + gi) * 3;
-    const float* xm = g.geom_xmat + ((int64_t)w * g.ngeom + gi) * 9;
     const float* sz = bat(g.geom_size, g.geom_size_n, w, 3 * g.ngeom) + 3 * gi;
+    int type = g.geom_type[gi];
+    int dataid = bat(g.geom_dataid, g.geom_dataid_n, w, g.ngeom)[gi];
+    float framed[9];
+    const float* xm = traced_rotation(g, type, dataid, g.geom_xmat + ((int64_t)w * g.ngeom + gi) * 9, framed);
     r[0] = xp[0];
     r[1] = xp[1];
     r[2] = xp[2];
@@ -61,8 +79,6 @@ MJ_DEV void stage_world_to(const MjpbrScene& sc, int w, float* dst) {
     r[14] = sz[2];
     int mid;
     float4 rgba = geom_rgba(g, w, gi, &mid);
-    int type = g.geom_type[gi];
-    int dataid = bat(g.geom_dataid, g.geom_dataid_n, w, g.ngeom)[gi];
     if (type == MJPBR_GEOM_HFIELD && dataid >= 0) dataid += g.nmesh;
     int traced = (type == MJPBR_GEOM_SDF || type == MJPBR_GEOM_HFIELD) ? MJPBR_GEOM_MESH : type;
     int bits = traced | (rgba.w == 0.0f ? 0x100 : 0) | ((dataid + 1) << 9);

@@ -289,7 +289,8 @@ def create_render_context(
   bvh_ids, direct_ids = enabled[~direct], enabled[direct]
   depth = bvh_lib.scene_depth(len(bvh_ids))
   tlas_box, tlas_leaf = bvh_lib.alloc_scene_bvh(nworld, depth, dev)
-  meshes = bvh_lib.build_mesh_bvhs(mjm)
+  frames = bvh_lib.select_mesh_frames(mjm) if mesh_bvh_frame == "auto" else {}
+  meshes = bvh_lib.build_mesh_bvhs(mjm, dev, precompute=precompute, frames=frames)
 
   base = RenderBase(
     nworld=nworld,
@@ -315,8 +316,8 @@ def create_render_context(
     blas_depth=torch.as_tensor(meshes.depth).to(dev),
     blas_node_adr=torch.as_tensor(meshes.node_adr).to(dev),
     blas_tri_adr=torch.as_tensor(meshes.tri_adr).to(dev),
-    blas_box=torch.as_tensor(meshes.box).to(dev),
-    blas_tri=torch.as_tensor(meshes.tri).to(dev),
+    blas_box=meshes.box,
+    blas_tri=meshes.tri,
     mesh_aabb=torch.as_tensor(meshes.aabb).to(dev),
     enable_backface_culling=enable_backface_culling,
     enable_vertex_normals=enable_vertex_normals,
@@ -329,6 +330,8 @@ def create_render_context(
   )
 
   base.bvh_geom_ids = torch.as_tensor(bvh_ids).to(dev)
+  base.mesh_frame, base.trace_mesh_vert, base.trace_mesh_normal = meshes.frame, meshes.vert, meshes.normal
+  base.mesh_bvhs = meshes
   base.direct_geom_ids = torch.as_tensor(direct_ids).to(dev)
   base.world_stage = torch.zeros((nworld, world_stage_floats(depth, mjm.ngeom, mjm.nlight, len(direct_ids))),
                                  dtype=torch.float32, device=dev)

@@ -129,6 +129,9 @@ class HostModel:
   hfield_ncol: np.ndarray = dataclasses.field(default_factory=lambda: np.zeros(0, np.int32))
   hfield_adr: np.ndarray = dataclasses.field(default_factory=lambda: np.zeros(0, np.int32))
   hfield_data: np.ndarray = dataclasses.field(default_factory=lambda: np.zeros(0, np.float32))
+  # mesh_quat: rotation (w, x, y, z) from a mesh's MuJoCo frame (its principal inertia frame,
+  # where mesh_vert lives) back to its authored frame; None: identity for every mesh
+  mesh_quat: np.ndarray | None = None
 
   @property
   def nhfield(self) -> int:

@@ -80,6 +80,7 @@ def geometry_struct(m, d, rc: PbrRenderContext):
   g.blas_tri = _abi.ptr(base.blas_tri)
   g.mesh_aabb = _abi.ptr(base.mesh_aabb)
   g.world_stage = _abi.ptr(base.world_stage)
+  g.mesh_frame = _abi.ptr(base.mesh_frame)
   return g
 
 
@@ -89,6 +90,7 @@ def scene_struct(m, d, rc: PbrRenderContext):
   sc = S["MjpbrScene"]()
   sc.geo = geometry_struct(m, d, rc)
   # materials
+  base = rc.base
   mt = sc.mats
   mt.geom_metallic = _abi.ptr(_check(rc.geom_metallic, torch.float32, "geom_metallic"))
   mt.geom_roughness = _abi.ptr(_check(rc.geom_roughness, torch.float32, "geom_roughness"))
@@ -120,6 +122,9 @@ def scene_struct(m, d, rc: PbrRenderContext):
     "mesh_texcoord",
   ):
     setattr(mt, name, _abi.ptr(getattr(m, name)))
+  if base.trace_mesh_vert is not None:  # re-framed mesh BVHs: shade from the traced-frame copies
+    mt.mesh_vert = _abi.ptr(base.trace_mesh_vert)
+    mt.mesh_normal = _abi.ptr(base.trace_mesh_normal)
   tex = rc.textures
   mt.ntex = tex.ntex
   mt.tex_level_adr = _abi.ptr(tex.level_adr)

@@ -119,6 +119,13 @@ class RenderBase:
   # enabled geoms split into the scene BVH's (bounded) and the directly tested ones (unbounded
   # planes, MjpbrGeometry.direct_geom); None: every enabled geom is in the BVH
   bvh_geom_ids: Any = None
+  # mesh BVHs built in another frame (mesh_bvh_frame="auto"): per-mesh rotations R, and the traced
+  # vertex / normal arrays (those meshes rotated); None: every mesh in its MuJoCo frame
+  mesh_frame: Any = None
+  trace_mesh_vert: Any = None
+  trace_mesh_normal: Any = None
+  # host description of the mesh BVHs (vertices, faces) for host-side harnesses
+  mesh_bvhs: Any = None
   direct_geom_ids: Any = None
 
 

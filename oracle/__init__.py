@@ -234,8 +234,12 @@ def dfg_lut(size: int = 128, nsample: int = 1024) -> np.ndarray:
 def fill_tables(rc, dfg_samples: int = 1024, prefilter_samples: int = 512) -> None:
   """Fills a host context made with precompute=False: DFG LUT, texture mip chains, radiance mip
   chains, SH9 coefficients, sampling tables and prefiltered stacks (oracle)."""
+  from oracle import blas
   from oracle import tables
 
+  mb = rc.base.mesh_bvhs
+  if mb is not None and mb.faces:
+    blas.fill_mesh_bvhs(rc.base, mb.host_vert, mb.faces)
   n = int(rc.dfg_lut.shape[0])
   rc.dfg_lut.copy_(torch.from_numpy(dfg_lut(n, dfg_samples)))
   tables.fill_textures(rc.textures)

@@ -185,6 +185,19 @@ static void geom_rgba_o(const MjpbrGeometry* g, int w, int gi, float rgba[4], in
 
 static float cut_o(float dist) { return dist * (1.0f + O_TOL_REL) + O_TOL_ABS; }
 
+/* Rotation of geom gi as traced: geom_xmat, or geom_xmat R^T for a mesh / SDF geom whose mesh
+ * BVH is built in frame R (reference trace.py:150-165, geom_xmat_trace). */
+static const float* traced_mat_o(const MjpbrGeometry* g, int w, int gi, float* tmp) {
+  const float* M = g->geom_xmat + ((int64_t)w * g->ngeom + gi) * 9;
+  int type = g->geom_type[gi];
+  int dataid = g->geom_dataid[bidx(w, g->geom_dataid_n) * g->ngeom + gi];
+  if (!g->mesh_frame || dataid < 0 || (type != MJPBR_GEOM_MESH && type != MJPBR_GEOM_SDF)) return M;
+  const float* R = g->mesh_frame + 9 * dataid;
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) tmp[3 * i + j] = M[3 * i] * R[3 * j] + M[3 * i + 1] * R[3 * j + 1] + M[3 * i + 2] * R[3 * j + 2];
+  return tmp;
+}
+
 /* One geom (traverse.py:131-192). any_hit: meshes report 0 on any triangle before max_t. */
 static float intersect_o(const Ctx* c, int gi, v3 o, v3 d, float max_t, int any_hit, v3* n, float* bu, float* bv,
                          int* face, int* mesh) {
@@ -193,7 +206,8 @@ static float intersect_o(const Ctx* c, int gi, v3 o, v3 d, float max_t, int any_
   int w = c->w;
   int type = g->geom_type[gi];
   v3 pos = load3(g->geom_xpos + ((int64_t)w * g->ngeom + gi) * 3);
-  const float* mat = g->geom_xmat + ((int64_t)w * g->ngeom + gi) * 9;
+  float framed[9];
+  const float* mat = traced_mat_o(g, w, gi, framed);
   v3 size = load3(g->geom_size + ((int64_t)bidx(w, g->geom_size_n) * g->ngeom + gi) * 3);
   switch (type) {
     case MJPBR_GEOM_PLANE: return plane_o(pos, mat, size, o, d, n);
@@ -686,7 +700,8 @@ static SurfO eval_surface_o(const Ctx* c, const HitO* hit, v3 point, v3 dir, flo
   sf.reflectance = m->geom_reflectance[(w % m->geom_reflectance_n) * ng + gid];
   float emission = mat_id >= 0 ? m->mat_emission[(w % m->mat_emission_n) * g->nmat + mat_id] : 0.0f;
   int gtype = g->geom_type[gid];
-  const float* R = g->geom_xmat + ((int64_t)w * ng + gid) * 9;
+  float framed[9];
+  const float* R = traced_mat_o(g, w, gid, framed);
   v3 ldir = mtmul(R, dir);
   v3 lnrm = mtmul(R, hit->normal);
   if (dot3(lnrm, ldir) > 0.0f) lnrm = neg3(lnrm);

@@ -187,34 +187,6 @@ def test_srgb_textures_decode_to_linear_f16():
   np.testing.assert_allclose(lv0[..., :3].astype(np.float64), material.srgb_to_linear(raw[..., :3] / 255.0), atol=1e-3)
 
 
-def test_mesh_bvh_layout_covers_every_triangle():
-  rng = np.random.default_rng(3)
-  verts, faces = scenes.convex_hull_mesh(rng, 300, 0.2)
-  depth, boxes, tri = bvh.build_mesh_bvh(verts, faces)
-  ids = tri[:, 3].view(np.int32)
-  assert sorted(ids[ids >= 0].tolist()) == list(range(len(faces)))
-  assert (1 << depth) * bvh.BLAS_LEAF == len(tri)
-  assert boxes.shape == ((1 << depth) - 1, 12)
-  # Every triangle lies inside the box its parent stores for its leaf.
-  nleaf = 1 << depth
-  for slot in np.nonzero(ids >= 0)[0]:
-    leaf = slot // bvh.BLAS_LEAF
-    node = nleaf - 1 + leaf
-    parent, side = (node - 1) // 2, (node - 1) % 2
-    lo, hi = boxes[parent, 6 * side : 6 * side + 3], boxes[parent, 6 * side + 3 : 6 * side + 6]
-    v0 = tri[slot, 0:3]
-    for p in (v0, v0 + tri[slot, 4:7], v0 + tri[slot, 8:11]):
-      assert np.all(p >= lo) and np.all(p <= hi)
-  # Parent boxes contain their children's boxes.
-  for i in range(1, (1 << depth) - 1):
-    parent, side = (i - 1) // 2, (i - 1) % 2
-    plo, phi = boxes[parent, 6 * side : 6 * side + 3], boxes[parent, 6 * side + 3 : 6 * side + 6]
-    for c in (0, 1):
-      clo, chi = boxes[i, 6 * c : 6 * c + 3], boxes[i, 6 * c + 3 : 6 * c + 6]
-      if clo[0] < 1e29:
-        assert np.all(clo >= plo) and np.all(chi <= phi)
-
-
 def test_per_world_tables_are_batched():
   mjm = scenes.primitives_scene(0, 8)
   m = mj.put_model(mjm, "cpu")
