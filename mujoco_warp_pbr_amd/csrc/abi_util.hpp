@@ -22,6 +22,20 @@ inline size_t host_stage_bytes(const MjpbrScene* sc) { return (size_t)stage_floa
 // LDS a render workgroup may use without lowering occupancy below 8 workgroups a compute unit.
 constexpr size_t kRenderLdsPerGroup = 20 * 1024;
 
+// Largest per-world record block the render kernels stage in LDS; larger worlds are read in
+// place from geo.world_stage (the kFGlobal kernel instances).
+constexpr size_t kMaxStagedLds = 64 * 1024;
+
+// Whether the render kernels read world records from global memory; an error when the records
+// need it and the scene carries no world_stage.
+inline int global_records(const MjpbrScene* sc, bool* global) {
+  *global = host_stage_bytes(sc) > kMaxStagedLds;
+  if (*global && !sc->geo.world_stage)
+    return set_error(MJPBR_EINVAL, "a world's records (%d geoms, %d lights) take %zu bytes, beyond the %zu staged in LDS: pass geo.world_stage",
+                     sc->geo.ngeom, sc->lit.nlight, host_stage_bytes(sc), kMaxStagedLds);
+  return MJPBR_OK;
+}
+
 // Sets tm's mesh-BVH cache (the top levels the render kernels stage in LDS after the world's
 // records) for a scene whose records take `stage_bytes`; returns the bytes the cache adds.
 inline size_t plan_mesh_cache(const MjpbrScene* sc, bool meshes, size_t stage_bytes, TileMap* tm) {

@@ -14,6 +14,8 @@ namespace mjpbr {
 
 // 8 waves per SIMD with spills beat 2 without: the path loop is latency bound (measured 1.8x on
 // the full-feature scene at 1024 x 128^2, 1 sample).
+// kGlobal: the world's records are read in place from geo.world_stage (too large for the LDS).
+template <bool kGlobal>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8)))
 pt_kernel(MjpbrScene sc, MjpbrSettings s, MjpbrOutputs out, int seed, TileMap tm) {
   extern __shared__ float lds[];
@@ -28,7 +30,8 @@ pt_kernel(MjpbrScene sc, MjpbrSettings s, MjpbrOutputs out, int seed, TileMap tm
   int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   int px = tx * kTile + (wave & 1) * 8 + (lane & 7);
   int py = ty * kTile + (wave >> 1) * 8 + (lane >> 3);
-  SceneView sv = stage_world(sc, w, lds, tm.mesh_cache_nodes, tm.mesh_cache_n);
+  SceneView sv = kGlobal ? global_world(sc, w, lds, tm.mesh_cache_nodes, tm.mesh_cache_n)
+                        : stage_world(sc, w, lds, tm.mesh_cache_nodes, tm.mesh_cache_n);
   Pixel pix;
   if (!setup_pixel(sc.cams, out, s, w, cidx, px, py, tm.ray_offset[cidx], &pix)) return;
   float exposure = sc.env.exposure[w % sc.env.exposure_n];
@@ -108,10 +111,14 @@ extern "C" int mjpbr_pt_render(const MjpbrScene* scene, const MjpbrSettings* s, 
   if (s->path_samples < 1) return set_error(MJPBR_EINVAL, "path_samples must be at least 1");
   if (s->max_depth < 0 || s->max_depth + 4 * s->max_layers >= kJitterDepth)
     return set_error(MJPBR_EINVAL, "max_depth + 4 * max_layers must be below %d", kJitterDepth);
-  size_t lds = host_stage_bytes(scene);
-  if (lds > 64 * 1024) return set_error(MJPBR_EINVAL, "scene too large for LDS staging (%d geoms, %d lights)", scene->geo.ngeom, scene->lit.nlight);
+  bool global;
+  if ((rc = global_records(scene, &global))) return rc;
+  size_t lds = global ? 0 : host_stage_bytes(scene);
   lds += plan_mesh_cache(scene, s->has_meshes != 0, lds, &tm);
   if ((rc = launch_stage_prep(scene, (hipStream_t)stream))) return rc;
-  hipLaunchKernelGGL(pt_kernel, dim3(nlaunch), dim3(256), lds, (hipStream_t)stream, *scene, *s, *out, seed, tm);
+  if (global)
+    hipLaunchKernelGGL(pt_kernel<true>, dim3(nlaunch), dim3(256), lds, (hipStream_t)stream, *scene, *s, *out, seed, tm);
+  else
+    hipLaunchKernelGGL(pt_kernel<false>, dim3(nlaunch), dim3(256), lds, (hipStream_t)stream, *scene, *s, *out, seed, tm);
   return check_launch("pt_kernel");
 }

@@ -121,7 +121,8 @@ rt_kernel(MjpbrScene sc, MjpbrSettings s, MjpbrOutputs out, int seed, TileMap tm
   int px = tx * kTile + (wave & 1) * 8 + (lane & 7);
   int py = ty * kTile + (wave >> 1) * 8 + (lane >> 3);
 
-  SceneView sv = stage_world(sc, w, lds, tm.mesh_cache_nodes, tm.mesh_cache_n);
+  SceneView sv = (F & kFGlobal) ? global_world(sc, w, lds, tm.mesh_cache_nodes, tm.mesh_cache_n)
+                                 : stage_world(sc, w, lds, tm.mesh_cache_nodes, tm.mesh_cache_n);
 
   Pixel pix;
   if (!setup_pixel(sc.cams, out, s, w, cidx, px, py, tm.ray_offset[cidx], &pix)) return;
@@ -187,6 +188,7 @@ const RtInstance kRtInstances[] = {
     MJPBR_RT_INSTANCE(kFMesh | kFTex, 6),
     MJPBR_RT_INSTANCE(kFMesh | kFSS | kFSoft | kFMirror | kFTex, 8),
     MJPBR_RT_INSTANCE(kFAll, 8),
+    MJPBR_RT_INSTANCE(kFAll | kFGlobal, 8),
 };
 #undef MJPBR_RT_INSTANCE
 
@@ -223,9 +225,10 @@ extern "C" int mjpbr_rt_render(const MjpbrScene* scene, const MjpbrSettings* s, 
   int rc = make_tile_map(scene, &tm, &nlaunch);
   if (rc) return rc;
   if (s->samples_per_axis < 1) return set_error(MJPBR_EINVAL, "samples_per_axis must be at least 1");
-  size_t lds = host_stage_bytes(scene);
-  if (lds > 64 * 1024) return set_error(MJPBR_EINVAL, "scene too large for LDS staging (%d geoms, %d lights)", scene->geo.ngeom, scene->lit.nlight);
-  const RtInstance& k = pick_instance(required_features(*s));
+  bool global;
+  if ((rc = global_records(scene, &global))) return rc;
+  size_t lds = global ? 0 : host_stage_bytes(scene);
+  const RtInstance& k = pick_instance(required_features(*s) | (global ? kFGlobal : 0u));
   // mesh-BVH top levels in the LDS left to a workgroup at 8 per compute unit
   lds += plan_mesh_cache(scene, (k.mask & kFMesh) && s->has_meshes, lds, &tm);
   if ((rc = launch_stage_prep(scene, (hipStream_t)stream))) return rc;

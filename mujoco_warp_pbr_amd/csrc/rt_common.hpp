@@ -17,7 +17,7 @@ constexpr float kMinThroughput = 1.0f / 255.0f;  // render.py:62
 //   kFMirror  has_reflection
 //   kFLayer   has_transparency
 //   kFTex     textures
-enum : unsigned { kFMesh = 1, kFSS = 2, kFSoft = 4, kFMirror = 8, kFLayer = 16, kFTex = 32, kFAll = 63 };
+enum : unsigned { kFMesh = 1, kFSS = 2, kFSoft = 4, kFMirror = 8, kFLayer = 16, kFTex = 32, kFAll = 63, kFGlobal = 64 };
 
 struct RtCtx {
   const MjpbrScene& sc;

@@ -86,7 +86,7 @@ MJ_DEV void stage_world_to(const MjpbrScene& sc, int w, float* dst) {
   }
 }
 
-MJ_DEV SceneView scene_view(const MjpbrScene& sc, float* lds) {
+MJ_DEV SceneView scene_view(const MjpbrScene& sc, const float* lds) {
   const MjpbrGeometry& g = sc.geo;
   int nleaf = 1 << g.tlas_depth;
   SceneView sv;
@@ -140,6 +140,21 @@ MJ_DEV SceneView stage_world(const MjpbrScene& sc, int w, float* lds, int cache_
   SceneView sv = scene_view(sc, lds);
   if (cache_n > 0) {
     sv.mesh_cache = cache;
+    sv.mesh_cache_nodes = cache_nodes;
+    sv.mesh_cache_n = cache_n;
+  }
+  return sv;
+}
+
+// World w's records read in place from geo.world_stage (stage_prep_kernel ran first on the
+// stream), for worlds whose records do not fit the LDS; the mesh-BVH top levels (cache_n > 0)
+// still go to LDS.
+MJ_DEV SceneView global_world(const MjpbrScene& sc, int w, float* lds, int cache_nodes = 0, int cache_n = 0) {
+  SceneView sv = scene_view(sc, sc.geo.world_stage + (int64_t)w * stage_floats(sc));
+  if (cache_n > 0) {
+    stage_mesh_cache(sc.geo, cache_nodes, cache_n, lds);
+    __syncthreads();
+    sv.mesh_cache = lds;
     sv.mesh_cache_nodes = cache_nodes;
     sv.mesh_cache_n = cache_n;
   }

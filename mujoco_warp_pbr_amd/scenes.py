@@ -391,6 +391,38 @@ def mesh_scene(seed: int = 0, res: int = 256, nmesh: int = 16, instances: int = 
   return b.build(stat_center=np.array([0, 0, 0.3], np.float32), stat_extent=2.0)
 
 
+def crowd_scene(seed: int = 0, res: int = 64, ngeom: int = 600, nmat: int = 12) -> mdl.HostModel:
+  """A floor and `ngeom` small primitives of every type on a jittered grid, four of them mesh
+  instances, sharing `nmat` materials; a sun and a point light. Worlds with hundreds of geoms:
+  records beyond the LDS and scene BVHs of more than 256 leaves."""
+  rng = np.random.default_rng(seed)
+  b = _Builder()
+  plane_mat = b.material(rgba=(0.7, 0.7, 0.7, 1), metallic=0.0, roughness=0.8)
+  b.geom(mdl.GEOM_PLANE, (3.0, 3.0, 0.1), (0, 0, 0), matid=plane_mat)
+  mats = [b.material(rgba=_random_rgb(rng), metallic=float(rng.choice([0.0, 1.0])), roughness=float(rng.uniform(0.2, 1.0)))
+          for _ in range(nmat)]
+  mesh = b.mesh(*convex_hull_mesh(rng, 200, 0.06))
+  kinds = [mdl.GEOM_SPHERE, mdl.GEOM_BOX, mdl.GEOM_CAPSULE, mdl.GEOM_CYLINDER, mdl.GEOM_ELLIPSOID]
+  side = int(math.ceil(math.sqrt(ngeom)))
+  pitch = 2.4 / side
+  for i in range(ngeom):
+    gx, gy = i % side, i // side
+    p = ((gx + 0.5) * pitch - 1.2 + rng.uniform(-0.2, 0.2) * pitch, (gy + 0.5) * pitch - 1.2 + rng.uniform(-0.2, 0.2) * pitch,
+         rng.uniform(0.05, 0.5))
+    r = 0.3 * pitch
+    mat = mats[i % nmat]
+    if i % (ngeom // 4) == 0:
+      b.geom(mdl.GEOM_MESH, (0, 0, 0), p, mat=random_rotation(rng), matid=mat, dataid=mesh)
+      continue
+    k = kinds[i % len(kinds)]
+    size = (r, 0, 0) if k == mdl.GEOM_SPHERE else ((r, r * 0.8, 0) if k in (mdl.GEOM_CAPSULE, mdl.GEOM_CYLINDER) else (r, 0.7 * r, 0.5 * r))
+    b.geom(k, size, p, mat=random_rotation(rng), matid=mat)
+  b.light(mdl.LIGHT_DIRECTIONAL, (0, 0, 3), (-0.4, 0.3, -0.86))
+  b.light(mdl.LIGHT_POINT, (0.6, -0.5, 1.5), (0, 0, -1))
+  b.camera((0.0, -2.6, 2.0), (0.0, 0.0, 0.2), 50.0, (res, res))
+  return b.build(stat_center=np.array([0, 0, 0.3], np.float32), stat_extent=2.0)
+
+
 def xyaxes_frame(xyaxes) -> np.ndarray:
   """Camera frame (3, 3) from MJCF `xyaxes` (x axis, then y axis, in world coordinates): y is
   made orthogonal to x, z = x × y; columns are the axes."""
