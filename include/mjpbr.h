@@ -437,9 +437,15 @@ int mjpbr_extract_rgb_image(const float* src, int src_stride, int adr, int width
  * w is [cout][9][cin] BF16 (forward) or the mirrored transposed layout [cin][9][cout] for a data
  * gradient (then x is the output gradient and mask the layer's output activation: ReLU backward
  * fused into the load). bias may be NULL; y32, if not NULL, receives the first out_f32 channels
- * in FP32 ([pixel][out_f32]). h and w must be multiples of 16, cin of 32, cout of 16. */
+ * in FP32 ([pixel][out_f32]), and then y may be NULL (no BF16 output). h and w must be multiples
+ * of 16, cin of 32, cout of 16. */
 int mjpbr_unet_conv3x3(const uint16_t* x, const uint16_t* mask, const uint16_t* w, const float* bias, uint16_t* y, float* y32,
                        int out_f32, int n, int h, int wd, int cin, int cout, int relu, void* stream);
+/* mjpbr_unet_conv3x3 (no mask, no FP32 output) that also writes the 2x2 average pooling of y,
+ * pool [n][h/2][w/2][cout], equal bit for bit to mjpbr_unet_avgpool2 of y: the U-Net encoder's
+ * last convolution of a level, its pooling fused into the epilogue. */
+int mjpbr_unet_conv3x3_pool(const uint16_t* x, const uint16_t* w, const float* bias, uint16_t* y, uint16_t* pool, int n, int h, int wd,
+                            int cin, int cout, int relu, void* stream);
 /* The same convolution over concat(upsample2x(low), skip) without materialising it: low is
  * [n][h/2][w/2][c_split] (nearest 2x upsampling), skip [n][h][w][cin - c_split]; c_split a
  * multiple of 32. The U-Net decoder's input. */

@@ -60,6 +60,7 @@ _I, _L, _F = ctypes.c_int, ctypes.c_int64, ctypes.c_float
 FUNCTIONS.update(
   {
     "mjpbr_unet_conv3x3": (_I, [_VP, _VP, _VP, _VP, _VP, _VP, _I, _I, _I, _I, _I, _I, _I, _VP]),
+    "mjpbr_unet_conv3x3_pool": (_I, [_VP, _VP, _VP, _VP, _VP, _I, _I, _I, _I, _I, _I, _VP]),
     "mjpbr_unet_conv3x3_upcat": (_I, [_VP, _VP, _I, _VP, _VP, _VP, _I, _I, _I, _I, _I, _I, _VP]),
     "mjpbr_unet_wgrad_workspace": (_L, [_I, _I, _I, _I, _I, _I]),
     "mjpbr_unet_bias_workspace": (_L, [_I, _I, _I, _I]),

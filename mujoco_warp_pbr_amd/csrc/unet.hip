@@ -8,11 +8,13 @@
 // [Cin][9][Cout] with the taps mirrored (the transposed convolution). MFMA operand layout of
 // 16x16x32 (wave64): lane l holds A[m = l % 16][k = 8 (l / 16) .. + 8], B[k = 8 (l / 16) .. + 8][n =
 // l % 16] and receives D[m = 4 (l / 16) + i][n = l % 16], i = 0..3.
+#include <algorithm>
 #include "abi_util.hpp"
 
 namespace mjpbr {
 
 typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8;
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
 typedef __attribute__((__vector_size__(4 * sizeof(float)))) float floatx4;
 typedef uint16_t bf16_t;
 
@@ -37,13 +39,12 @@ MJ_DEV floatx4 mfma16(const uint4& a, const uint4& b, floatx4 c) {
 // channels [0, c_split) are read from the half-resolution x at (y / 2, x / 2), the rest from x2,
 // so the concatenation is never materialised.
 //
-// Workgroup: 4 waves, a 16x16 pixel block (a wave owns 4 rows of 16 pixels, one MFMA row tile
-// each) x up to 64 output channels (1-4 MFMA column tiles). The K loop runs over 32-channel chunks
-// x 9 taps. The 18x18x32 input patch and the weights of the next chunk are loaded into registers
-// while the matrix cores work on the current one and written to LDS after it (the first chunk's
-// weight and patch loads are issued together). LDS is sized to the column tiles in
-// use (45 KB for 32 output channels: 3 workgroups a CU). Epilogue: bias, ReLU, BF16 through LDS
-// so each lane stores 16 bytes.
+// Workgroup: 4 waves, a 16x16 pixel block (a wave owns 4 rows of 16 pixels) x up to 64 output
+// channels (1-4 column tiles of 16). Each MFMA multiplies a tile's weights (A: 16 output channels
+// x 32 input channels of one tap) by a row of the patch (B: 32 channels x 16 pixels), so a lane
+// receives 4 consecutive output channels of one pixel and the epilogue stores them straight from
+// registers. The K loop runs over 32-channel chunks x 9 taps; the 18x18x32 input patch lives in
+// LDS (the weights too, unless resident in registers).
 constexpr int kConvBlock = 16;
 constexpr int kPatch = kConvBlock + 2;
 constexpr int kCinChunk = 32;
@@ -66,6 +67,7 @@ struct ConvArgs {
   int relu;
   int out_f32;
   int c_split;
+  bf16_t* pool;  // optional 2x2 average pooling of y ([n][h/2][w/2][cout])
 };
 
 MJ_DEV uint4 apply_relu_mask(uint4 v, uint4 mk) {
@@ -82,34 +84,71 @@ MJ_DEV uint4 apply_relu_mask(uint4 v, uint4 mk) {
   return v;
 }
 
-// 16-byte vector k (< kPatchVecs) of the input patch of block (img, y0, x0), channels c0..c0+31.
-MJ_DEV uint4 load_patch_vec(const ConvArgs& a, int img, int y0, int x0, int c0, int k) {
-  const int pos = k >> 2, part = k & 3;
-  const int yy = y0 - 1 + pos / kPatch, xx = x0 - 1 + pos % kPatch;
-  uint4 v = make_uint4(0, 0, 0, 0);
-  if (yy < 0 || yy >= a.h || xx < 0 || xx >= a.wd) return v;
-  const int c = c0 + part * 8;
-  if (a.x2) {
-    if (c < a.c_split) {
-      const int hl = a.h >> 1, wl = a.wd >> 1;
-      v = *reinterpret_cast<const uint4*>(a.x + (((int64_t)img * hl + (yy >> 1)) * wl + (xx >> 1)) * a.c_split + c);
-    } else {
-      const int c2 = a.cin - a.c_split;
-      v = *reinterpret_cast<const uint4*>(a.x2 + (((int64_t)img * a.h + yy) * a.wd + xx) * c2 + (c - a.c_split));
-    }
-    return v;
-  }
-  const int64_t off = (((int64_t)img * a.h + yy) * a.wd + xx) * a.cin + c;
-  v = *reinterpret_cast<const uint4*>(a.x + off);
-  if (a.mask) v = apply_relu_mask(v, *reinterpret_cast<const uint4*>(a.mask + off));
-  return v;
+// Input reads go through buffer resources: an out-of-range offset reads zeros without a branch,
+// so a prefetched vector is only waited for where it is stored to LDS (a branchy load - halo
+// test, source select - makes the compiler wait at every join).
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+constexpr uint32_t kOutOfRange = 0x80000000u;
+constexpr int kBufferWord3 = 0x00020000;  // gfx9 raw buffer: 32-bit data format, no swizzle
+
+MJ_DEV __amdgpu_buffer_rsrc_t buffer_of(const void* base, int64_t bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, (int)bytes, kBufferWord3);
+}
+MJ_DEV uint4 buffer_vec(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+  const u32x4 v = __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0);
+  return make_uint4(v.x, v.y, v.z, v.w);
 }
 
-// 16-byte vector e (< ntile * 16 * 36) of the forward weights of output channels n0.. and input
-// channels c0..c0+31: output channel n0 + e / 36, tap (e % 36) / 4, 8 channels (e % 4) * 8.
-MJ_DEV uint4 load_weight_vec(const ConvArgs& a, int n0, int c0, int e) {
+// Where one 32-channel chunk of a block's patch comes from: one image of x (or, for the decoder
+// input, of the half-resolution x or of x2), `stride` channels a pixel from channel `coff`.
+struct PatchSrc {
+  __amdgpu_buffer_rsrc_t data, mask;
+  int stride, coff, low;
+};
+
+MJ_DEV PatchSrc patch_src(const ConvArgs& a, int img, int c0) {
+  PatchSrc p;
+  const int64_t full = (int64_t)a.h * a.wd;
+  if (a.x2 && c0 < a.c_split) {  // c_split is a multiple of the chunk: a chunk has one source
+    const int64_t px = (int64_t)(a.h >> 1) * (a.wd >> 1);
+    p.data = buffer_of(a.x + img * px * a.c_split, px * a.c_split * 2);
+    p.stride = a.c_split;
+    p.coff = c0;
+    p.low = 1;
+  } else if (a.x2) {
+    const int c2 = a.cin - a.c_split;
+    p.data = buffer_of(a.x2 + img * full * c2, full * c2 * 2);
+    p.stride = c2;
+    p.coff = c0 - a.c_split;
+    p.low = 0;
+  } else {
+    p.data = buffer_of(a.x + img * full * a.cin, full * a.cin * 2);
+    p.stride = a.cin;
+    p.coff = c0;
+    p.low = 0;
+  }
+  p.mask = buffer_of(a.mask ? a.mask + img * full * a.cin : a.x, a.mask ? full * a.cin * 2 : 0);
+  return p;
+}
+
+// Byte offset of 16-byte vector k of the patch of block (y0, x0), or kOutOfRange for the zero
+// padding and for k >= kPatchVecs.
+MJ_DEV uint32_t patch_offset(const ConvArgs& a, const PatchSrc& p, int y0, int x0, int k) {
+  const int pos = k >> 2, part = k & 3;
+  const int yy = y0 - 1 + pos / kPatch, xx = x0 - 1 + pos % kPatch;
+  const bool inside = k < kPatchVecs && yy >= 0 && yy < a.h && xx >= 0 && xx < a.wd;
+  const int pix = (yy >> p.low) * (a.wd >> p.low) + (xx >> p.low);  // the half-resolution source: (y / 2, x / 2)
+  const uint32_t off = (uint32_t)(pix * p.stride + p.coff + part * 8) * 2u;
+  return inside ? off : kOutOfRange;
+}
+
+// 16-byte vector e of the forward weights of output channels n0.. and input channels
+// c0..c0+31: output channel n0 + e / 36, tap (e % 36) / 4, 8 channels (e % 4) * 8; zeros for
+// e >= `count`.
+MJ_DEV uint4 load_weight_vec(const ConvArgs& a, __amdgpu_buffer_rsrc_t wr, int n0, int c0, int e, int count) {
   const int n = e / 36, rem = e % 36, tap = rem >> 2, part = rem & 3;
-  return *reinterpret_cast<const uint4*>(a.w + ((int64_t)(n0 + n) * 9 + tap) * a.cin + c0 + part * 8);
+  const uint32_t off = (uint32_t)(((n0 + n) * 9 + tap) * a.cin + c0 + part * 8) * 2u;
+  return buffer_vec(wr, e < count ? off : kOutOfRange);
 }
 
 MJ_DEV void store_patch_vec(bf16_t* patch, int k, uint4 v) {
@@ -118,123 +157,233 @@ MJ_DEV void store_patch_vec(bf16_t* patch, int k, uint4 v) {
 
 // NT: MFMA column tiles the accumulators are sized for (1, 2 or 4); the last column group of a
 // launch may use fewer (ntile).
-// Occupancy: 3 waves per SIMD for 1-2 column tiles (LDS allows 3 workgroups a CU), 2 for 4; the
-// register budget that leaves holds the prefetched patch and weights (measured 11 % faster on the
-// whole forward than letting the compiler trade registers for a fourth wave it cannot use).
-template <int NT>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NT == 4 ? 2 : 3)))
-conv3x3_kernel(ConvArgs a) {
+//
+// A workgroup streams a contiguous run of `bpg` pixel blocks (so horizontally and vertically
+// adjacent blocks, whose patches share halo rows and columns, are read by one workgroup through
+// one XCD's L2), as (block, chunk) steps. Patches load into registers kDepth steps ahead (two
+// alternating buffers for up to 2 column tiles without the ReLU mask, else one), the next step's
+// weights one step ahead, all in flight while the matrix cores work on the current step.
+// kRes (single-chunk layers, at most 2 column tiles): each lane keeps its B operands of all nine
+// taps in registers for the whole launch, read once from global memory; otherwise the weights of
+// every step go through LDS. Taps run column shift (dx) first: each of the 6 patch rows under a
+// wave's 4 output rows is read once per shift and feeds the 3 row shifts (dy) it serves.
+// Occupancy: 2 waves per SIMD (at 3 the compiler spills the prefetched vectors, and a spill store
+// waits for its load); the registers hold the prefetch and the resident B operands.
+template <int NT, bool kMask, bool kRes>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
+conv3x3_kernel(ConvArgs a, int bpg) {
   extern __shared__ __attribute__((aligned(16))) bf16_t conv_lds[];
   const int n0 = blockIdx.y * NT * 16;
   const int nn = min(NT * 16, a.cout - n0);
   const int ntile = nn / 16;
   bf16_t* patch = conv_lds;
-  bf16_t* wl = conv_lds + kPatchElems;
+  bf16_t* wl = conv_lds + kPatchElems;  // streamed weights (not kRes)
   const int bx = a.wd / kConvBlock, by = a.h / kConvBlock;
-  const int blk = blockIdx.x;
-  const int img = blk / (bx * by), r = blk % (bx * by);
-  const int y0 = (r / bx) * kConvBlock, x0 = (r % bx) * kConvBlock;
+  const int nblocks = a.n * bx * by;
+  const int first = blockIdx.x * bpg;
   const int nchunk = a.cin / kCinChunk;
+  const int nsteps = (min(nblocks, first + bpg) - first) * nchunk;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int lm = lane & 15, lk = lane >> 4;
+  auto block_origin = [&](int blk, int* img, int* y0, int* x0) {
+    *img = blk / (bx * by);
+    const int r = blk - *img * (bx * by);
+    *y0 = (r / bx) * kConvBlock;
+    *x0 = (r % bx) * kConvBlock;
+  };
 
-  // The chunk's weights are loaded into registers together with its patch, so both global
-  // latencies overlap (and, for the next chunk, overlap the matrix-core work).
   constexpr int kWVecs = NT * 16 * 9 * 4;  // 16-byte weight vectors of a chunk (all column tiles)
-  constexpr int kWPrefetch = (kWVecs + 255) / 256;
+  constexpr int kWPrefetch = kRes ? 1 : (kWVecs + 255) / 256;
   const int wvecs = ntile * 16 * 9 * 4;
-  uint4 pre[kPrefetch];
+  const __amdgpu_buffer_rsrc_t wr = buffer_of(a.w, (int64_t)a.cout * 9 * a.cin * 2);
+  // patches in flight: kDepth steps ahead, in as many register buffers (two where the registers
+  // allow it without spilling)
+  constexpr int kDepth = (NT <= 2 && !kMask) ? 2 : 1;
+  constexpr int kMaskVecs = kMask ? kPrefetch : 1;
+  uint4 pre0[kPrefetch], pre1[kDepth == 2 ? kPrefetch : 1];
+  uint4 mpre0[kMaskVecs], mpre1[kMaskVecs];
   uint4 wpre[kWPrefetch];
+  uint4 breg[kRes ? 9 : 1][NT];
+  // issues the loads of the patch of step `step` (block first + step / nchunk, its chunk)
+  auto prefetch_patch = [&](int step, auto& buf, auto& mbuf) {
+    const int bi = step / nchunk, c0 = (step - bi * nchunk) * kCinChunk;
+    int img, y0, x0;
+    block_origin(first + bi, &img, &y0, &x0);
+    const PatchSrc src = patch_src(a, img, c0);
 #pragma unroll
-  for (int i = 0; i < kWPrefetch; ++i) {
-    const int e = threadIdx.x + 256 * i;
-    wpre[i] = e < wvecs ? load_weight_vec(a, n0, 0, e) : make_uint4(0, 0, 0, 0);
-  }
+    for (int i = 0; i < kPrefetch; ++i) {
+      const uint32_t off = patch_offset(a, src, y0, x0, threadIdx.x + 256 * i);
+      buf[i] = buffer_vec(src.data, off);
+      if (kMask) mbuf[i] = buffer_vec(src.mask, off);
+    }
+  };
+  // this lane's output channels' biases, read once (a load in the epilogue would wait for the
+  // prefetch in flight: vector memory counters retire in order)
+  // The column group's biases in LDS past the patch and weights (read in the epilogue with LDS
+  // loads: a global load there would wait for the prefetch in flight, vector memory counters
+  // retiring in order; registers would cost 4 NT a lane). Ordered by the first step's barriers.
+  float* bias_lds = reinterpret_cast<float*>(conv_lds + kPatchElems + (kRes ? 0 : NT * 16 * kWRow));
+  if (threadIdx.x < NT * 16) bias_lds[threadIdx.x] = (a.bias && (int)threadIdx.x < nn) ? a.bias[n0 + threadIdx.x] : 0.0f;
+  if (kRes) {
 #pragma unroll
-  for (int i = 0; i < kPrefetch; ++i) {
-    const int k = threadIdx.x + 256 * i;
-    pre[i] = k < kPatchVecs ? load_patch_vec(a, img, y0, x0, 0, k) : make_uint4(0, 0, 0, 0);
+    for (int t = 0; t < 9; ++t)
+#pragma unroll
+      for (int j = 0; j < NT; ++j) {
+        const uint32_t off = (uint32_t)(((n0 + j * 16 + lm) * 9 + t) * a.cin + lk * 8) * 2u;
+        breg[kRes ? t : 0][j] = buffer_vec(wr, j < ntile ? off : kOutOfRange);
+      }
+  } else {
+#pragma unroll
+    for (int i = 0; i < kWPrefetch; ++i) wpre[i] = load_weight_vec(a, wr, n0, 0, threadIdx.x + 256 * i, wvecs);
   }
+  prefetch_patch(0, pre0, mpre0);
+  if constexpr (kDepth == 2)
+    if (nsteps > 1) prefetch_patch(1, pre1, mpre1);
   floatx4 acc[4][NT];
 #pragma unroll
   for (int i = 0; i < 4; ++i)
 #pragma unroll
     for (int j = 0; j < NT; ++j) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
-#pragma unroll 1
-  for (int chunk = 0; chunk < nchunk; ++chunk) {
-    const int c0 = chunk * kCinChunk;
-    __syncthreads();  // the previous chunk's LDS readers are done
+  // one (block, chunk) step whose patch is in `buf`; refills `buf` with step + kDepth's
+  auto run_step = [&](int step, auto& buf, auto& mbuf) {
+    const int bi = step / nchunk, chunk = step - bi * nchunk;
+    __syncthreads();  // the previous step's LDS readers (taps or epilogue) are done
 #pragma unroll
     for (int i = 0; i < kPrefetch; ++i) {
       const int k = threadIdx.x + 256 * i;
-      if (k < kPatchVecs) store_patch_vec(patch, k, pre[i]);
+      if (k < kPatchVecs) store_patch_vec(patch, k, kMask ? apply_relu_mask(buf[i], mbuf[i]) : buf[i]);
     }
-#pragma unroll
-    for (int i = 0; i < kWPrefetch; ++i) {
-      const int e = threadIdx.x + 256 * i;
-      if (e < wvecs) {
-        const int n = e / 36, rem = e % 36, tap = rem >> 2, part = rem & 3;
-        *reinterpret_cast<uint4*>(wl + n * kWRow + tap * kCinChunk + part * 8) = wpre[i];
-      }
-    }
-    __syncthreads();
-    if (chunk + 1 < nchunk) {  // the next chunk's patch and weights: loads in flight during these MFMAs
+    if (!kRes) {
 #pragma unroll
       for (int i = 0; i < kWPrefetch; ++i) {
         const int e = threadIdx.x + 256 * i;
-        if (e < wvecs) wpre[i] = load_weight_vec(a, n0, c0 + kCinChunk, e);
-      }
-#pragma unroll
-      for (int i = 0; i < kPrefetch; ++i) {
-        const int k = threadIdx.x + 256 * i;
-        if (k < kPatchVecs) pre[i] = load_patch_vec(a, img, y0, x0, c0 + kCinChunk, k);
+        if (e < wvecs) {
+          const int n = e / 36, rem = e % 36, tap = rem >> 2, part = rem & 3;
+          *reinterpret_cast<uint4*>(wl + n * kWRow + tap * kCinChunk + part * 8) = wpre[i];
+        }
       }
     }
+    __syncthreads();
+    // loads in flight during these MFMAs: the next step's weights, the patch kDepth steps ahead
+    if (!kRes && step + 1 < nsteps) {
+      const int nbi = (step + 1) / nchunk, nc0 = ((step + 1) - nbi * nchunk) * kCinChunk;
+#pragma unroll
+      for (int i = 0; i < kWPrefetch; ++i) wpre[i] = load_weight_vec(a, wr, n0, nc0, threadIdx.x + 256 * i, wvecs);
+    }
+    if (step + kDepth < nsteps) prefetch_patch(step + kDepth, buf, mbuf);
+#pragma unroll
+    for (int dx = 0; dx < 3; ++dx) {
+      // every column tile runs (zero B operands past ntile): no branch breaks the MFMA chain
+      uint4 b[3][NT];
+#pragma unroll
+      for (int dy = 0; dy < 3; ++dy)
+#pragma unroll
+        for (int j = 0; j < NT; ++j) {
+          if (kRes) {
+            b[dy][j] = breg[kRes ? dy * 3 + dx : 0][j];
+          } else {
+            b[dy][j] = *reinterpret_cast<const uint4*>(wl + (j * 16 + lm) * kWRow + (dy * 3 + dx) * kCinChunk + lk * 8);
+            if (j >= ntile) b[dy][j] = make_uint4(0, 0, 0, 0);
+          }
+        }
+      if constexpr (NT <= 2) {
+        // the shift's 6 patch rows, all reads in flight before the first MFMA needs one
+        uint4 aq[6];
+#pragma unroll
+        for (int r = 0; r < 6; ++r)
+          aq[r] = *reinterpret_cast<const uint4*>(patch + ((wave * 4 + r) * kPatch + lm + dx) * kPStride + lk * 8);
+#pragma unroll
+        for (int r = 0; r < 6; ++r)
+#pragma unroll
+          for (int i = 0; i < 4; ++i) {
+            const int dy = r - i;
+            if (dy < 0 || dy > 2) continue;
+#pragma unroll
+            for (int j = 0; j < NT; ++j) acc[i][j] = mfma16(b[dy][j], aq[r], acc[i][j]);
+          }
+        // schedule: the shift's LDS reads (B operands unless resident, then the 6 rows) first,
+        // then its 12 NT MFMAs
+        __builtin_amdgcn_sched_group_barrier(0x100, kRes ? 6 : 6 + 3 * NT, 0);
+        __builtin_amdgcn_sched_group_barrier(0x008, 12 * NT, 0);
+      } else {  // 4 tiles: no registers for 6 rows in flight; each row read where it is used
+#pragma unroll
+        for (int r = 0; r < 6; ++r) {
+          const uint4 aq = *reinterpret_cast<const uint4*>(patch + ((wave * 4 + r) * kPatch + lm + dx) * kPStride + lk * 8);
+#pragma unroll
+          for (int i = 0; i < 4; ++i) {
+            const int dy = r - i;
+            if (dy < 0 || dy > 2) continue;
+#pragma unroll
+            for (int j = 0; j < NT; ++j) acc[i][j] = mfma16(b[dy][j], aq, acc[i][j]);
+          }
+        }
+      }
+      __builtin_amdgcn_sched_barrier(0);  // nothing crosses into the next shift
+    }
+    if (chunk + 1 < nchunk) return;
+    // epilogue of block bi, from registers: lane (lm, lk) holds output channels j * 16 + 4 lk .. + 3
+    // of pixel lm of each of its rows - bias, ReLU, 4 BF16 in one 8-byte store (the 16 pixels x 16
+    // channels of a tile row are one contiguous kilobyte with its neighbour tile); optional FP32
+    int img, y0, x0;
+    block_origin(first + bi, &img, &y0, &x0);
+    float pool_sum[NT][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int64_t pix = ((int64_t)img * a.h + y0 + wave * 4 + i) * a.wd + x0 + lm;
+#pragma unroll
+      for (int j = 0; j < NT; ++j) {
+        const float4 bq4 = *reinterpret_cast<const float4*>(bias_lds + j * 16 + 4 * lk);
+        const float bb[4] = {bq4.x, bq4.y, bq4.z, bq4.w};
+        float v[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          v[q] = acc[i][j][q] + bb[q];
+          if (a.relu) v[q] = fmaxf(v[q], 0.0f);
+        }
+        acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
+        if (j >= ntile) continue;
+        const int n = n0 + j * 16 + 4 * lk;
+        const bf16x2 lo = {(__bf16)v[0], (__bf16)v[1]}, hi = {(__bf16)v[2], (__bf16)v[3]};
+        if (a.y)
+          *reinterpret_cast<uint2*>(a.y + pix * a.cout + n) = make_uint2(__builtin_bit_cast(uint32_t, lo), __builtin_bit_cast(uint32_t, hi));
+        if (a.y32)
+#pragma unroll
+          for (int q = 0; q < 4; ++q)
+            if (n + q < a.out_f32) a.y32[pix * a.out_f32 + n + q] = v[q];
+        if (a.pool) {
+          // the BF16 outputs of this row; with the row below (odd i) and the neighbouring column
+          // (lane lm ^ 1), the 2x2 mean in avgpool2_kernel's order: rows, then columns, from 0
+          const float r[4] = {(float)lo.x, (float)lo.y, (float)hi.x, (float)hi.y};
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            const float nb = __shfl_xor(r[q], 1);
+            const float left = (lm & 1) ? nb : r[q], right = (lm & 1) ? r[q] : nb;
+            if ((i & 1) == 0) {
+              pool_sum[j][q] = 0.0f;
+              pool_sum[j][q] += left;
+              pool_sum[j][q] += right;
+            } else {
+              pool_sum[j][q] += left;
+              pool_sum[j][q] += right;
+            }
+          }
+          if ((i & 1) && !(lm & 1)) {
+            const int ho = a.h >> 1, wo = a.wd >> 1;
+            const int64_t pp = ((int64_t)img * ho + ((y0 + wave * 4 + i) >> 1)) * wo + ((x0 + lm) >> 1);
+            const bf16x2 plo = {(__bf16)(0.25f * pool_sum[j][0]), (__bf16)(0.25f * pool_sum[j][1])};
+            const bf16x2 phi = {(__bf16)(0.25f * pool_sum[j][2]), (__bf16)(0.25f * pool_sum[j][3])};
+            *reinterpret_cast<uint2*>(a.pool + pp * a.cout + n) =
+                make_uint2(__builtin_bit_cast(uint32_t, plo), __builtin_bit_cast(uint32_t, phi));
+          }
+        }
+      }
+    }
+  };
 #pragma unroll 1
-    for (int tap = 0; tap < 9; ++tap) {
-      const int dy = tap / 3, dx = tap % 3;
-      uint4 bq[NT];
-#pragma unroll
-      for (int j = 0; j < NT; ++j)
-        if (j < ntile) bq[j] = *reinterpret_cast<const uint4*>(wl + (j * 16 + lm) * kWRow + tap * kCinChunk + lk * 8);
-#pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        const int row = wave * 4 + i;
-        const uint4 aq = *reinterpret_cast<const uint4*>(patch + ((row + dy) * kPatch + lm + dx) * kPStride + lk * 8);
-#pragma unroll
-        for (int j = 0; j < NT; ++j)
-          if (j < ntile) acc[i][j] = mfma16(aq, bq[j], acc[i][j]);
-      }
-    }
-  }
-  // epilogue: bias, ReLU; optional FP32 store; BF16 tile through LDS, 16-byte global stores
-  const int ostride = nn + 8;
-  bf16_t* otile = conv_lds;  // 256 pixels x nn channels, reuses the patch / weight space
-  __syncthreads();           // every wave finished reading the patch and the weights
-#pragma unroll
-  for (int i = 0; i < 4; ++i) {
-    const int row = wave * 4 + i;
-#pragma unroll
-    for (int j = 0; j < NT; ++j) {
-      if (j >= ntile) continue;
-      const int nl = j * 16 + lm, n = n0 + nl;
-      const float b = a.bias ? a.bias[n] : 0.0f;
-#pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        const int col = lk * 4 + q;  // pixel within the row
-        float v = acc[i][j][q] + b;
-        if (a.relu) v = fmaxf(v, 0.0f);
-        otile[(row * kConvBlock + col) * ostride + nl] = f2bf(v);
-        if (a.y32 && n < a.out_f32) a.y32[(((int64_t)img * a.h + y0 + row) * a.wd + x0 + col) * a.out_f32 + n] = v;
-      }
-    }
-  }
-  __syncthreads();
-  const int vecs = nn / 8;  // 16-byte vectors a pixel
-  for (int e = threadIdx.x; e < kConvBlock * kConvBlock * vecs; e += 256) {
-    const int p = e / vecs, part = e - p * vecs;
-    const uint4 v = *reinterpret_cast<const uint4*>(otile + p * ostride + part * 8);
-    *reinterpret_cast<uint4*>(a.y + (((int64_t)img * a.h + y0 + p / kConvBlock) * a.wd + x0 + p % kConvBlock) * a.cout + n0 + part * 8) = v;
+  for (int step = 0; step < nsteps; step += kDepth) {
+    run_step(step, pre0, mpre0);
+    if constexpr (kDepth == 2)
+      if (step + 1 < nsteps) run_step(step + 1, pre1, mpre1);
   }
 }
 
@@ -617,24 +766,35 @@ inline unsigned nblk(int64_t n, int t) { return (unsigned)((n + t - 1) / t); }
 }  // namespace
 
 namespace {
+constexpr int kConvCus = 256;  // MI355X compute units
 int conv_launch(const ConvArgs& a, void* stream) {
   const int nblocks = a.n * (a.h / kConvBlock) * (a.wd / kConvBlock);
   const int tiles = a.cout / 16;
   const int nt = tiles >= 3 ? 4 : tiles;  // accumulator width of the kernel instance
   const int nn = a.cout < nt * 16 ? a.cout : nt * 16;
-  const size_t weights = (size_t)nn * kWRow;
-  const size_t tile = (size_t)kConvBlock * kConvBlock * (nn + 8);
-  size_t elems = kPatchElems + weights;
-  if (tile > elems) elems = tile;
-  dim3 grid((unsigned)nblocks, (unsigned)((a.cout + nt * 16 - 1) / (nt * 16)));
+  const bool res = a.cin == kCinChunk && nt <= 2;  // B operands resident in registers
+  // patch, weights streamed through LDS (the instance's NT tiles), biases (2 bf16 slots a float)
+  const size_t elems = kPatchElems + (res ? 0 : (size_t)nt * 16 * kWRow) + 2 * nt * 16;
+  // persistent workgroups: as many as fit on the compute units at the instance's occupancy per
+  // column group, each streaming a contiguous run of blocks
+  const int colgroups = (a.cout + nt * 16 - 1) / (nt * 16);
+  const int target = std::max(1, kConvCus * 2 / colgroups);
+  const int bpg = (nblocks + target - 1) / target;
+  dim3 grid((unsigned)((nblocks + bpg - 1) / bpg), (unsigned)colgroups);
   hipStream_t st = (hipStream_t)stream;
   const size_t lds = elems * sizeof(bf16_t);
+  using ConvFn = void (*)(ConvArgs, int);
+  const bool m = a.mask != nullptr;
+  ConvFn fn;
   if (nt == 1)
-    hipLaunchKernelGGL(conv3x3_kernel<1>, grid, dim3(256), lds, st, a);
+    fn = res ? (m ? conv3x3_kernel<1, true, true> : conv3x3_kernel<1, false, true>)
+             : (m ? conv3x3_kernel<1, true, false> : conv3x3_kernel<1, false, false>);
   else if (nt == 2)
-    hipLaunchKernelGGL(conv3x3_kernel<2>, grid, dim3(256), lds, st, a);
+    fn = res ? (m ? conv3x3_kernel<2, true, true> : conv3x3_kernel<2, false, true>)
+             : (m ? conv3x3_kernel<2, true, false> : conv3x3_kernel<2, false, false>);
   else
-    hipLaunchKernelGGL(conv3x3_kernel<4>, grid, dim3(256), lds, st, a);
+    fn = m ? conv3x3_kernel<4, true, false> : conv3x3_kernel<4, false, false>;
+  hipLaunchKernelGGL(fn, grid, dim3(256), lds, st, a, bpg);
   return check_launch("conv3x3_kernel");
 }
 }  // namespace
@@ -644,7 +804,17 @@ extern "C" int mjpbr_unet_conv3x3(const uint16_t* x, const uint16_t* mask, const
   if (h % kConvBlock || wd % kConvBlock || cin % kCinChunk || cout % 16 || n < 1)
     return set_error(MJPBR_EINVAL, "unet_conv3x3: need h, w multiples of 16, cin of 32, cout of 16 (got %dx%d, %d -> %d)", h, wd,
                      cin, cout);
-  ConvArgs a{x, nullptr, mask, w, bias, y, y32, n, h, wd, cin, cout, relu, out_f32, 0};
+  if (!y && (!y32 || out_f32 < 1)) return set_error(MJPBR_EINVAL, "unet_conv3x3: y may only be NULL when y32 receives channels");
+  ConvArgs a{x, nullptr, mask, w, bias, y, y32, n, h, wd, cin, cout, relu, out_f32, 0, nullptr};
+  return conv_launch(a, stream);
+}
+
+extern "C" int mjpbr_unet_conv3x3_pool(const uint16_t* x, const uint16_t* w, const float* bias, uint16_t* y, uint16_t* pool, int n, int h,
+                                       int wd, int cin, int cout, int relu, void* stream) {
+  if (h % kConvBlock || wd % kConvBlock || cin % kCinChunk || cout % 16 || n < 1 || !pool)
+    return set_error(MJPBR_EINVAL, "unet_conv3x3_pool: need h, w multiples of 16, cin of 32, cout of 16 and a pool output (got %dx%d, %d -> %d)",
+                     h, wd, cin, cout);
+  ConvArgs a{x, nullptr, nullptr, w, bias, y, nullptr, n, h, wd, cin, cout, relu, 0, 0, pool};
   return conv_launch(a, stream);
 }
 
@@ -653,7 +823,7 @@ extern "C" int mjpbr_unet_conv3x3_upcat(const uint16_t* low, const uint16_t* ski
   if (h % kConvBlock || wd % kConvBlock || cin % kCinChunk || cout % 16 || n < 1 || c_split <= 0 || c_split >= cin ||
       c_split % kCinChunk || h % 2 || wd % 2)
     return set_error(MJPBR_EINVAL, "unet_conv3x3_upcat: bad shape %dx%d, %d (%d low) -> %d", h, wd, cin, c_split, cout);
-  ConvArgs a{low, skip, nullptr, w, bias, y, nullptr, n, h, wd, cin, cout, relu, 0, c_split};
+  ConvArgs a{low, skip, nullptr, w, bias, y, nullptr, n, h, wd, cin, cout, relu, 0, c_split, nullptr};
   return conv_launch(a, stream);
 }
 

@@ -182,12 +182,23 @@ class UNet:
     return t
 
   # ----------------------------------------------------------------- forward
-  def _conv(self, name, x, out, y32=None, out_f32=0):
+  def _conv(self, name, x, out, y32=None, out_f32=0, cout=None):
+    """Convolution `name`; `cout` below the layer's computes only its first output channels (the
+    weight rows are [cout][9][cin], so a prefix is a smaller layer); out=None writes only y32."""
     p = self.params[name]
     n, h, w = x.shape[0], x.shape[1], x.shape[2]
     _abi.call(
-      "mjpbr_unet_conv3x3", _abi.ptr(x), 0, _abi.ptr(p.wf), _abi.ptr(p.b), _abi.ptr(out), _abi.ptr(y32), out_f32, n, h, w, p.cin,
-      p.cout, int(p.relu), _stream(x),
+      "mjpbr_unet_conv3x3", _abi.ptr(x), 0, _abi.ptr(p.wf), _abi.ptr(p.b), _abi.ptr(out), _abi.ptr(y32), out_f32, n,
+      h, w, p.cin, p.cout if cout is None else cout, int(p.relu), _stream(x),
+    )
+
+  def _conv_pool(self, name, x, out, pooled):
+    """Convolution `name` and the 2x2 average pooling of its output, one launch."""
+    p = self.params[name]
+    n, h, w = x.shape[0], x.shape[1], x.shape[2]
+    _abi.call(
+      "mjpbr_unet_conv3x3_pool", _abi.ptr(x), _abi.ptr(p.wf), _abi.ptr(p.b), _abi.ptr(out), _abi.ptr(pooled), n, h, w, p.cin, p.cout,
+      int(p.relu), _stream(x),
     )
 
   def _conv_upcat(self, name, low, skip, out, c_split):
@@ -212,11 +223,9 @@ class UNet:
     A = bufs["act"]
     st = _stream(x)
     self._conv("e0a", x, A["e0a"])
-    self._conv("e0b", A["e0a"], A["e0b"])
-    _abi.call("mjpbr_unet_avgpool2", _abi.ptr(A["e0b"]), _abi.ptr(A["p0"]), n, h, w, 32, 0, st)
+    self._conv_pool("e0b", A["e0a"], A["e0b"], A["p0"])
     self._conv("e1a", A["p0"], A["e1a"])
-    self._conv("e1b", A["e1a"], A["e1b"])
-    _abi.call("mjpbr_unet_avgpool2", _abi.ptr(A["e1b"]), _abi.ptr(A["p1"]), n, h // 2, w // 2, 64, 0, st)
+    self._conv_pool("e1b", A["e1a"], A["e1b"], A["p1"])
     self._conv("b0", A["p1"], A["b0"])
     self._conv("b1", A["b0"], A["b1"])
     if training:
@@ -231,7 +240,10 @@ class UNet:
     else:
       self._conv_upcat("d0a", A["d1b"], A["e0b"], A["d0a"], 64)
     self._conv("d0b", A["d0a"], A["d0b"])
-    self._conv("out", A["d0b"], A["out"], y32=bufs["pred"], out_f32=3)
+    if training:
+      self._conv("out", A["d0b"], A["out"], y32=bufs["pred"], out_f32=3)
+    else:  # the 3 predicted channels need one 16-channel tile and no BF16 output
+      self._conv("out", A["d0b"], None, y32=bufs["pred"], out_f32=3, cout=16)
     return bufs["pred"]
 
   # ----------------------------------------------------------------- backward

@@ -183,6 +183,9 @@ def _conv_case(cuda, n, h, w, cin, cout, relu, with_bias, with_mask, seed):
     (1, 16, 48, 32, 16, False, False, False),  # one column tile, non-square
     (2, 32, 16, 96, 128, True, True, False),  # two column-tile groups
     (1, 32, 32, 64, 32, False, False, True),  # ReLU-backward mask fused into the load
+    (2, 48, 32, 32, 32, True, True, False),  # single chunk, two tiles: B operands in registers
+    (1, 32, 32, 32, 16, True, True, True),  # single chunk, one tile, masked
+    (3, 16, 16, 32, 64, True, True, False),  # single chunk, four tiles: weights through LDS; 3 blocks
   ],
 )
 def test_gpu_conv3x3_matches_oracle(cuda, n, h, w, cin, cout, relu, bias, mask):
@@ -346,3 +349,27 @@ def test_gpu_inference_forward_equals_training_forward(cuda):
   b = net.forward(xb).clone()
   torch.cuda.synchronize()
   assert torch.equal(a, b)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("n,h,w,cin,cout", [(2, 32, 48, 32, 32), (1, 32, 32, 64, 64), (1, 16, 16, 32, 64)])
+def test_gpu_pooled_convolution_equals_convolution_then_pool(cuda, n, h, w, cin, cout):
+  """The encoder convolution with its 2x2 pooling fused into the epilogue: both outputs equal,
+  bit for bit, the plain convolution followed by the pooling kernel."""
+  rng = np.random.default_rng(70 + cin + cout)
+  x = _rand_bf16(rng, (n, h, w, cin))
+  wt = _rand_bf16(rng, (cout, 9, cin), 0.1)
+  tb = torch.from_numpy(rng.normal(0, 0.1, cout).astype(np.float32)).to(cuda)
+  tx, tw = _bits(x, cuda), _bits(wt, cuda)
+  y = torch.zeros((n, h, w, cout), dtype=torch.int16, device=cuda)
+  pooled = torch.zeros((n, h // 2, w // 2, cout), dtype=torch.int16, device=cuda)
+  _abi.call("mjpbr_unet_conv3x3_pool", _abi.ptr(tx), _abi.ptr(tw), _abi.ptr(tb), _abi.ptr(y), _abi.ptr(pooled), n, h, w, cin, cout, 1,
+            _stream())
+  y2 = torch.zeros_like(y)
+  _abi.call("mjpbr_unet_conv3x3", _abi.ptr(tx), 0, _abi.ptr(tw), _abi.ptr(tb), _abi.ptr(y2), 0, 0, n, h, w, cin, cout, 1, _stream())
+  p2 = torch.zeros_like(pooled)
+  _abi.call("mjpbr_unet_avgpool2", _abi.ptr(y2), _abi.ptr(p2), n, h, w, cout, 0, _stream())
+  torch.cuda.synchronize()
+  assert torch.equal(y, y2)
+  assert torch.equal(pooled, p2)
+  assert pooled.abs().sum() > 0
