@@ -15,7 +15,9 @@ worlds of the timed render against the oracle's images: segmentation bit-exact, 
 relative, sRGB8 within 1), `small_case` (16 worlds x 64^2 on the GPU and on every host core the
 way the reference's CPU benchmark runs), and `reference_suite` / `reference_suite_path_traced`
 (the reference suite's procedural scenes next to its published rates). Path-tracing sections
-follow in the same line (skip with --no-pt):
+follow in the same line (skip with --no-pt); their Mrays/s count every ray a render traced
+(path segments, light and environment shadow rays: one untimed render through the kernels'
+counting instances) over the timed render:
   pt_wavefront      4096 worlds at mixed resolutions (2048 x 64^2, 1536 x 128^2, 512 x 256^2),
                     4 spp, depth 8: the persistent wavefront tracer (autotuned, graph replay)
                     against the per-pixel megakernel, images identical;
@@ -349,12 +351,16 @@ def pt_suite_section(device, nworld: int = 1024, res: int = 128, reps: int = 3) 
     b.synchronize()
     ms = a.elapsed_time(b) / reps
     mpx = nworld * res * res / ms / 1e3
+    rays = mj.count_path_rays(m, d, rc)  # one untimed render through the counting kernel instance
     pub = PT_SUITE_PUBLISHED_MPX[name]
-    log(f"suite PT {name}: {ms:.1f} ms = {mpx:.1f} Mpx/s (reference RTX PRO 6000 {pub['rtx_pro_6000']:.1f})")
+    log(f"suite PT {name}: {ms:.1f} ms = {mpx:.1f} Mpx/s, {rays['total'] / ms / 1e3:.0f} Mrays/s "
+        f"(reference RTX PRO 6000 {pub['rtx_pro_6000']:.1f} Mpx/s)")
     out[name] = {
       "ms": round(ms, 3),
       "Mpixels_s": round(mpx, 2),
       "Mpaths_s": round(mpx * 16, 1),
+      "rays_per_render": rays,
+      "Mrays_s": round(rays["total"] / ms / 1e3, 1),
       "reference_published_Mpixels_s": {k: round(v, 2) for k, v in pub.items()},
       "vs_rtx_pro_6000": round(mpx / pub["rtx_pro_6000"], 3),
       "vs_h200": round(mpx / pub["h200"], 3),
@@ -422,7 +428,9 @@ def pt_wavefront_section(device) -> dict:
   graph = batch.capture(seed=3, config=cfg)
   wf_ms = timed(graph.replay)
   identical = all(torch.equal(r, rc.base.rgb_data) for r, (_, _, rc) in zip(ref, groups))
-  log(f"pt wavefront {wf_ms:.1f} ms vs megakernel {mega_ms:.1f} ms, identical images: {identical}")
+  rays = batch.count_rays(seed=3, config=cfg)  # untimed; the megakernel traces the same rays
+  log(f"pt wavefront {wf_ms:.1f} ms vs megakernel {mega_ms:.1f} ms, identical images: {identical}; "
+      f"{rays['total'] / 1e6:.0f} M rays: {rays['total'] / wf_ms / 1e3:.0f} vs {rays['total'] / mega_ms / 1e3:.0f} Mrays/s")
   return {
     "workload": "path tracing, full-feature scene, 4096 worlds: 2048 x 64^2 + 1536 x 128^2 + 512 x 256^2, 4 spp, depth 8, RR from 3",
     "pixels": npix,
@@ -433,6 +441,9 @@ def pt_wavefront_section(device) -> dict:
     "wavefront_Mpaths_s": round(paths / wf_ms / 1e3, 1),
     "wavefront_Mpixels_s": round(npix / wf_ms / 1e3, 1),
     "speedup": round(mega_ms / wf_ms, 3),
+    "rays_per_render": rays,
+    "wavefront_Mrays_s": round(rays["total"] / wf_ms / 1e3, 1),
+    "megakernel_Mrays_s": round(rays["total"] / mega_ms / 1e3, 1),
     "identical_to_megakernel": identical,
     "config": {k: getattr(cfg, k) for k in ("waves_per_eu", "groups_per_cu", "refill_threshold")},
     "autotune_table": table,

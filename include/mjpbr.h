@@ -266,6 +266,10 @@ typedef struct MjpbrOutputs {
   float* aov_depth;
   float* aov_emission;    /* x3 */
   float* aov_variance;
+  /* Optional [3]: mjpbr_pt_render adds the rays it traced - path segments (closest hit), light
+   * shadow rays, environment shadow rays - through a counting instance of its kernel (same
+   * image, not for timing). NULL: not counted. */
+  uint64_t* ray_counts;
 } MjpbrOutputs;
 
 /* One frame's inputs. */
@@ -302,6 +306,10 @@ typedef struct MjpbrWavefrontConfig {
   int32_t groups_per_cu;     /* persistent workgroups per compute unit (1..16) */
   int32_t refill_threshold;  /* a wave refills its lanes once this many paths ended (1..64) */
   int32_t item_limit;        /* > 0: only the first item_limit tiles (autotuner samples) */
+  int32_t pad0;
+  /* Optional [3]: rays the launch traced, added as MjpbrOutputs.ray_counts (a counting instance
+   * of the kernel). NULL: not counted. */
+  uint64_t* ray_counts;
 } MjpbrWavefrontConfig;
 
 /* What mjpbr_pt_wavefront_prepare recorded for the launches (host memory, filled by prepare). */

@@ -25,6 +25,7 @@ from mujoco_warp_pbr_amd.output import get_rgb
 from mujoco_warp_pbr_amd.output import get_segmentation
 from mujoco_warp_pbr_amd.rays import CameraRays
 from mujoco_warp_pbr_amd.rays import set_camera_rays
+from mujoco_warp_pbr_amd.render import count_path_rays
 from mujoco_warp_pbr_amd.render import render
 from mujoco_warp_pbr_amd.bvh import refit_bvh
 from mujoco_warp_pbr_amd.types import LightModel
@@ -59,6 +60,7 @@ __all__ = [
   "put_data",
   "put_model",
   "refit_bvh",
+  "count_path_rays",
   "render",
   "render_batch",
   "set_camera_rays",

@@ -41,13 +41,33 @@ MJ_DEV float power_heuristic(float a, float b) {
   return a2 / fmaxf(a2 + b * b, 1.0e-30f);
 }
 
+// Rays one lane traced. The counting kernel instances add them up (flush_ray_counts); elsewhere
+// the counts are never read and the compiler drops them.
+struct RayCount {
+  uint32_t segments, lights, env;
+};
+
 struct PtCtx {
   const MjpbrScene& sc;
   const MjpbrSettings& s;
   const SceneView& sv;
   int w;
   float exposure;
+  RayCount* rays;
 };
+
+// Adds the wave's counts to counts[0..2] (segments, light shadow rays, environment shadow rays).
+// Every lane of the wave must be active.
+MJ_DEV void flush_ray_counts(uint64_t* counts, const RayCount& r) {
+  const uint32_t v[3] = {r.segments, r.lights, r.env};
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    unsigned long long x = v[k];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off);
+    if ((threadIdx.x & 63) == 0) atomicAdd(reinterpret_cast<unsigned long long*>(counts) + k, x);
+  }
+}
 
 // First-hit record of a path (pathtrace.py:98-108).
 struct Primary {
@@ -72,6 +92,7 @@ MJ_DEV V3 light_sample(const PtCtx& c, Light L, const Lobes& lb, V3 point, V3 no
   if (ll.z <= 0.0f || (len_sq(ds) == 0.0f && len_sq(ss) == 0.0f)) return v3s(0.0f);
   if (c.s.use_shadows && L.castshadow) {
     float max_t = L.type != MJPBR_LIGHT_DIRECTIONAL ? dist - kShadowGap : kUnboundedShadow;
+    ++c.rays->lights;
     if (trace_any(c.sv, point + normal * kRayOffset, l, max_t, c.s.cull_backfaces != 0)) return v3s(0.0f);
   }
   V3 diffuse = eval_bsdf_cos(lb.diffuse_color, v3s(0.0f), lb.alpha, v3s(0.0f), v, ll);
@@ -139,6 +160,7 @@ MJ_DEV bool path_step(const PtCtx& c, PathState* st, V3 head_dir, int seed, int 
   const MjpbrEnvironment& env = c.sc.env;
   const bool cull = s.cull_backfaces != 0;
   const float footprint = 0.0f;
+  ++c.rays->segments;
   Hit hit = trace_closest(c.sv, st->origin, st->dir, st->max_dist, cull);
   if (st->camera_ray && st->passes == 0) first->hit = hit;
   if (hit.geom_id == -1) {
@@ -215,6 +237,7 @@ MJ_DEV bool path_step(const PtCtx& c, PathState* st, V3 head_dir, int seed, int 
     V3 l_env = V3{dot(d_env, t1), dot(d_env, t2), dot(d_env, normal)};
     if (l_env.z > 0.0f && pdf_env > 0.0f) {
       V3 f = eval_bsdf_cos(lb.diffuse_color, lb.f0, alpha, lb.energy_comp, v, l_env);
+      ++c.rays->env;
       if (!trace_any(c.sv, point + normal * kRayOffset, d_env, kUnboundedShadow, cull)) {
         float weight = power_heuristic(pdf_env, pdf_bsdf(alpha, p_spec, v, l_env));
         st->radiance += cw(st->beta, cw(f, environment(c, d_env))) * (weight / pdf_env);

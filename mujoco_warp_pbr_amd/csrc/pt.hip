@@ -15,7 +15,8 @@ namespace mjpbr {
 // 8 waves per SIMD with spills beat 2 without: the path loop is latency bound (measured 1.8x on
 // the full-feature scene at 1024 x 128^2, 1 sample).
 // kGlobal: the world's records are read in place from geo.world_stage (too large for the LDS).
-template <bool kGlobal>
+// kCount: adds the rays traced to out.ray_counts (the same image).
+template <bool kGlobal, bool kCount>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8)))
 pt_kernel(MjpbrScene sc, MjpbrSettings s, MjpbrOutputs out, int seed, TileMap tm) {
   extern __shared__ float lds[];
@@ -32,70 +33,77 @@ pt_kernel(MjpbrScene sc, MjpbrSettings s, MjpbrOutputs out, int seed, TileMap tm
   int py = ty * kTile + (wave >> 1) * 8 + (lane >> 3);
   SceneView sv = kGlobal ? global_world(sc, w, lds, tm.mesh_cache_nodes, tm.mesh_cache_n)
                         : stage_world(sc, w, lds, tm.mesh_cache_nodes, tm.mesh_cache_n);
-  Pixel pix;
-  if (!setup_pixel(sc.cams, out, s, w, cidx, px, py, tm.ray_offset[cidx], &pix)) return;
-  float exposure = sc.env.exposure[w % sc.env.exposure_n];
-  PtCtx c{sc, s, sv, w, exposure};
-  const int centre = kSubpixel / 2;
-  if (!pix.want_rgb) {
-    V3 o, d, dc;
-    float md;
-    primary_ray(pix, s, kSubpixel, centre, centre, &o, &d, &md, &dc);
-    Hit hit = trace_closest(sv, o, d, md, s.cull_backfaces != 0);
-    write_geometry(out, w, pix, hit, dc);
-    return;
-  }
-  V3 head_dir = -V3{pix.xmat[2], pix.xmat[5], pix.xmat[8]};
-  int pseed = seed + w * sc.cams.total_rays + pix.rayid;
-  V3 hdr = v3s(0.0f), albedo_sum = v3s(0.0f), normal_sum = v3s(0.0f), emission_sum = v3s(0.0f);
-  float first_depth = 0.0f, lum_sq = 0.0f;
-  int nsample = s.path_samples;
-  for (int smp = 0; smp < nsample; ++smp) {
-    V3 o, d, dc;
-    float md;
-    path_primary(pix, s, pseed, smp, &o, &d, &md, &dc);
-    Primary first;
-    V3 radiance = trace_path(c, o, d, md, head_dir, pseed, smp, &first);
-    if (smp == 0) {
-      write_geometry(out, w, pix, first.hit, dc);
-      if (s.render_guides) {
-        int64_t a = (int64_t)w * out.guide_stride + out.rgb_adr[pix.camid] + pix.pixelid;
-        st3(out.albedo + 3 * a, first.base_color);
-        st3(out.normal + 3 * a, first.normal);
+  RayCount rays{0u, 0u, 0u};
+  // the pixel's work; its early returns end here, where every lane of the wave is active again
+  auto render_pixel = [&]() {
+    Pixel pix;
+    if (!setup_pixel(sc.cams, out, s, w, cidx, px, py, tm.ray_offset[cidx], &pix)) return;
+    float exposure = sc.env.exposure[w % sc.env.exposure_n];
+    PtCtx c{sc, s, sv, w, exposure, &rays};
+    const int centre = kSubpixel / 2;
+    if (!pix.want_rgb) {
+      V3 o, d, dc;
+      float md;
+      primary_ray(pix, s, kSubpixel, centre, centre, &o, &d, &md, &dc);
+      ++rays.segments;
+      Hit hit = trace_closest(sv, o, d, md, s.cull_backfaces != 0);
+      write_geometry(out, w, pix, hit, dc);
+      return;
+    }
+    V3 head_dir = -V3{pix.xmat[2], pix.xmat[5], pix.xmat[8]};
+    int pseed = seed + w * sc.cams.total_rays + pix.rayid;
+    V3 hdr = v3s(0.0f), albedo_sum = v3s(0.0f), normal_sum = v3s(0.0f), emission_sum = v3s(0.0f);
+    float first_depth = 0.0f, lum_sq = 0.0f;
+    int nsample = s.path_samples;
+    for (int smp = 0; smp < nsample; ++smp) {
+      V3 o, d, dc;
+      float md;
+      path_primary(pix, s, pseed, smp, &o, &d, &md, &dc);
+      Primary first;
+      V3 radiance = trace_path(c, o, d, md, head_dir, pseed, smp, &first);
+      if (smp == 0) {
+        write_geometry(out, w, pix, first.hit, dc);
+        if (s.render_guides) {
+          int64_t a = (int64_t)w * out.guide_stride + out.rgb_adr[pix.camid] + pix.pixelid;
+          st3(out.albedo + 3 * a, first.base_color);
+          st3(out.normal + 3 * a, first.normal);
+        }
+        if (first.hit.geom_id != -1) first_depth = first.hit.dist * -dc.z;
       }
-      if (first.hit.geom_id != -1) first_depth = first.hit.dist * -dc.z;
+      hdr += radiance;
+      albedo_sum += first.albedo;
+      normal_sum += first.normal;
+      emission_sum += first.emission;
+      if (s.denoise) {
+        float lum = luminance(radiance) * exposure;
+        lum_sq += lum * lum;
+      }
     }
-    hdr += radiance;
-    albedo_sum += first.albedo;
-    normal_sum += first.normal;
-    emission_sum += first.emission;
+    float fn = (float)nsample;
+    hdr = hdr / fn;
+    int64_t adr = out.rgb_adr[pix.camid] + pix.pixelid;
     if (s.denoise) {
-      float lum = luminance(radiance) * exposure;
-      lum_sq += lum * lum;
+      int64_t a = (int64_t)w * out.aov_stride + adr;
+      st3(out.hdr + 3 * ((int64_t)w * out.hdr_stride + adr), hdr);
+      V3 alb = albedo_sum / fn;
+      out.aov_albedo[3 * a] = float_to_half(alb.x);
+      out.aov_albedo[3 * a + 1] = float_to_half(alb.y);
+      out.aov_albedo[3 * a + 2] = float_to_half(alb.z);
+      st3(out.aov_emission + 3 * a, emission_sum * (exposure / fn));
+      if (len_sq(normal_sum) > 0.0f) normal_sum = normalize(normal_sum);
+      out.aov_normal[3 * a] = float_to_half(normal_sum.x);
+      out.aov_normal[3 * a + 1] = float_to_half(normal_sum.y);
+      out.aov_normal[3 * a + 2] = float_to_half(normal_sum.z);
+      out.aov_depth[a] = first_depth;
+      float mean_lum = luminance(hdr) * exposure;
+      out.aov_variance[a] = fmaxf(lum_sq / fn - mean_lum * mean_lum, 0.0f) / fn;
+      return;
     }
-  }
-  float fn = (float)nsample;
-  hdr = hdr / fn;
-  int64_t adr = out.rgb_adr[pix.camid] + pix.pixelid;
-  if (s.denoise) {
-    int64_t a = (int64_t)w * out.aov_stride + adr;
-    st3(out.hdr + 3 * ((int64_t)w * out.hdr_stride + adr), hdr);
-    V3 alb = albedo_sum / fn;
-    out.aov_albedo[3 * a] = float_to_half(alb.x);
-    out.aov_albedo[3 * a + 1] = float_to_half(alb.y);
-    out.aov_albedo[3 * a + 2] = float_to_half(alb.z);
-    st3(out.aov_emission + 3 * a, emission_sum * (exposure / fn));
-    if (len_sq(normal_sum) > 0.0f) normal_sum = normalize(normal_sum);
-    out.aov_normal[3 * a] = float_to_half(normal_sum.x);
-    out.aov_normal[3 * a + 1] = float_to_half(normal_sum.y);
-    out.aov_normal[3 * a + 2] = float_to_half(normal_sum.z);
-    out.aov_depth[a] = first_depth;
-    float mean_lum = luminance(hdr) * exposure;
-    out.aov_variance[a] = fmaxf(lum_sq / fn - mean_lum * mean_lum, 0.0f) / fn;
-    return;
-  }
-  if (s.render_hdr) st3(out.hdr + 3 * ((int64_t)w * out.hdr_stride + adr), hdr);
-  out.rgb[(int64_t)w * out.rgb_stride + adr] = encode_display(display_transform(hdr * exposure, s.tone_mapper));
+    if (s.render_hdr) st3(out.hdr + 3 * ((int64_t)w * out.hdr_stride + adr), hdr);
+    out.rgb[(int64_t)w * out.rgb_stride + adr] = encode_display(display_transform(hdr * exposure, s.tone_mapper));
+  };
+  render_pixel();
+  if (kCount) flush_ray_counts(out.ray_counts, rays);
 }
 
 }  // namespace mjpbr
@@ -116,9 +124,9 @@ extern "C" int mjpbr_pt_render(const MjpbrScene* scene, const MjpbrSettings* s, 
   size_t lds = global ? 0 : host_stage_bytes(scene);
   lds += plan_mesh_cache(scene, s->has_meshes != 0, lds, &tm);
   if ((rc = launch_stage_prep(scene, (hipStream_t)stream))) return rc;
-  if (global)
-    hipLaunchKernelGGL(pt_kernel<true>, dim3(nlaunch), dim3(256), lds, (hipStream_t)stream, *scene, *s, *out, seed, tm);
-  else
-    hipLaunchKernelGGL(pt_kernel<false>, dim3(nlaunch), dim3(256), lds, (hipStream_t)stream, *scene, *s, *out, seed, tm);
+  using PtFn = void (*)(MjpbrScene, MjpbrSettings, MjpbrOutputs, int, TileMap);
+  const bool count = out->ray_counts != nullptr;
+  PtFn fn = global ? (count ? pt_kernel<true, true> : pt_kernel<true, false>) : (count ? pt_kernel<false, true> : pt_kernel<false, false>);
+  hipLaunchKernelGGL(fn, dim3(nlaunch), dim3(256), lds, (hipStream_t)stream, *scene, *s, *out, seed, tm);
   return check_launch("pt_kernel");
 }

@@ -57,9 +57,11 @@ __global__ void wf_reset_kernel(int* counter) {
   if (threadIdx.x == 0) counter[0] = 0;
 }
 
-template <int kWavesPerEu>
+// kCount: adds the rays traced to ray_counts (the same image).
+template <int kWavesPerEu, bool kCount>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kWavesPerEu, kWavesPerEu)))
-wf_kernel(const WfHeader* __restrict__ hdr, int* counter, float* slots, MjpbrSettings s, int seed, int refill, int64_t item_limit) {
+wf_kernel(const WfHeader* __restrict__ hdr, int* counter, float* slots, MjpbrSettings s, int seed, int refill, int64_t item_limit,
+          uint64_t* ray_counts) {
   extern __shared__ float lds[];
   __shared__ int s_item;
   __shared__ int s_pool;
@@ -67,6 +69,7 @@ wf_kernel(const WfHeader* __restrict__ hdr, int* counter, float* slots, MjpbrSet
   const int64_t nitems = item_limit > 0 && item_limit < hdr->nitems ? item_limit : hdr->nitems;
   const int spp = s.path_samples;
   const int pool = kTilePixels * spp;
+  RayCount rays{0u, 0u, 0u};
   for (;;) {
     __syncthreads();  // every wave is done with the previous item's LDS and pool
     if (threadIdx.x == 0) {
@@ -88,7 +91,7 @@ wf_kernel(const WfHeader* __restrict__ hdr, int* counter, float* slots, MjpbrSet
     const int tx = tl % G.tm.tiles_x[cidx], ty = tl / G.tm.tiles_x[cidx];
     SceneView sv = stage_world(sc, w, lds, G.tm.mesh_cache_nodes, G.tm.mesh_cache_n);
     const float exposure = sc.env.exposure[w % sc.env.exposure_n];
-    PtCtx c{sc, s, sv, w, exposure};
+    PtCtx c{sc, s, sv, w, exposure, &rays};
     const int total_rays = sc.cams.total_rays;
     const int ray_offset = G.tm.ray_offset[cidx];
     const float* xm = sc.cams.cam_xmat + ((int64_t)w * sc.cams.ncam + sc.cams.cam_id_map[cidx]) * 9;
@@ -124,6 +127,7 @@ wf_kernel(const WfHeader* __restrict__ hdr, int* counter, float* slots, MjpbrSet
                 if (sample == 0) {
                   const int centre = kSubpixel / 2;
                   primary_ray(pix, s, kSubpixel, centre, centre, &o, &d, &md, &dc);
+                  ++rays.segments;
                   Hit hit = trace_closest(sv, o, d, md, s.cull_backfaces != 0);
                   write_geometry(G.out, w, pix, hit, dc);
                 }
@@ -162,6 +166,7 @@ wf_kernel(const WfHeader* __restrict__ hdr, int* counter, float* slots, MjpbrSet
       }
     }
   }
+  if (kCount) flush_ray_counts(ray_counts, rays);  // the item loop's exit is uniform: every lane is here
 }
 
 // Mean of each pixel's samples in sample order, HDR and display colour (pathtrace.py:420-438).
@@ -290,18 +295,21 @@ extern "C" int mjpbr_pt_wavefront_launch(void* workspace, const MjpbrWavefrontPl
   int64_t grid = (int64_t)device_cus() * cfg->groups_per_cu;
   if (grid > items) grid = items;
   const MjpbrSettings& s = plan->settings;
+  using WfFn = void (*)(const WfHeader*, int*, float*, MjpbrSettings, int, int, int64_t, uint64_t*);
+  const bool count = cfg->ray_counts != nullptr;
+  WfFn fn;
   switch (cfg->waves_per_eu) {
     case 4:
-      hipLaunchKernelGGL(wf_kernel<4>, dim3((unsigned)grid), dim3(256), plan->lds_bytes, st, hdr, counter, slots, s, seed, refill,
-                         (int64_t)cfg->item_limit);
+      fn = count ? wf_kernel<4, true> : wf_kernel<4, false>;
       break;
     case 8:
-      hipLaunchKernelGGL(wf_kernel<8>, dim3((unsigned)grid), dim3(256), plan->lds_bytes, st, hdr, counter, slots, s, seed, refill,
-                         (int64_t)cfg->item_limit);
+      fn = count ? wf_kernel<8, true> : wf_kernel<8, false>;
       break;
     default:
       return set_error(MJPBR_EINVAL, "wavefront: waves_per_eu must be 4 or 8, got %d", cfg->waves_per_eu);
   }
+  hipLaunchKernelGGL(fn, dim3((unsigned)grid), dim3(256), plan->lds_bytes, st, hdr, counter, slots, s, seed, refill, (int64_t)cfg->item_limit,
+                     cfg->ray_counts);
   rc = check_launch("wf_kernel");
   if (rc) return rc;
   hipLaunchKernelGGL(wf_resolve_kernel, dim3((unsigned)((plan->npixels + 255) / 256)), dim3(256), 0, st, hdr, slots, s);

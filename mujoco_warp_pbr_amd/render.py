@@ -288,7 +288,28 @@ def outputs_struct(rc: PbrRenderContext):
   o.aov_depth = _abi.ptr(rc.aov_depth)
   o.aov_emission = _abi.ptr(rc.aov_emission)
   o.aov_variance = _abi.ptr(rc.aov_variance)
+  o.ray_counts = _abi.ptr(rc._host_cache.get("ray_counts"))
   return o
+
+
+RAY_KINDS = ("segments", "light_shadow", "env_shadow")
+
+
+def count_path_rays(m, d, rc: PbrRenderContext, seed: int = 0) -> dict:
+  """Renders once with the path tracer's counting kernel instance (the same image as
+  render(m, d, rc, seed)) and returns the rays it traced: path segments (closest-hit rays), light
+  shadow rays and environment shadow rays, and their total. Path-traced tier only."""
+  if rc.tier != Tier.PATH:
+    raise ValueError("count_path_rays needs a path-traced context (tier=Tier.PATH)")
+  counts = torch.zeros(3, dtype=torch.int64, device=rc.base.rgb_data.device)
+  rc._host_cache["ray_counts"] = counts
+  try:
+    render(m, d, rc, seed=seed)
+  finally:
+    rc._host_cache.pop("ray_counts", None)
+  out = dict(zip(RAY_KINDS, (int(x) for x in counts.cpu().tolist())))
+  out["total"] = sum(out[k] for k in RAY_KINDS)
+  return out
 
 
 def check_env_id(rc: PbrRenderContext) -> None:

@@ -35,6 +35,8 @@ class WavefrontConfig:
   groups_per_cu: int = 8
   refill_threshold: int = 32
   item_limit: int = 0
+  # int64 device tensor of 3: the launch adds the rays it traced (a counting kernel instance)
+  ray_counts: object = dataclasses.field(default=None, compare=False, hash=False)
 
   def to_c(self):
     c = _abi.structs()["MjpbrWavefrontConfig"]()
@@ -42,6 +44,7 @@ class WavefrontConfig:
     c.groups_per_cu = self.groups_per_cu
     c.refill_threshold = self.refill_threshold
     c.item_limit = self.item_limit
+    c.ray_counts = _abi.ptr(self.ray_counts)
     return c
 
 
@@ -130,6 +133,17 @@ class WavefrontBatch:
     if self._key != bytes(scenes) + bytes(outs) + bytes(settings):
       self.prepare()
     self._launch(seed, config or self.config)
+
+  def count_rays(self, seed: int = 0, config: WavefrontConfig | None = None) -> dict:
+    """Renders once with the counting kernel instance (the same images as render()) and returns
+    the rays traced: path segments, light and environment shadow rays, and their total."""
+    from mujoco_warp_pbr_amd.render import RAY_KINDS
+
+    counts = torch.zeros(3, dtype=torch.int64, device=self.device)
+    self.render(seed, dataclasses.replace(config or self.config, ray_counts=counts))
+    out = dict(zip(RAY_KINDS, (int(x) for x in counts.cpu().tolist())))
+    out["total"] = sum(out[k] for k in RAY_KINDS)
+    return out
 
   def capture(self, seed: int = 0, config: WavefrontConfig | None = None) -> torch.cuda.CUDAGraph:
     """A graph of one render (counter reset, persistent kernel, resolve); replay() re-renders with

@@ -140,3 +140,45 @@ def test_gpu_autotuner_searches_and_caches(cuda, tmp_path, monkeypatch):
   batch.render(seed=2, config=best)
   for i, (_, _, rc) in enumerate(groups):
     _assert_same(_outputs(rc), a[i], f"tuned group {i}")
+
+
+@pytest.mark.gpu
+def test_gpu_ray_counts_agree_between_kernels_and_leave_images_unchanged(cuda):
+  """The counting kernel instances render the same images and count the same rays in the
+  per-pixel megakernel and the wavefront tracer (the same paths); every path traces at least its
+  camera segment, and each light and the sky add shadow rays."""
+  groups = [_group(cuda, n, res, 10 * i) for i, (n, res) in enumerate(GROUPS)]
+  mega = {k: 0 for k in ("segments", "light_shadow", "env_shadow", "total")}
+  ref = []
+  for m, d, rc in groups:
+    mj.render(m, d, rc, seed=5)
+    ref.append(_outputs(rc))
+    _clear(rc)
+    for k, v in mj.count_path_rays(m, d, rc, seed=5).items():
+      mega[k] += v
+  for i, (_, _, rc) in enumerate(groups):
+    _assert_same(_outputs(rc), ref[i], f"counting megakernel, group {i}")
+    _clear(rc)
+  batch = WavefrontBatch(groups)
+  wf = batch.count_rays(seed=5)
+  for i, (_, _, rc) in enumerate(groups):
+    _assert_same(_outputs(rc), ref[i], f"counting wavefront, group {i}")
+  assert wf == mega
+  paths = sum(rc.base.nworld * rc.base.total_rays for _, _, rc in groups) * groups[0][2].path_samples
+  assert mega["segments"] >= paths
+  assert mega["light_shadow"] > 0 and mega["env_shadow"] > 0
+  assert mega["total"] == mega["segments"] + mega["light_shadow"] + mega["env_shadow"]
+
+
+@pytest.mark.gpu
+def test_gpu_ray_counts_of_camera_segments_only(cuda):
+  """max_depth 0: a path ends at its first hit before any next-event estimation, so the count is
+  exactly one segment a path and no shadow rays."""
+  m, d, rc = make(_scene(), 3, device=cuda, cam_res=(32, 32), envmap=scenes.procedural_sky(1, 32, 64),
+                  **{**PT, "max_depth": 0, "path_samples": 3})
+  mj.refit_bvh(m, d, rc)
+  counts = mj.count_path_rays(m, d, rc, seed=2)
+  assert counts == {"segments": 3 * rc.base.total_rays * 3, "light_shadow": 0, "env_shadow": 0, "total": 3 * rc.base.total_rays * 3}
+  with pytest.raises(ValueError, match="path-traced"):
+    m2, d2, rc2 = make(_scene(), 1, device=cuda, cam_res=(32, 32))
+    mj.count_path_rays(m2, d2, rc2)
