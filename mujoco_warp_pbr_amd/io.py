@@ -16,6 +16,7 @@ from mujoco_warp_pbr_amd import bvh as bvh_lib
 from mujoco_warp_pbr_amd import envmap as envmap_lib
 from mujoco_warp_pbr_amd import material
 from mujoco_warp_pbr_amd import model as mdl
+from mujoco_warp_pbr_amd.mjmodel import as_host_model
 from mujoco_warp_pbr_amd import texture
 from mujoco_warp_pbr_amd.types import LightModel
 from mujoco_warp_pbr_amd.types import PbrRenderContext
@@ -131,7 +132,7 @@ def _envmap_images(envmap) -> list[np.ndarray]:
 
 
 def create_render_context(
-  mjm: mdl.HostModel,
+  mjm,
   nworld: int = 1,
   cam_res=None,
   render_rgb=None,
@@ -184,10 +185,12 @@ def create_render_context(
 ) -> PbrRenderContext:
   """Creates a PBR render context on `device` (reference io.py:331-634; same arguments).
 
-  Differences from the reference: the model is a `HostModel`; `render_rgb/depth/seg=None`
-  render all three outputs; BVH construction options (`mesh_bvh_constructor`,
-  `mesh_bvh_frame`, `ordered_traversal`, `block_dim`, `max_registers`, `use_fast_math`) are
-  accepted for compatibility: this renderer always traverses nearest-first over its own BVHs.
+  The model is a `mujoco.MjModel` (its poses from mj_forward at qpos0, as the reference
+  computes them) or a `HostModel`. Differences from the reference: `render_rgb/depth/seg=None`
+  render all three outputs; `mesh_bvh_frame="auto"` re-frames elongated meshes as the reference
+  does, but every BVH is this renderer's median-split tree, so `mesh_bvh_constructor`,
+  `ordered_traversal`, `block_dim`, `max_registers` and `use_fast_math` are accepted for
+  compatibility only: traversal is always nearest-first.
   `traversal="optix"` is not available (no OptiX on this hardware) and falls back to
   "software" with a warning, as the reference does without OptiX.
 
@@ -205,6 +208,7 @@ def create_render_context(
   """
   if precompute and torch.device(device).type != "cuda":
     raise ValueError("create_render_context builds its tables with HIP kernels: use a GPU device (or precompute=False)")
+  mjm = as_host_model(mjm)
   light_model = _as_enum(LightModel, light_model, "light_model")
   tone_mapper = _as_enum(ToneMapper, tone_mapper, "tone_mapper")
   unset_material = _as_enum(UnsetMaterial, unset_material, "unset_material")

@@ -387,14 +387,34 @@ class Data:
     self.light_xdir = tile(mjm.light_xdir, 3)
 
 
-def put_model(mjm: HostModel, device="cuda") -> Model:
-  """Device model of a host model (MuJoCo Warp's put_model)."""
+def put_model(mjm, device="cuda") -> Model:
+  """Device model of a HostModel or of a `mujoco.MjModel` (MuJoCo Warp's put_model)."""
+  from mujoco_warp_pbr_amd.mjmodel import as_host_model
+
+  mjm = as_host_model(mjm)
   mjm.validate()
   return Model(mjm, device)
 
 
-def put_data(mjm: HostModel, nworld: int = 1, device="cuda") -> Data:
-  """Per-world device data at the host model's poses (MuJoCo Warp's put_data)."""
+def put_data(mjm, *args, device="cuda", nworld=None) -> Data:
+  """Per-world device data at the model's poses (MuJoCo Warp's put_data).
+
+  put_data(mjm, nworld=1, device) with a HostModel (its poses), or MuJoCo Warp's form
+  put_data(mjm, mjd, nworld=...) with a `mujoco.MjModel` and its MjData (the MjData's poses).
+  """
+  from mujoco_warp_pbr_amd.mjmodel import as_host_model
+
+  rest = list(args)
+  mjd = rest.pop(0) if rest and not isinstance(rest[0], (int, np.integer)) else None
+  if rest:
+    if nworld is not None:
+      raise TypeError("put_data: nworld given twice")
+    nworld = rest.pop(0)
+  if rest:
+    device = rest.pop(0)
+  if rest:
+    raise TypeError("put_data takes (mjm, [mjd], [nworld], [device])")
+  nworld = 1 if nworld is None else int(nworld)
   if nworld < 1:
     raise ValueError(f"nworld must be at least 1, got {nworld}")
-  return Data(mjm, nworld, device)
+  return Data(as_host_model(mjm, mjd), nworld, device)
