@@ -23,6 +23,13 @@ from mujoco_warp_pbr_amd.output import get_hdr
 from mujoco_warp_pbr_amd.output import get_normal
 from mujoco_warp_pbr_amd.output import get_rgb
 from mujoco_warp_pbr_amd.output import get_segmentation
+from mujoco_warp_pbr_amd.output import albedo_view
+from mujoco_warp_pbr_amd.output import bgra_view
+from mujoco_warp_pbr_amd.output import depth_view
+from mujoco_warp_pbr_amd.output import hdr_view
+from mujoco_warp_pbr_amd.output import normal_view
+from mujoco_warp_pbr_amd.output import packed_rgb_view
+from mujoco_warp_pbr_amd.output import segmentation_view
 from mujoco_warp_pbr_amd.rays import CameraRays
 from mujoco_warp_pbr_amd.rays import set_camera_rays
 from mujoco_warp_pbr_amd.render import count_path_rays
@@ -56,6 +63,13 @@ __all__ = [
   "get_normal",
   "get_rgb",
   "get_segmentation",
+  "albedo_view",
+  "bgra_view",
+  "depth_view",
+  "hdr_view",
+  "normal_view",
+  "packed_rgb_view",
+  "segmentation_view",
   "load_hdr",
   "put_data",
   "put_model",

@@ -1,7 +1,8 @@
-"""Per-camera readback into caller-allocated tensors (reference output.py:34-152).
+"""Per-camera readback (reference output.py:34-152).
 
-Each function copies camera `camera_index` of every world (the first `out.shape[0]` worlds)
-into `out`, whose shape is checked; nothing synchronises with the host.
+The get_* functions copy camera `camera_index` of every world (the first `out.shape[0]` worlds)
+into a caller-allocated `out`, whose shape and dtype are checked; nothing synchronises with the
+host. The *_view functions return views of the same pixels in the context's buffers instead.
 """
 
 from __future__ import annotations
@@ -101,3 +102,62 @@ def get_normal(rc: PbrRenderContext, camera_index: int, normal_out: torch.Tensor
   """First-hit world shading normal, (n, h, w, 3); needs render_guides."""
   _check_guides(rc)
   _get_rgb_image(rc, rc.guide_normal, camera_index, normal_out, "normal_out")
+
+
+# ----------------------------------------------------------------------------- zero-copy views
+# Views of one camera's pixels inside the context's own buffers: no copy, no launch; they alias
+# what the next render writes. Torch views with the buffers' strides, so torch.utils.dlpack
+# hands them to other frameworks without a copy either.
+
+
+def _pixels(buffer: torch.Tensor, adr: int, w: int, h: int) -> torch.Tensor:
+  return buffer[:, adr : adr + w * h].unflatten(1, (h, w))
+
+
+def bgra_view(rc: PbrRenderContext, camera_index: int) -> torch.Tensor:
+  """(nworld, h, w, 4) uint8 view of the packed display colour: each 0xAARRGGBB word is the bytes
+  B, G, R, A in memory (little endian), so channels are (B, G, R, A), A = 255."""
+  adr, w, h = _camera(rc, camera_index, "rgb_adr", "rgb")
+  return rc.base.rgb_data[:, adr : adr + w * h].view(torch.uint8).unflatten(1, (h, w, 4))
+
+
+def packed_rgb_view(rc: PbrRenderContext, camera_index: int) -> torch.Tensor:
+  """(nworld, h, w) int32 view of the packed 0xAARRGGBB words (MuJoCo Warp's rgb_data layout)."""
+  adr, w, h = _camera(rc, camera_index, "rgb_adr", "rgb")
+  return _pixels(rc.base.rgb_data, adr, w, h)
+
+
+def depth_view(rc: PbrRenderContext, camera_index: int) -> torch.Tensor:
+  """(nworld, h, w) float32 view of the planar depth (0 for background; unscaled, unclamped)."""
+  adr, w, h = _camera(rc, camera_index, "depth_adr", "depth")
+  return _pixels(rc.base.depth_data, adr, w, h)
+
+
+def segmentation_view(rc: PbrRenderContext, camera_index: int) -> torch.Tensor:
+  """(nworld, h, w, 2) int32 view of (geom id, object type); (-1, -1) for the background."""
+  adr, w, h = _camera(rc, camera_index, "seg_adr", "segmentation")
+  return rc.base.seg_data[:, adr : adr + w * h].unflatten(1, (h, w))
+
+
+def _vec3_view(rc: PbrRenderContext, buffer: torch.Tensor, camera_index: int) -> torch.Tensor:
+  adr, w, h = _camera(rc, camera_index, "rgb_adr", "rgb")
+  return buffer[:, adr : adr + w * h].unflatten(1, (h, w))
+
+
+def hdr_view(rc: PbrRenderContext, camera_index: int) -> torch.Tensor:
+  """(nworld, h, w, 3) float32 view of the linear HDR radiance; needs render_hdr."""
+  if not rc.render_hdr:
+    raise ValueError("create the render context with render_hdr=True to read HDR output")
+  return _vec3_view(rc, rc.hdr_data, camera_index)
+
+
+def albedo_view(rc: PbrRenderContext, camera_index: int) -> torch.Tensor:
+  """(nworld, h, w, 3) float32 view of the first-hit base colour; needs render_guides."""
+  _check_guides(rc)
+  return _vec3_view(rc, rc.guide_albedo, camera_index)
+
+
+def normal_view(rc: PbrRenderContext, camera_index: int) -> torch.Tensor:
+  """(nworld, h, w, 3) float32 view of the first-hit shading normal; needs render_guides."""
+  _check_guides(rc)
+  return _vec3_view(rc, rc.guide_normal, camera_index)

@@ -153,6 +153,46 @@ def test_world_count_mismatch_is_refused_before_any_launch():
     render_path_samples(m, d3, prc)
 
 
+def test_views_alias_the_context_buffers_and_match_the_copies():
+  """The *_view functions return the camera's pixels in place: same values as the get_* copies
+  (colour as BGRA bytes of the 0xAARRGGBB words), same storage as the context's buffers, and a
+  DLPack export shares it too."""
+  m, d, rc, out = _shot()
+  rc.hdr_data.copy_(torch.from_numpy(out["hdr"]))
+  rc.guide_albedo.copy_(torch.from_numpy(out["albedo"]))
+  rc.guide_normal.copy_(torch.from_numpy(out["normal"]))
+  w, h = (int(x) for x in rc.base.cam_res[0])
+  bgra = mj.bgra_view(rc, 0)
+  assert bgra.shape == (2, h, w, 4) and bgra.dtype == torch.uint8
+  assert bgra.data_ptr() == rc.base.rgb_data.data_ptr()
+  rgb = torch.zeros((2, h, w, 3))
+  mj.get_rgb(rc, 0, rgb)
+  np.testing.assert_array_equal(bgra[..., [2, 1, 0]].numpy().astype(np.float32) / 255.0, rgb.numpy())
+  assert (bgra[..., 3] == 255).all()
+  np.testing.assert_array_equal(mj.packed_rgb_view(rc, 0).numpy(), out["rgb"][:, : w * h].reshape(2, h, w).view(np.int32))
+  for name in ("hdr", "albedo", "normal"):  # (their get_* copies run a HIP kernel: compared on the GPU)
+    view = getattr(mj, f"{name}_view")(rc, 0)
+    assert view.shape == (2, h, w, 3)
+    np.testing.assert_array_equal(view.numpy(), out[name][:, : w * h].reshape(2, h, w, 3))
+  for cam in (0, 1):
+    cw_, ch_ = (int(x) for x in rc.base.cam_res[cam])
+    adr = int(rc.base.depth_adr[cam])
+    depth = mj.depth_view(rc, cam)
+    np.testing.assert_array_equal(depth.numpy(), out["depth"][:, adr : adr + cw_ * ch_].reshape(2, ch_, cw_))
+    seg = torch.zeros((2, ch_, cw_, 2), dtype=torch.int32)
+    mj.get_segmentation(rc, cam, seg)
+    np.testing.assert_array_equal(mj.segmentation_view(rc, cam).numpy(), seg.numpy())
+  # aliasing: a write into the context shows through the view and through a DLPack export
+  depth = mj.depth_view(rc, 1)
+  exported = torch.utils.dlpack.from_dlpack(torch.utils.dlpack.to_dlpack(depth))
+  rc.base.depth_data[1, int(rc.base.depth_adr[1])] = 123.0
+  assert depth[1, 0, 0] == 123.0 and exported[1, 0, 0] == 123.0
+  with pytest.raises(ValueError, match="does not render rgb"):
+    mj.bgra_view(rc, 1)
+  with pytest.raises(ValueError, match="camera_index"):
+    mj.depth_view(rc, 2)
+
+
 @pytest.mark.gpu
 def test_gpu_readback_matches_oracle_buffers(cuda):
   """Every get_* on GPU tensors after a GPU render of 2 worlds x 2 cameras (16 x 16 with RGB,
@@ -185,6 +225,16 @@ def test_gpu_readback_matches_oracle_buffers(cuda):
     mj.get_segmentation(rc, cam, seg)
     sadr = int(rc.base.seg_adr[cam])
     np.testing.assert_array_equal(seg.cpu().numpy(), ref["seg"][:, sadr : sadr + w * h].reshape(2, h, w, 2))
+    # the zero-copy views of the same render
+    np.testing.assert_array_equal(mj.segmentation_view(rc, cam).cpu().numpy(), seg.cpu().numpy())
+    np.testing.assert_allclose(mj.depth_view(rc, cam).cpu().numpy(), raw, rtol=1e-5, atol=1e-7)
+  # the view's bytes are get_rgb's values x 255 (on the GPU, torch divides by 255 as a multiply by
+  # its reciprocal: compare bytes)
+  np.testing.assert_array_equal(mj.bgra_view(rc, 0)[..., [2, 1, 0]].cpu().numpy(), np.rint(rgb.cpu().numpy() * 255.0).astype(np.uint8))
+  for name in ("hdr", "albedo", "normal"):
+    copy = torch.zeros((2, h0, w0, 3), device=cuda)
+    getattr(mj, f"get_{name}")(rc, 0, copy)
+    torch.testing.assert_close(getattr(mj, f"{name}_view")(rc, 0), copy, rtol=0, atol=0)
 
 
 @pytest.mark.parametrize("flag", ["render_hdr", "render_guides", "denoise"])
