@@ -511,6 +511,47 @@ def learned_mesh_section(device, time_worlds: int = 1024, quality_worlds: int = 
   }
 
 
+def kernels_only(nworld: int, res: int) -> int:
+  """The U-Net forward (packaged weights, random BF16 input) and a 1-spp path-traced render of the
+  full-feature scene with a-trous denoising (3 iterations), nworld x res^2, three times each;
+  prints one JSON line of mean times."""
+  import torch
+
+  import mujoco_warp_pbr_amd as mj
+  from mujoco_warp_pbr_amd import learned
+  from mujoco_warp_pbr_amd import scenes
+
+  device = torch.device("cuda:0")
+  stream = torch.cuda.current_stream()
+
+  def timed(fn, reps=3):
+    fn()
+    torch.cuda.synchronize()
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record(stream)
+    for _ in range(reps):
+      fn()
+    b.record(stream)
+    b.synchronize()
+    return a.elapsed_time(b) / reps
+
+  net, _ = learned.load(learned.DEFAULT_WEIGHTS, device)
+  x = torch.randint(-16000, 16000, (nworld, res, res, learned.IN_CHANNELS), dtype=torch.int16, device=device)
+  unet_ms = timed(lambda: net.forward(x))
+  del x
+  net.release()
+  mjm = scenes.full_scene(seed=0, res=res)
+  m, d = mj.put_model(mjm, device), mj.put_data(mjm, nworld, device)
+  rc = mj.create_render_context(mjm, nworld=nworld, cam_res=(res, res), envmap=scenes.procedural_sky(0, 256, 512), tier=mj.Tier.PATH,
+                                path_samples=1, denoise=True, device=device)
+  scenes.randomize_worlds(m, rc, nworld, seed=1)
+  mj.refit_bvh(m, d, rc)
+  pt_ms = timed(lambda: mj.render(m, d, rc))
+  print(json.dumps({"kernels_only": f"{nworld} worlds x {res}^2", "unet_forward_ms": round(unet_ms, 3),
+                    "pt_1spp_atrous_ms": round(pt_ms, 3)}), flush=True)
+  return 0
+
+
 def main(argv=None) -> int:
   p = argparse.ArgumentParser()
   p.add_argument("--gpus", type=int, default=1)
@@ -522,7 +563,12 @@ def main(argv=None) -> int:
   p.add_argument("--no-pt", action="store_true", help="skip the path-tracing sections (wavefront batch, learned denoiser)")
   p.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                  help="only copy BYTES between two device buffers 3 times (the reference kernel that scales HBM counters)")
+  p.add_argument("--kernels-only", action="store_true",
+                 help="only the U-Net forward and a 1-spp path-traced render with a-trous, --nworld x --res^2, 3 times each "
+                      "(the workloads the profiles of those kernels are taken on)")
   args = p.parse_args(argv)
+  if args.kernels_only:
+    return kernels_only(args.nworld, args.res)
   if args.copy_calibration:
     import torch
 
