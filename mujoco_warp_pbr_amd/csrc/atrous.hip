@@ -85,9 +85,13 @@ __global__ void demodulate_kernel(MjpbrAtrousParams p, MjpbrOutputs out, float4*
 }
 
 // denoise.py:105-178; `finish` fuses denoise.py:181-201 into the last pass.
+// Workgroups are dispatched to the 8 XCDs round-robin; workgroup b covers the pixels of block
+// (b % 8) * xcd_chunk + b / 8, so each XCD filters a contiguous run of rows and the dilated 5x5
+// footprints of neighbouring rows are served by that XCD's L2 instead of being fetched by all 8.
 __global__ void atrous_kernel(MjpbrAtrousParams p, MjpbrOutputs out, int step, const float4* src, const uint4* guides,
-                              float4* dst, int finish) {
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+                              float4* dst, int finish, int xcd_chunk) {
+  const int64_t blk = (int64_t)(blockIdx.x & 7) * xcd_chunk + (blockIdx.x >> 3);
+  int64_t i = blk * blockDim.x + threadIdx.x;
   if (i >= (int64_t)p.nworld * p.stride) return;
   int w = (int)(i / p.stride), index = (int)(i - (int64_t)w * p.stride);
   if (p.world_select && !p.world_select[w]) return;
@@ -166,6 +170,7 @@ extern "C" int mjpbr_atrous(const MjpbrAtrousParams* p, const MjpbrOutputs* out,
   int64_t n = (int64_t)p->nworld * p->stride;
   int threads = 256;
   unsigned blocks = (unsigned)((n + threads - 1) / threads);
+  const unsigned xcd_chunk = (blocks + 7) / 8;  // a-trous blocks each XCD filters (see atrous_kernel)
   hipStream_t st = (hipStream_t)stream;
   float4* a = reinterpret_cast<float4*>(work0);
   float4* b = reinterpret_cast<float4*>(work1);
@@ -176,7 +181,7 @@ extern "C" int mjpbr_atrous(const MjpbrAtrousParams* p, const MjpbrOutputs* out,
   if (rc) return rc;
   for (int it = 0; it < p->iterations; ++it) {
     int last = it == p->iterations - 1;
-    hipLaunchKernelGGL(atrous_kernel, dim3(blocks), dim3(threads), 0, st, *p, *out, 1 << it, a, g, b, last);
+    hipLaunchKernelGGL(atrous_kernel, dim3(xcd_chunk * 8), dim3(threads), 0, st, *p, *out, 1 << it, a, g, b, last, (int)xcd_chunk);
     rc = check_launch("atrous_kernel");
     if (rc) return rc;
     float4* t = a;
