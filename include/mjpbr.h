@@ -454,6 +454,12 @@ int mjpbr_unet_conv3x3(const uint16_t* x, const uint16_t* mask, const uint16_t* 
  * last convolution of a level, its pooling fused into the epilogue. */
 int mjpbr_unet_conv3x3_pool(const uint16_t* x, const uint16_t* w, const float* bias, uint16_t* y, uint16_t* pool, int n, int h, int wd,
                             int cin, int cout, int relu, void* stream);
+/* Two consecutive 3x3 convolutions whose intermediate stays on chip (the U-Net's last two layers
+ * at inference): m = relu(conv(x, w1) + b1) with x [n][h][w][32], w1 [32][9][32]; then
+ * y32 [pixel][out_f32] = the first out_f32 (1..4) channels of conv(m, w2) + b2, w2 [>= 16][9][32]
+ * (its first 16 output channels computed). Equal bit for bit to mjpbr_unet_conv3x3 run twice. */
+int mjpbr_unet_conv3x3_pair(const uint16_t* x, const uint16_t* w1, const float* b1, const uint16_t* w2, const float* b2, float* y32,
+                            int out_f32, int n, int h, int wd, void* stream);
 /* The same convolution over concat(upsample2x(low), skip) without materialising it: low is
  * [n][h/2][w/2][c_split] (nearest 2x upsampling), skip [n][h][w][cin - c_split]; c_split a
  * multiple of 32. The U-Net decoder's input. */

@@ -61,6 +61,7 @@ FUNCTIONS.update(
   {
     "mjpbr_unet_conv3x3": (_I, [_VP, _VP, _VP, _VP, _VP, _VP, _I, _I, _I, _I, _I, _I, _I, _VP]),
     "mjpbr_unet_conv3x3_pool": (_I, [_VP, _VP, _VP, _VP, _VP, _I, _I, _I, _I, _I, _I, _VP]),
+    "mjpbr_unet_conv3x3_pair": (_I, [_VP, _VP, _VP, _VP, _VP, _VP, _I, _I, _I, _I, _VP]),
     "mjpbr_unet_conv3x3_upcat": (_I, [_VP, _VP, _I, _VP, _VP, _VP, _I, _I, _I, _I, _I, _I, _VP]),
     "mjpbr_unet_wgrad_workspace": (_L, [_I, _I, _I, _I, _I, _I]),
     "mjpbr_unet_bias_workspace": (_L, [_I, _I, _I, _I]),

@@ -387,6 +387,221 @@ conv3x3_kernel(ConvArgs a, int bpg) {
   }
 }
 
+// ------------------------------------------------------------------ fused layer pair (inference)
+// y32 = conv3x3(relu(conv3x3(x, w1) + b1), w2) + b2 for 32 -> 32 -> 16 channels, the U-Net's
+// last two layers: the 32-channel intermediate never leaves the chip. A block of 16x16 outputs
+// needs the intermediate on 18x18 pixels (one-pixel halo) and so a 20x20 input patch. The
+// intermediate is computed as 21 MFMA tiles of 16 pixels: 18 rows of the block's 16 columns (rows
+// 0..15 through the row-reuse path of conv3x3_kernel, rows 16 and 17 gathered) and 3 tiles of
+// the 36 halo-column pixels (gathered), stored (bias, ReLU, BF16; zero outside the image) as an
+// 18x18 patch in LDS, which the second convolution reads exactly as conv3x3_kernel reads its
+// input patch. Every output accumulates its taps in conv3x3_kernel's order (column shift, then row
+// shift), so the result is bit for bit the two layers run separately. Layer 1's B operands are
+// resident in registers, layer 2's in LDS; workgroups stream contiguous block runs.
+constexpr int kPairIn = kConvBlock + 4;  // 20: input patch side
+constexpr int kPairInElems = kPairIn * kPairIn * kPStride;
+constexpr int kPairInVecs = kPairIn * kPairIn * (kCinChunk / 8);
+constexpr int kPairPrefetch = (kPairInVecs + 255) / 256;
+
+struct PairArgs {
+  const bf16_t* x;
+  const bf16_t* w1;  // [32][9][32]
+  const float* b1;
+  const bf16_t* w2;  // [>= 16][9][32], the first 16 output channels used
+  const float* b2;
+  float* y32;        // [pixel][out_f32]
+  int n, h, wd, out_f32;
+};
+
+// Byte offset of 16-byte vector k of the 20x20 input patch of block (y0, x0) in image `img`'s
+// buffer, or kOutOfRange (zero padding, k past the patch).
+MJ_DEV uint32_t pair_patch_offset(const PairArgs& a, int y0, int x0, int k) {
+  const int pos = k >> 2, part = k & 3;
+  const int yy = y0 - 2 + pos / kPairIn, xx = x0 - 2 + pos % kPairIn;
+  // unsigned compares and bitwise ands keep the address computation free of branches
+  const bool inside = (k < kPairInVecs) & ((unsigned)yy < (unsigned)a.h) & ((unsigned)xx < (unsigned)a.wd);
+  const uint32_t off = (uint32_t)((yy * a.wd + xx) * kCinChunk + part * 8) * 2u;
+  return inside ? off : kOutOfRange;
+}
+
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
+conv3x3_pair_kernel(PairArgs a, int bpg) {
+  extern __shared__ __attribute__((aligned(16))) bf16_t pair_lds[];
+  bf16_t* in_lds = pair_lds;                   // 20 x 20 positions x kPStride
+  bf16_t* mid = pair_lds + kPairInElems;       // 18 x 18 positions x kPStride
+  bf16_t* w2l = mid + kPatchElems;             // layer 2's weights: 16 rows of kWRow
+  float* bias_lds = reinterpret_cast<float*>(w2l + 16 * kWRow);  // b1 [32], b2 [16]
+  const int bx = a.wd / kConvBlock, by = a.h / kConvBlock;
+  const int nblocks = a.n * bx * by;
+  const int first = blockIdx.x * bpg;
+  const int nsteps = min(nblocks, first + bpg) - first;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int lm = lane & 15, lk = lane >> 4;
+  auto block_origin = [&](int blk, int* img, int* y0, int* x0) {
+    *img = blk / (bx * by);
+    const int r = blk - *img * (bx * by);
+    *y0 = (r / bx) * kConvBlock;
+    *x0 = (r % bx) * kConvBlock;
+  };
+  const int64_t image_bytes = (int64_t)a.h * a.wd * kCinChunk * 2;
+  uint4 pre[kPairPrefetch];
+  auto prefetch = [&](int step) {
+    int img, y0, x0;
+    block_origin(first + step, &img, &y0, &x0);
+    const __amdgpu_buffer_rsrc_t src = buffer_of(a.x + (int64_t)img * (image_bytes / 2), image_bytes);
+    // an opaque thread index: the patch coordinates are recomputed each step rather than kept
+    // live (and spilled) across the loop
+    int tid = threadIdx.x;
+    asm volatile("" : "+v"(tid));
+#pragma unroll
+    for (int i = 0; i < kPairPrefetch; ++i) pre[i] = buffer_vec(src, pair_patch_offset(a, y0, x0, tid + 256 * i));
+  };
+  if (threadIdx.x < 48) bias_lds[threadIdx.x] = threadIdx.x < 32 ? a.b1[threadIdx.x] : a.b2[threadIdx.x - 32];
+  // layer 1's B operands resident in registers (2 tiles of 16 output channels); layer 2's in LDS
+  uint4 w1r[9][2];
+  {
+    const __amdgpu_buffer_rsrc_t r1 = buffer_of(a.w1, 32 * 9 * 32 * 2);
+#pragma unroll
+    for (int t = 0; t < 9; ++t)
+#pragma unroll
+      for (int j = 0; j < 2; ++j) w1r[t][j] = buffer_vec(r1, (uint32_t)(((j * 16 + lm) * 9 + t) * 32 + lk * 8) * 2u);
+    for (int e = threadIdx.x; e < 16 * 36; e += 256) {  // 16 rows x 9 taps x 4 vectors
+      const int row = e / 36, rem = e % 36;
+      *reinterpret_cast<uint4*>(w2l + row * kWRow + (rem >> 2) * kCinChunk + (rem & 3) * 8) =
+          *reinterpret_cast<const uint4*>(a.w2 + (row * 9 + (rem >> 2)) * 32 + (rem & 3) * 8);
+    }
+  }
+  prefetch(0);
+  // the gathered tiles of this wave: rows 16 and 17 (waves 0 and 1), halo-column tiles 0..2
+  // (waves 2, 3, 3)
+  const int nextra = wave == 3 ? 2 : 1;
+#pragma unroll 1
+  for (int step = 0; step < nsteps; ++step) {
+    int img, y0, x0;
+    block_origin(first + step, &img, &y0, &x0);
+    __syncthreads();  // the previous step's readers of both patches are done
+#pragma unroll
+    for (int i = 0; i < kPairPrefetch; ++i) {
+      const int k = threadIdx.x + 256 * i;
+      if (k < kPairInVecs) *reinterpret_cast<uint4*>(in_lds + (k >> 2) * kPStride + (k & 3) * 8) = pre[i];
+    }
+    __syncthreads();
+    if (step + 1 < nsteps) prefetch(step + 1);
+    // ---- layer 1: rows 4 wave .. + 3 of the intermediate through the row-reuse path
+    floatx4 acc[4][2], ext[2][2];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 2; ++j) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int e = 0; e < 2; ++e)
+#pragma unroll
+      for (int j = 0; j < 2; ++j) ext[e][j] = floatx4{0.f, 0.f, 0.f, 0.f};
+    // the gathered pixels of this lane in the wave's extra tiles: intermediate (iy, ix), -1 pad
+    int eiy[2], eix[2];
+#pragma unroll
+    for (int e = 0; e < 2; ++e) {
+      const int tile = wave < 2 ? wave : (wave == 2 ? 2 : 3 + e);  // 0, 1: rows 16, 17; 2..4: halo tiles
+      if (tile < 2) {
+        eiy[e] = 16 + tile;
+        eix[e] = 1 + lm;
+      } else {
+        const int q = (tile - 2) * 16 + lm;  // halo pixel 0..47: column 0 (q < 18), 17 (q < 36), pad
+        eiy[e] = q < 36 ? q % 18 : -1;
+        eix[e] = q < 18 ? 0 : 17;
+      }
+      if (e >= nextra) eiy[e] = -1;
+    }
+#pragma unroll
+    for (int dx = 0; dx < 3; ++dx) {
+      uint4 aq[6];
+#pragma unroll
+      for (int r = 0; r < 6; ++r)
+        aq[r] = *reinterpret_cast<const uint4*>(in_lds + ((wave * 4 + r) * kPairIn + 1 + lm + dx) * kPStride + lk * 8);
+#pragma unroll
+      for (int r = 0; r < 6; ++r)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const int dy = r - i;
+          if (dy < 0 || dy > 2) continue;
+#pragma unroll
+          for (int j = 0; j < 2; ++j) acc[i][j] = mfma16(w1r[dy * 3 + dx][j], aq[r], acc[i][j]);
+        }
+#pragma unroll
+      for (int e = 0; e < 2; ++e) {
+        if (e >= nextra) continue;
+        const int iy = eiy[e] < 0 ? 0 : eiy[e];  // pad lanes read a valid position, stored nowhere
+#pragma unroll
+        for (int dy = 0; dy < 3; ++dy) {
+          const uint4 bq = *reinterpret_cast<const uint4*>(in_lds + ((iy + dy) * kPairIn + eix[e] + dx) * kPStride + lk * 8);
+#pragma unroll
+          for (int j = 0; j < 2; ++j) ext[e][j] = mfma16(w1r[dy * 3 + dx][j], bq, ext[e][j]);
+        }
+      }
+      __builtin_amdgcn_sched_barrier(0);
+    }
+    // layer 1 epilogue: bias, ReLU, BF16 into the intermediate patch; zero outside the image
+    auto store_mid = [&](int iy, int ix, int j, const floatx4& v) {
+      const int yy = y0 - 1 + iy, xx = x0 - 1 + ix;
+      const bool inside = yy >= 0 && yy < a.h && xx >= 0 && xx < a.wd;
+      const int c = j * 16 + 4 * lk;
+      const float4 bb = *reinterpret_cast<const float4*>(bias_lds + c);
+      const float b4[4] = {bb.x, bb.y, bb.z, bb.w};
+      float o[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) o[q] = inside ? fmaxf(v[q] + b4[q], 0.0f) : 0.0f;
+      const bf16x2 lo = {(__bf16)o[0], (__bf16)o[1]}, hi = {(__bf16)o[2], (__bf16)o[3]};
+      *reinterpret_cast<uint2*>(mid + (iy * kPatch + ix) * kPStride + c) =
+          make_uint2(__builtin_bit_cast(uint32_t, lo), __builtin_bit_cast(uint32_t, hi));
+    };
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 2; ++j) store_mid(wave * 4 + i, 1 + lm, j, acc[i][j]);
+#pragma unroll
+    for (int e = 0; e < 2; ++e)
+      if (e < nextra && eiy[e] >= 0)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) store_mid(eiy[e], eix[e], j, ext[e][j]);
+    __syncthreads();
+    // ---- layer 2 (16 output channels) from the intermediate, as conv3x3_kernel
+    floatx4 out[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) out[i] = floatx4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int dx = 0; dx < 3; ++dx) {
+      uint4 b[3], aq[6];
+#pragma unroll
+      for (int dy = 0; dy < 3; ++dy) b[dy] = *reinterpret_cast<const uint4*>(w2l + lm * kWRow + (dy * 3 + dx) * kCinChunk + lk * 8);
+#pragma unroll
+      for (int r = 0; r < 6; ++r) aq[r] = *reinterpret_cast<const uint4*>(mid + ((wave * 4 + r) * kPatch + lm + dx) * kPStride + lk * 8);
+#pragma unroll
+      for (int r = 0; r < 6; ++r)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const int dy = r - i;
+          if (dy < 0 || dy > 2) continue;
+          out[i] = mfma16(b[dy], aq[r], out[i]);
+        }
+      __builtin_amdgcn_sched_group_barrier(0x100, 9, 0);
+      __builtin_amdgcn_sched_group_barrier(0x008, 12, 0);
+      __builtin_amdgcn_sched_barrier(0);
+    }
+    // layer 2 epilogue: the first out_f32 channels in FP32 (lane lk == 0 holds channels 0..3)
+    if (lk == 0) {
+      const float4 bb = *reinterpret_cast<const float4*>(bias_lds + 32);
+      const float b4[4] = {bb.x, bb.y, bb.z, bb.w};
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int64_t pix = ((int64_t)img * a.h + y0 + wave * 4 + i) * a.wd + x0 + lm;
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+          if (q < a.out_f32) a.y32[pix * a.out_f32 + q] = out[i][q] + b4[q];
+      }
+    }
+  }
+}
+
 // ------------------------------------------------------------------ weight gradient
 // part[chunk][n][tap][c] = sum over the chunk's pixels p of x[p + off(tap)][c] * g[p][n] * (mask[p][n] > 0)
 // Workgroup: 16 input channels x 9 taps (M = 144) x up to 64 output channels, looping over 8x8
@@ -816,6 +1031,21 @@ extern "C" int mjpbr_unet_conv3x3_pool(const uint16_t* x, const uint16_t* w, con
                      h, wd, cin, cout);
   ConvArgs a{x, nullptr, nullptr, w, bias, y, nullptr, n, h, wd, cin, cout, relu, 0, 0, pool};
   return conv_launch(a, stream);
+}
+
+extern "C" int mjpbr_unet_conv3x3_pair(const uint16_t* x, const uint16_t* w1, const float* b1, const uint16_t* w2, const float* b2, float* y32,
+                                       int out_f32, int n, int h, int wd, void* stream) {
+  if (h % kConvBlock || wd % kConvBlock || n < 1 || !x || !w1 || !b1 || !w2 || !b2 || !y32 || out_f32 < 1 || out_f32 > 4)
+    return set_error(MJPBR_EINVAL, "unet_conv3x3_pair: need h, w multiples of 16, every buffer, and 1..4 FP32 channels (got %dx%d, %d)", h, wd,
+                     out_f32);
+  PairArgs a{reinterpret_cast<const bf16_t*>(x), reinterpret_cast<const bf16_t*>(w1), b1, reinterpret_cast<const bf16_t*>(w2), b2, y32,
+             n, h, wd, out_f32};
+  const int nblocks = n * (h / kConvBlock) * (wd / kConvBlock);
+  const int target = kConvCus * 2;
+  const int bpg = (nblocks + target - 1) / target;
+  const size_t lds = (size_t)(kPairInElems + kPatchElems + 16 * kWRow) * sizeof(bf16_t) + 48 * sizeof(float);
+  hipLaunchKernelGGL(conv3x3_pair_kernel, dim3((unsigned)((nblocks + bpg - 1) / bpg)), dim3(256), lds, (hipStream_t)stream, a, bpg);
+  return check_launch("conv3x3_pair_kernel");
 }
 
 extern "C" int mjpbr_unet_conv3x3_upcat(const uint16_t* low, const uint16_t* skip, int c_split, const uint16_t* w, const float* bias,

@@ -201,6 +201,16 @@ class UNet:
       int(p.relu), _stream(x),
     )
 
+  def _conv_pair(self, first, second, x, y32, out_f32):
+    """Layers `first` (32 -> 32, ReLU) and `second` (its first 16 output channels, FP32 into
+    y32) in one launch: the intermediate activation stays on chip."""
+    p1, p2 = self.params[first], self.params[second]
+    n, h, w = x.shape[0], x.shape[1], x.shape[2]
+    _abi.call(
+      "mjpbr_unet_conv3x3_pair", _abi.ptr(x), _abi.ptr(p1.wf), _abi.ptr(p1.b), _abi.ptr(p2.wf), _abi.ptr(p2.b), _abi.ptr(y32), out_f32, n,
+      h, w, _stream(x),
+    )
+
   def _conv_upcat(self, name, low, skip, out, c_split):
     p = self.params[name]
     n, h, w = skip.shape[0], skip.shape[1], skip.shape[2]
@@ -239,11 +249,11 @@ class UNet:
       self._conv("d0a", A["u0"], A["d0a"])
     else:
       self._conv_upcat("d0a", A["d1b"], A["e0b"], A["d0a"], 64)
-    self._conv("d0b", A["d0a"], A["d0b"])
     if training:
+      self._conv("d0b", A["d0a"], A["d0b"])
       self._conv("out", A["d0b"], A["out"], y32=bufs["pred"], out_f32=3)
-    else:  # the 3 predicted channels need one 16-channel tile and no BF16 output
-      self._conv("out", A["d0b"], None, y32=bufs["pred"], out_f32=3, cout=16)
+    else:  # d0b and the 3 predicted channels (one 16-channel tile) fused, d0b kept on chip
+      self._conv_pair("d0b", "out", A["d0a"], bufs["pred"], 3)
     return bufs["pred"]
 
   # ----------------------------------------------------------------- backward

@@ -352,6 +352,55 @@ def test_gpu_inference_forward_equals_training_forward(cuda):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize(
+  "n,h,w,out_f32,w2_rows",
+  [(1, 16, 16, 3, 16), (2, 32, 48, 1, 32), (3, 48, 32, 4, 16), (9, 128, 128, 3, 32)],  # the last: 576 blocks, 2 a workgroup
+)
+def test_gpu_fused_layer_pair_equals_two_convolutions(cuda, n, h, w, out_f32, w2_rows):
+  """The inference pair (32 -> 32 with ReLU, then the first 16 output channels of the next layer
+  in FP32) equals, bit for bit, the two convolutions run one after the other; its second layer
+  agrees with the float64 oracle convolution of the intermediate."""
+  rng = np.random.default_rng(80 + n + out_f32)
+  x = _rand_bf16(rng, (n, h, w, 32))
+  w1 = _rand_bf16(rng, (32, 9, 32), 0.1)
+  w2 = _rand_bf16(rng, (w2_rows, 9, 32), 0.1)
+  b1 = rng.normal(0, 0.1, 32).astype(np.float32)
+  b2 = rng.normal(0, 0.1, w2_rows).astype(np.float32)
+  tx, tw1, tw2 = _bits(x, cuda), _bits(w1, cuda), _bits(w2, cuda)
+  tb1, tb2 = torch.from_numpy(b1).to(cuda), torch.from_numpy(b2).to(cuda)
+  fused = torch.full((n * h * w, out_f32), float("nan"), dtype=torch.float32, device=cuda)
+  _abi.call(
+    "mjpbr_unet_conv3x3_pair", _abi.ptr(tx), _abi.ptr(tw1), _abi.ptr(tb1), _abi.ptr(tw2), _abi.ptr(tb2), _abi.ptr(fused), out_f32, n, h,
+    w, _stream(),
+  )
+  mid = torch.zeros((n, h, w, 32), dtype=torch.int16, device=cuda)
+  _abi.call("mjpbr_unet_conv3x3", _abi.ptr(tx), 0, _abi.ptr(tw1), _abi.ptr(tb1), _abi.ptr(mid), 0, 0, n, h, w, 32, 32, 1, _stream())
+  two = torch.full_like(fused, float("nan"))
+  _abi.call(
+    "mjpbr_unet_conv3x3", _abi.ptr(mid), 0, _abi.ptr(tw2), _abi.ptr(tb2), 0, _abi.ptr(two), out_f32, n, h, w, 32, 16, 0, _stream()
+  )
+  torch.cuda.synchronize()
+  assert torch.equal(fused, two)
+  m = _vals(mid)
+  ref = ou.conv3x3(m, w2[:16], b2[:16].astype(np.float64))[..., :out_f32].reshape(-1, out_f32)
+  mag = ou.conv3x3(np.abs(m), np.abs(w2[:16]), np.abs(b2[:16]).astype(np.float64))[..., :out_f32].reshape(-1, out_f32)
+  got = fused.cpu().numpy().astype(np.float64)
+  assert np.all(np.abs(got - ref) <= 1e-5 * mag + 1e-7)
+  assert np.abs(m).sum() > 0 and (m == 0).any()  # the ReLU clipped some channels, kept others
+
+
+@pytest.mark.gpu
+def test_gpu_fused_layer_pair_rejects_bad_shapes(cuda):
+  z = torch.zeros(16 * 16 * 32, dtype=torch.int16, device=cuda)
+  f = torch.zeros(16 * 16 * 4, dtype=torch.float32, device=cuda)
+  wt = torch.zeros(16 * 9 * 32, dtype=torch.int16, device=cuda)
+  with pytest.raises(_abi.MjpbrError):
+    _abi.call("mjpbr_unet_conv3x3_pair", _abi.ptr(z), _abi.ptr(wt), _abi.ptr(f), _abi.ptr(wt), _abi.ptr(f), _abi.ptr(f), 3, 1, 16, 24, _stream())
+  with pytest.raises(_abi.MjpbrError):
+    _abi.call("mjpbr_unet_conv3x3_pair", _abi.ptr(z), _abi.ptr(wt), _abi.ptr(f), _abi.ptr(wt), _abi.ptr(f), _abi.ptr(f), 5, 1, 16, 16, _stream())
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("n,h,w,cin,cout", [(2, 32, 48, 32, 32), (1, 32, 32, 64, 64), (1, 16, 16, 32, 64)])
 def test_gpu_pooled_convolution_equals_convolution_then_pool(cuda, n, h, w, cin, cout):
   """The encoder convolution with its 2x2 pooling fused into the epilogue: both outputs equal,
