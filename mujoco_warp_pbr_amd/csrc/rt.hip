@@ -229,8 +229,12 @@ extern "C" int mjpbr_rt_render(const MjpbrScene* scene, const MjpbrSettings* s, 
   if ((rc = global_records(scene, &global))) return rc;
   size_t lds = global ? 0 : host_stage_bytes(scene);
   const RtInstance& k = pick_instance(required_features(*s) | (global ? kFGlobal : 0u));
-  // mesh-BVH top levels in the LDS left to a workgroup at 8 per compute unit
-  lds += plan_mesh_cache(scene, (k.mask & kFMesh) && s->has_meshes, lds, &tm);
+  // mesh-BVH top levels in the LDS left to a workgroup at 8 per compute unit (the instances with
+  // meshes also hold the waves' mesh job tables, static LDS)
+  const size_t job_lds = (k.mask & kFMesh) ? (size_t)kJobTableBytes : 0;
+  if ((k.mask & kFMesh) && scene->geo.ngeom >= (1 << kPendShift))
+    return set_error(MJPBR_EINVAL, "ngeom must be below %d, got %d", 1 << kPendShift, scene->geo.ngeom);
+  lds += plan_mesh_cache(scene, (k.mask & kFMesh) && s->has_meshes, lds + job_lds, &tm);
   if ((rc = launch_stage_prep(scene, (hipStream_t)stream))) return rc;
   hipLaunchKernelGGL(k.fn, dim3(nlaunch), dim3(256), lds, (hipStream_t)stream, *scene, *s, *out, seed, tm);
   return check_launch("rt_kernel");

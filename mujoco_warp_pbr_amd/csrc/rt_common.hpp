@@ -49,27 +49,37 @@ MJ_DEV float light_visibility(const RtCtx& c, const Light& L, V3 point, V3 norma
   bool cull = s.cull_backfaces != 0;
   const bool directional = L.type == MJPBR_LIGHT_DIRECTIONAL;
   const bool soft = (F & kFSoft) && !directional && s.shadow_samples > 1;
-  if (!soft) {
+  if (!soft && !kMesh) {
     const bool hit = trace_any<kMesh, F == 0u>(c.sv, origin, l, directional ? kUnboundedShadow : dist - kShadowGap, cull);
     return hit ? s.shadow_light_fraction : 1.0f;
   }
-  const int n = s.shadow_samples;
-  V3 t1, t2;
-  orthonormal_basis(l, &t1, &t2);
+  // One packet call site for both cases (with meshes, one ray goes through it too, so its mesh
+  // tests are shared out over the wave's lanes as well).
+  const int n = soft ? s.shadow_samples : 1;
+  V3 t1 = v3s(0.0f), t2 = v3s(0.0f);
+  if (soft) orthonormal_basis(l, &t1, &t2);
   float visible = 0.0f;
   for (int k0 = 0; k0 < n; k0 += kShadowPacket) {
     V3 dirs[kShadowPacket];
     float tmax[kShadowPacket];
+    if (soft) {
 #pragma unroll
-    for (int j = 0; j < kShadowPacket; ++j) {
-      V2 u = shadow_rand2(c.seed, stream, nlight, li, n, min(k0 + j, n - 1));
-      float r = L.bulbradius * sqrtf(u.x);
-      float sp, cp;
-      sincos_turns(u.y, &sp, &cp);
-      V3 to_target = L.pos + (t1 * cp + t2 * sp) * r - origin;
-      float d = len(to_target);
-      dirs[j] = to_target / d;
-      tmax[j] = d - kShadowGap;
+      for (int j = 0; j < kShadowPacket; ++j) {
+        V2 u = shadow_rand2(c.seed, stream, nlight, li, n, min(k0 + j, n - 1));
+        float r = L.bulbradius * sqrtf(u.x);
+        float sp, cp;
+        sincos_turns(u.y, &sp, &cp);
+        V3 to_target = L.pos + (t1 * cp + t2 * sp) * r - origin;
+        float d = len(to_target);
+        dirs[j] = to_target / d;
+        tmax[j] = d - kShadowGap;
+      }
+    } else {
+#pragma unroll
+      for (int j = 0; j < kShadowPacket; ++j) {
+        dirs[j] = l;
+        tmax[j] = directional ? kUnboundedShadow : dist - kShadowGap;
+      }
     }
     const int m = min(kShadowPacket, n - k0);
     const uint32_t hit = trace_any_packet<kMesh, kShadowPacket>(c.sv, origin, dirs, tmax, m, cull);

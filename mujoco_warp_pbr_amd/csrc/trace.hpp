@@ -231,6 +231,123 @@ MJ_DEV bool mesh_any(const SceneView& sv, int mesh, V3 lo, V3 ld, float max_t) {
   return hit;
 }
 
+// ---------------------------------------------------------------- mesh any-hit jobs of a wave
+// Shadow rays reach mesh geoms in few lanes of a wave at a time, and each such lane walked the
+// mesh BVH once per ray while the others waited (a tenth of the lanes active in those walks).
+// trace_any_packet therefore defers the mesh geoms a packet reaches until its scene-BVH walk
+// ends (primitives tested first may already block the rays), then the wave deals every pending
+// (ray, mesh) pair of its lanes out to its lanes, one pair a lane a round. Any-hit answers do not
+// depend on the order of the tests, so each ray's answer is unchanged.
+
+// Pending mesh geoms of one lane's packet: geom id in the low kPendShift bits, the rays still to
+// test against it above them (0 = free slot).
+constexpr int kMeshPending = 4;
+constexpr int kPendShift = 20;
+
+MJ_DEV int lane_rank(uint64_t mask) {
+  return (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+}
+MJ_DEV int pull_int(int lane, int v) { return __builtin_amdgcn_ds_bpermute(lane << 2, v); }
+MJ_DEV float pull_float(int lane, float v) { return __int_as_float(__builtin_amdgcn_ds_bpermute(lane << 2, __float_as_int(v))); }
+
+// Job table of each wave of a 256-thread workgroup (LDS): slot j holds the descriptor of the
+// round's job j (source lane, ray, pending slot), and its answer in bit 31 after the round.
+constexpr int kJobTableBytes = 4 * 64 * 4;
+
+// Tests the pending (ray, mesh) pairs of every lane executing this call, spread over those lanes;
+// returns this lane's rays that hit. The lanes executing the call must be the ones that reached
+// it together (the exec mask), and every one of them runs the same number of rounds.
+template <int N>
+MJ_DEV uint32_t run_mesh_jobs(const SceneView& sv, V3 o, const V3 (&d)[N], const float (&tmax)[N],
+                              uint32_t (&pend)[kMeshPending], uint32_t hit) {
+  __shared__ uint32_t job_table[4][64];
+  uint32_t* tab = job_table[(threadIdx.x >> 6) & 3];
+  const uint64_t part = __ballot(1);
+  const int me = lane_rank(~0ull);
+  const int rank = lane_rank(part);
+  const int nslot = __builtin_popcountll(part);
+#pragma unroll
+  for (int p = 0; p < kMeshPending; ++p) pend[p] &= ~(hit << kPendShift);
+  for (;;) {
+    int cnt = 0;
+#pragma unroll
+    for (int p = 0; p < kMeshPending; ++p) cnt += __builtin_popcount(pend[p] >> kPendShift);
+    // wave-wide count and exclusive prefix of the lanes' job counts (cnt < 32: five bit planes)
+    int total = 0, prefix = 0;
+#pragma unroll
+    for (int b = 0; b < 5; ++b) {
+      const uint64_t plane = __ballot((cnt >> b) & 1);
+      total += __builtin_popcountll(plane) << b;
+      prefix += lane_rank(plane) << b;
+    }
+    if (total == 0) break;
+    // sources: descriptors of the jobs that fall into this round's nslot slots
+    uint32_t sent[kMeshPending];
+    int j = prefix;
+#pragma unroll
+    for (int p = 0; p < kMeshPending; ++p) {
+      sent[p] = 0u;
+#pragma unroll
+      for (int k = 0; k < N; ++k) {
+        if (!((pend[p] >> (kPendShift + k)) & 1u)) continue;
+        if (j < nslot) {
+          tab[j] = (uint32_t)me | ((uint32_t)k << 6) | ((uint32_t)p << 9);
+          sent[p] |= 1u << k;
+        }
+        ++j;
+      }
+      pend[p] &= ~(sent[p] << kPendShift);
+    }
+    __builtin_amdgcn_wave_barrier();
+    // destinations: the lane of rank r runs job r
+    const bool has = rank < min(total, nslot);
+    const uint32_t desc = has ? tab[rank] : 0u;
+    const int src = desc & 63, k = (desc >> 6) & 7, p = (desc >> 9) & 3;
+    int gslot = pull_int(src, (int)pend[0]);
+#pragma unroll
+    for (int q = 1; q < kMeshPending; ++q) {
+      const int v = pull_int(src, (int)pend[q]);
+      if (p == q) gslot = v;
+    }
+    // the slot's geom id survives in the low bits even once all its rays were sent
+    const V3 so = V3{pull_float(src, o.x), pull_float(src, o.y), pull_float(src, o.z)};
+    V3 sd = v3s(0.0f);
+    float st = 0.0f;
+#pragma unroll
+    for (int q = 0; q < N; ++q) {
+      const V3 v = V3{pull_float(src, d[q].x), pull_float(src, d[q].y), pull_float(src, d[q].z)};
+      const float t = pull_float(src, tmax[q]);
+      if (k == q) {
+        sd = v;
+        st = t;
+      }
+    }
+    if (has) {
+      const float* g = sv.geoms + kGeomRec * (gslot & ((1 << kPendShift) - 1));
+      const int mid = (__float_as_int(g[15]) >> 9) - 1;
+      const float* mat = g + 4;
+      const bool h = mesh_any(sv, mid, mat_tmul(mat, so - V3{g[0], g[1], g[2]}), mat_tmul(mat, sd), st);
+      tab[rank] = desc | (h ? 0x80000000u : 0u);
+    }
+    __builtin_amdgcn_wave_barrier();
+    // sources: answers of the jobs they sent, in the order they were written
+    j = prefix;
+#pragma unroll
+    for (int q = 0; q < kMeshPending; ++q) {
+#pragma unroll
+      for (int kk = 0; kk < N; ++kk) {
+        if (!((sent[q] >> kk) & 1u)) continue;
+        if (tab[j] >> 31) hit |= 1u << kk;
+        ++j;
+      }
+    }
+    __builtin_amdgcn_wave_barrier();
+#pragma unroll
+    for (int q = 0; q < kMeshPending; ++q) pend[q] &= ~(hit << kPendShift);
+  }
+  return hit;
+}
+
 // One geom against the ray (traverse.py:131-192). Returns the distance or -1. kMesh = false
 // drops the mesh / SDF / height-field branch (for scenes without such geoms).
 // kPlaneOnly: the geom is known to be a plane (the direct geoms).
@@ -362,7 +479,9 @@ MJ_DEV bool trace_any(const SceneView& sv, V3 o, V3 d, float max_dist, bool cull
 // which the rays' directions differ in sign or have a zero component constrains nothing). Each
 // unresolved ray is tested against each reached leaf's geom exactly as in trace_any, so every
 // ray's result is that of trace_any: a geom a ray hits lies in its padded box, which its own slab
-// test, and so the hull test, accepts.
+// test, and so the hull test, accepts. With kMesh, the mesh geoms reached are tested after the
+// walk (up to kMeshPending of them; more are tested where reached), shared out over the wave's
+// lanes by run_mesh_jobs.
 template <bool kMesh, int N>
 MJ_DEV uint32_t trace_any_packet(const SceneView& sv, V3 o, const V3 (&d)[N], const float (&tmax)[N], int n, bool cull) {
   const uint32_t all = n >= 32 ? 0xffffffffu : ((1u << n) - 1u);
@@ -411,19 +530,43 @@ MJ_DEV uint32_t trace_any_packet(const SceneView& sv, V3 o, const V3 (&d)[N], co
     }
     return (tmx >= tmin && tmin < cut) ? tmin : -1.0f;
   };
+  uint32_t pend[kMeshPending] = {0u, 0u, 0u, 0u};
   // tests the unresolved rays against geom gi; true when every ray is resolved
   auto visit = [&](int gi, auto kPlane) {
     if (gi < 0) return false;
     const float* g = sv.geoms + kGeomRec * gi;
     const int bits = __float_as_int(g[15]);
     if (bits & 0x100) return false;
+    if (kMesh && !decltype(kPlane)::value && (bits & 0xff) == MJPBR_GEOM_MESH) {
+      // mesh, SDF and height-field geoms (staged as meshes): deferred to run_mesh_jobs
+      if ((bits >> 9) - 1 < 0) return false;
+      const uint32_t rec = (uint32_t)gi | ((all & ~hit) << kPendShift);
+      bool queued = false;
+#pragma unroll
+      for (int p = 0; p < kMeshPending; ++p)
+        if (!queued && pend[p] == 0u) {
+          pend[p] = rec;
+          queued = true;
+        }
+      if (queued) return false;
+      // more pending meshes than slots: test this one here, ray by ray
+#pragma unroll
+      for (int k = 0; k < N; ++k) {
+        if (k >= n || (hit >> k) & 1u) continue;
+        V3 nrm;
+        float bu, bv;
+        int face, mesh;
+        if (intersect_geom<true, true>(sv, g, o, d[k], tmax[k], &nrm, &bu, &bv, &face, &mesh) >= 0.0f) hit |= 1u << k;
+      }
+      return hit == all;
+    }
 #pragma unroll
     for (int k = 0; k < N; ++k) {
       if (k >= n || (hit >> k) & 1u) continue;
       V3 nrm = v3s(0.0f);
       float bu, bv;
       int face, mesh;
-      float dd = intersect_geom<true, kMesh, decltype(kPlane)::value>(sv, g, o, d[k], tmax[k], &nrm, &bu, &bv, &face, &mesh);
+      float dd = intersect_geom<true, false, decltype(kPlane)::value>(sv, g, o, d[k], tmax[k], &nrm, &bu, &bv, &face, &mesh);
       if (cull && dd >= 0.0f && dot(d[k], nrm) > 0.0f) dd = -1.0f;
       if (dd >= 0.0f && dd < tmax[k]) hit |= 1u << k;
     }
@@ -449,13 +592,15 @@ MJ_DEV uint32_t trace_any_packet(const SceneView& sv, V3 o, const V3 (&d)[N], co
     } else if (visit(sv.tlas_leaf[node - ninner], std::false_type{})) {
       return hit;
     }
-    if (trail == 0u) return hit;
+    if (trail == 0u) break;
     const int L = 31 - __builtin_clz(trail);
     trail &= ~(1u << L);
     const int anc1 = (node + 1) >> (level - L - 1);
     node = (anc1 ^ 1) - 1;
     level = L + 1;
   }
+  if (kMesh) hit = run_mesh_jobs<N>(sv, o, d, tmax, pend, hit);
+  return hit;
 }
 
 }  // namespace mjpbr
