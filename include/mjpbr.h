@@ -35,6 +35,8 @@ extern "C" {
 #define MJPBR_NTEXROLE 10
 /* Mip levels per texture / environment map at most. */
 #define MJPBR_MAX_LEVELS 16
+/* World groups of one wavefront batch at most (mjpbr_pt_wavefront_prepare). */
+#define MJPBR_WF_MAX_GROUPS 64
 /* Triangles per mesh-BVH leaf (padding slots have face id -1). */
 #define MJPBR_BLAS_LEAF 4
 
@@ -316,6 +318,8 @@ typedef struct MjpbrWavefrontConfig {
 typedef struct MjpbrWavefrontPlan {
   int64_t nitems;   /* 16x16 tiles over every world of every group */
   int64_t npixels;  /* pixels over every world of every group */
+  int64_t nstaged;  /* worlds of the groups whose records exceed the LDS (staged into their
+                       geo.world_stage by each launch and read from there) */
   int32_t lds_bytes;
   int32_t ngroup;
   MjpbrSettings settings;
@@ -374,7 +378,7 @@ int mjpbr_rt_kernel_features(const MjpbrSettings* s);
 int mjpbr_pt_render(const MjpbrScene* scene, const MjpbrSettings* s, MjpbrOutputs* out, int32_t seed,
                     void* stream);
 
-/* Persistent wavefront path tracer over up to 8 world groups, each with its own cameras, world
+/* Persistent wavefront path tracer over up to MJPBR_WF_MAX_GROUPS world groups, each with its own cameras, world
  * count and outputs (mixed resolutions in one launch); images bit-identical to mjpbr_pt_render
  * of each group with the same seed (no counterpart in the reference, whose path tracer is the
  * per-pixel megakernel of pathtrace.py:340-438). Workspace bytes: mjpbr_pt_wavefront_workspace

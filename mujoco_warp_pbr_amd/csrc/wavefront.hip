@@ -4,7 +4,9 @@
 //
 // Work items are 16x16 pixel tiles of one world of one group (every group has its own cameras,
 // world count and outputs), handed out by a global atomic counter to a persistent grid. A
-// workgroup stages the item's world (scene BVH + geom frames) in LDS; its four waves then draw the
+// workgroup stages the item's world (scene BVH + geom frames) in LDS (a world whose records exceed
+// the LDS is staged into its group's world_stage by a launch before, and read from there); its
+// four waves then draw the
 // tile's 256 x spp paths from an LDS counter. Paths end after different numbers of segments
 // (misses, Russian roulette, depth limit): after each segment a wave ballots its finished lanes
 // and, once at least `refill` lanes are free, claims that many new paths with one LDS atomic and
@@ -14,14 +16,17 @@
 // so the images are bit-identical to the megakernel's.
 //
 // Every launch is asynchronous and allocation-free on the caller's stream (the work counter is
-// cleared by a kernel, not a memset), so the three launches can be captured in a graph.
+// cleared by a kernel, not a memset), so the launches (counter reset, the staging of oversized
+// worlds when a group has them, the persistent kernel, the resolve) can be captured in a graph.
+// A batch holds up to MJPBR_WF_MAX_GROUPS groups; items and pixels find their group by a binary
+// search over the groups' first indices.
 #pragma clang fp contract(off)
 #include "abi_util.hpp"
 #include "path.hpp"
 
 namespace mjpbr {
 
-constexpr int kWfMaxGroups = 8;
+constexpr int kWfMaxGroups = MJPBR_WF_MAX_GROUPS;
 constexpr int kTilePixels = kTile * kTile;
 
 struct WfGroup {
@@ -30,6 +35,8 @@ struct WfGroup {
   TileMap tm;
   int64_t item_base;   // first work item of the group
   int64_t pixel_base;  // first resolve index (worlds x total_rays before this group)
+  int64_t stage_base;  // global groups: first index of their worlds in wf_stage_kernel's grid
+  int32_t global;      // 1: the world's records are read in place from sc.geo.world_stage
 };
 
 struct WfHeader {
@@ -40,6 +47,19 @@ struct WfHeader {
   WfGroup groups[kWfMaxGroups];
 };
 
+// The records of every world of the global groups into their world_stage (as stage_prep_kernel
+// does for one scene), one workgroup a world; runs before wf_kernel on the stream.
+__global__ void __launch_bounds__(256) wf_stage_kernel(const WfHeader* __restrict__ hdr) {
+  const int64_t b = blockIdx.x;
+  for (int g = 0; g < hdr->ngroup; ++g) {
+    const WfGroup& G = hdr->groups[g];
+    if (!G.global || b < G.stage_base || b >= G.stage_base + G.sc.geo.nworld) continue;
+    const int w = (int)(b - G.stage_base);
+    stage_world_to(G.sc, w, G.sc.geo.world_stage + (int64_t)w * stage_floats(G.sc));
+    return;
+  }
+}
+
 constexpr size_t kHeaderBytes = (sizeof(WfHeader) + 255) / 256 * 256;
 constexpr size_t kCounterBytes = 256;
 
@@ -47,10 +67,17 @@ MJ_DEV int lanes_below(uint64_t mask) {
   return (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
 }
 
+// The last group whose first work item (by_pixel: first resolve index) is at most `index`: a
+// binary search over the groups' non-decreasing bases.
 MJ_DEV int find_group(const WfHeader* h, int64_t index, bool by_pixel) {
-  int g = 0;
-  while (g + 1 < h->ngroup && index >= (by_pixel ? h->groups[g + 1].pixel_base : h->groups[g + 1].item_base)) ++g;
-  return g;
+  int lo = 0, hi = h->ngroup - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    const int64_t base = by_pixel ? h->groups[mid].pixel_base : h->groups[mid].item_base;
+    if (index >= base) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
 }
 
 __global__ void wf_reset_kernel(int* counter) {
@@ -89,7 +116,9 @@ wf_kernel(const WfHeader* __restrict__ hdr, int* counter, float* slots, MjpbrSet
     while (cidx + 1 < G.tm.nrender && t >= G.tm.tile_base[cidx + 1]) ++cidx;
     const int tl = t - G.tm.tile_base[cidx];
     const int tx = tl % G.tm.tiles_x[cidx], ty = tl / G.tm.tiles_x[cidx];
-    SceneView sv = stage_world(sc, w, lds, G.tm.mesh_cache_nodes, G.tm.mesh_cache_n);
+    // (G is the same for the whole workgroup, so both branches keep their barriers uniform)
+    SceneView sv = G.global ? global_world(sc, w, lds, G.tm.mesh_cache_nodes, G.tm.mesh_cache_n)
+                            : stage_world(sc, w, lds, G.tm.mesh_cache_nodes, G.tm.mesh_cache_n);
     const float exposure = sc.env.exposure[w % sc.env.exposure_n];
     PtCtx c{sc, s, sv, w, exposure, &rays};
     const int total_rays = sc.cams.total_rays;
@@ -233,24 +262,29 @@ extern "C" int mjpbr_pt_wavefront_prepare(const MjpbrScene* scenes, const MjpbrO
   WfHeader* h = new WfHeader();
   h->ngroup = ngroup;
   h->spp = s->path_samples;
-  int64_t items = 0, pixels = 0;
+  int64_t items = 0, pixels = 0, staged = 0;
   size_t lds = 0;
   for (int g = 0; g < ngroup; ++g) {
     WfGroup& G = h->groups[g];
     G.sc = scenes[g];
-    G.sc.geo.world_stage = nullptr;  // items stage their world directly (no per-render prep)
     G.out = outs[g];
     int nlaunch;
     int rc = make_tile_map(&scenes[g], &G.tm, &nlaunch);
+    bool global = false;
+    if (!rc) rc = global_records(&scenes[g], &global);
+    if (!rc && global && reinterpret_cast<uintptr_t>(scenes[g].geo.world_stage) % 16)
+      rc = set_error(MJPBR_EINVAL, "wavefront: group %d world_stage must be 16-byte aligned", g);
     if (rc) {
       delete h;
       return rc;
     }
-    size_t b = host_stage_bytes(&scenes[g]);
-    if (b > 64 * 1024) {
-      delete h;
-      return set_error(MJPBR_EINVAL, "wavefront: group %d scene too large for LDS staging (%d geoms)", g, scenes[g].geo.ngeom);
-    }
+    // groups whose records fit the LDS stage their world directly (no per-render prep); the others
+    // are staged into their world_stage by wf_stage_kernel and read from there
+    G.global = global ? 1 : 0;
+    G.stage_base = staged;
+    if (global) staged += scenes[g].geo.nworld;
+    else G.sc.geo.world_stage = nullptr;
+    size_t b = global ? 0 : host_stage_bytes(&scenes[g]);
     b += plan_mesh_cache(&scenes[g], s->has_meshes != 0, b, &G.tm);
     lds = b > lds ? b : lds;
     G.item_base = items;
@@ -271,6 +305,7 @@ extern "C" int mjpbr_pt_wavefront_prepare(const MjpbrScene* scenes, const MjpbrO
   if (e != hipSuccess) return set_error(MJPBR_ELAUNCH, "wavefront prepare: %s", hipGetErrorString(e));
   plan->nitems = items;
   plan->npixels = pixels;
+  plan->nstaged = staged;
   plan->lds_bytes = (int32_t)lds;
   plan->ngroup = ngroup;
   plan->settings = *s;
@@ -291,6 +326,12 @@ extern "C" int mjpbr_pt_wavefront_launch(void* workspace, const MjpbrWavefrontPl
   hipLaunchKernelGGL(wf_reset_kernel, dim3(1), dim3(64), 0, st, counter);
   int rc = check_launch("wf_reset_kernel");
   if (rc) return rc;
+  if (plan->nstaged > 0) {
+    if (plan->nstaged >= (1ll << 31)) return set_error(MJPBR_EINVAL, "wavefront: too many staged worlds (%lld)", (long long)plan->nstaged);
+    hipLaunchKernelGGL(wf_stage_kernel, dim3((unsigned)plan->nstaged), dim3(256), 0, st, hdr);
+    rc = check_launch("wf_stage_kernel");
+    if (rc) return rc;
+  }
   int64_t items = cfg->item_limit > 0 && cfg->item_limit < plan->nitems ? cfg->item_limit : plan->nitems;
   int64_t grid = (int64_t)device_cus() * cfg->groups_per_cu;
   if (grid > items) grid = items;

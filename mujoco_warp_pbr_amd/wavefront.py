@@ -23,7 +23,8 @@ from mujoco_warp_pbr_amd.render import scene_struct
 from mujoco_warp_pbr_amd.render import settings_struct
 from mujoco_warp_pbr_amd.types import Tier
 
-MAX_GROUPS = 8
+# world groups a batch may hold (MJPBR_WF_MAX_GROUPS of include/mjpbr.h)
+MAX_GROUPS = _abi._defines(_abi._header_text())["MJPBR_WF_MAX_GROUPS"]
 
 
 @dataclasses.dataclass(frozen=True)

@@ -59,6 +59,16 @@ def test_oversized_world_without_world_stage_is_refused_without_a_gpu():
     rc = getattr(lib, fn)(ctypes.addressof(sc), ctypes.addressof(s), ctypes.addressof(out), 0, None)
     assert rc == -1
     assert b"pass geo.world_stage" in lib.mjpbr_last_error(), fn
+  # the wavefront batch refuses such a group in prepare, before it touches the device
+  plan = S["MjpbrWavefrontPlan"]()
+  scenes_arr = (S["MjpbrScene"] * 1)(sc)
+  outs_arr = (S["MjpbrOutputs"] * 1)(out)
+  need = lib.mjpbr_pt_wavefront_workspace(ctypes.addressof(scenes_arr), 1, ctypes.addressof(s))
+  assert need > 0
+  rc = lib.mjpbr_pt_wavefront_prepare(ctypes.addressof(scenes_arr), ctypes.addressof(outs_arr), 1, ctypes.addressof(s), ctypes.c_void_p(256),
+                                      need, ctypes.addressof(plan), None)
+  assert rc == -1
+  assert b"pass geo.world_stage" in lib.mjpbr_last_error()
 
 
 def test_scene_bvh_limit_is_4096_geoms():
@@ -170,3 +180,35 @@ def test_gpu_crowd_path_traced_matches_oracle(cuda):
   np.testing.assert_allclose(gpu["depth"], ref["depth"], rtol=1e-5, atol=1e-7)
   diff = np.abs(gpu["hdr"] - ref["hdr"]).max(axis=-1)
   assert np.mean(diff <= 1e-3) >= 0.999, f"{np.mean(diff > 1e-3):.4%} of samples differ, max {diff.max()}"
+
+
+@pytest.mark.gpu
+def test_gpu_wavefront_batch_with_oversized_worlds_matches_megakernel(cuda):
+  """A wavefront batch whose first group's records exceed the LDS (600 geoms: staged into its
+  world_stage by the launch and read from there) beside a group staged in LDS: both groups'
+  images equal the per-pixel megakernel's bit for bit, also on a second launch after the geoms
+  moved (the staging runs every launch)."""
+  from mujoco_warp_pbr_amd.wavefront import WavefrontBatch
+
+  big = scenes.crowd_scene(4, 32, 600)
+  small = scenes.crowd_scene(5, 16, 40)
+  kw = dict(envmap=scenes.procedural_sky(2, 64, 128), path_samples=2, max_depth=4, **PATH)
+  groups = [make(big, nworld=2, device="cuda", cam_res=(32, 32), **kw), make(small, nworld=3, device="cuda", cam_res=(16, 16), **kw)]
+  assert io.world_stage_floats(groups[0][2].base.tlas_depth, big.ngeom, big.nlight, int(groups[0][2].base.direct_geom_ids.numel())) * 4 > 64 * 1024
+  batch = WavefrontBatch(groups)
+  for step in range(2):
+    if step:
+      for m, d, rc in groups:
+        d.geom_xpos += torch.tensor([0.05, -0.03, 0.0], device=cuda)
+    ref = []
+    for m, d, rc in groups:
+      mj.refit_bvh(m, d, rc)
+      mj.render(m, d, rc, seed=9)
+      ref.append(gpu_outputs(rc))
+      for t in (rc.base.rgb_data, rc.hdr_data, rc.base.depth_data, rc.base.seg_data):
+        t.fill_(7)
+    batch.render(seed=9)
+    for i, (m, d, rc) in enumerate(groups):
+      got = gpu_outputs(rc)
+      for k in ref[i]:
+        np.testing.assert_array_equal(got[k], ref[i][k], err_msg=f"launch {step} group {i} {k}")

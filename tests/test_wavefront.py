@@ -84,8 +84,8 @@ def test_batch_validation():
     WavefrontBatch(groups + [(m2, d2, rc2)])
   with pytest.raises(ValueError, match="runs on a GPU"):
     WavefrontBatch(groups)
-  with pytest.raises(ValueError, match="1 to 8"):
-    WavefrontBatch(groups * 9)
+  with pytest.raises(ValueError, match="1 to 64"):
+    WavefrontBatch(groups * 65)
 
 
 @pytest.mark.gpu
@@ -102,6 +102,25 @@ def test_gpu_wavefront_matches_megakernel(cuda):
     batch.render(seed=11, config=cfg)
     for i, (_, _, rc) in enumerate(groups):
       _assert_same(_outputs(rc), ref[i], f"group {i} {cfg}")
+
+
+@pytest.mark.gpu
+def test_gpu_wavefront_batch_of_many_groups_matches_megakernel(cuda):
+  """Twelve world groups (more than the first builds allowed) of mixed resolutions, camera counts
+  and world counts in one launch: every group's images equal the megakernel's bit for bit."""
+  shapes = [(1, [(16, 16), (8, 8)]), (2, [(16, 16), (8, 24)]), (1, [(32, 16), (16, 16)]), (3, [(16, 16), (16, 8)]),
+            (1, [(24, 24), (16, 16)]), (2, [(8, 8), (24, 16)])] * 2
+  groups = [_group(cuda, n, res, 31 + 7 * i) for i, (n, res) in enumerate(shapes)]
+  ref = []
+  for m, d, rc in groups:
+    mj.render(m, d, rc, seed=5)
+    ref.append(_outputs(rc))
+  batch = WavefrontBatch(groups)
+  for _, _, rc in groups:
+    _clear(rc)
+  batch.render(seed=5)
+  for i, (_, _, rc) in enumerate(groups):
+    _assert_same(_outputs(rc), ref[i], f"group {i}")
 
 
 @pytest.mark.gpu
