@@ -29,8 +29,9 @@ extern "C" {
 #define MJPBR_EINVAL (-1)  /* bad argument (shape, size, enum) */
 #define MJPBR_ELAUNCH (-2) /* HIP launch or runtime error */
 
-/* Rendering cameras are few; their layout is passed by value. */
-#define MJPBR_MAX_CAMS 16
+/* Rendering cameras of one context at most; their layout is passed by value (kernel arguments of
+ * the render kernels stay under 4 KB). */
+#define MJPBR_MAX_CAMS 32
 /* Texture roles (MuJoCo mjtTextureRole). */
 #define MJPBR_NTEXROLE 10
 /* Mip levels per texture / environment map at most. */

@@ -432,7 +432,8 @@ def create_render_context(
 
 
 def render_struct_max_cams() -> int:
-  return 16
+  """Rendering cameras a context may have (MJPBR_MAX_CAMS of include/mjpbr.h)."""
+  return _abi._defines(_abi._header_text())["MJPBR_MAX_CAMS"]
 
 
 # Planes tested before the scene BVH when a model has at most this many (see create_render_context).

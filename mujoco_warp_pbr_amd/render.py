@@ -17,7 +17,6 @@ from mujoco_warp_pbr_amd import _abi
 from mujoco_warp_pbr_amd.types import PbrRenderContext
 from mujoco_warp_pbr_amd.types import Tier
 
-MAX_CAMS = 16
 NTEXROLE = 10
 
 

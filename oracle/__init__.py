@@ -14,6 +14,7 @@ from __future__ import annotations
 
 import copy
 import ctypes
+import hashlib
 import dataclasses
 import subprocess
 import threading
@@ -41,6 +42,10 @@ def library() -> ctypes.CDLL:
       if not LIB.exists():
         build()
       lib = ctypes.CDLL(str(LIB))
+      lib.oracle_header_hash.restype = ctypes.c_char_p
+      want = hashlib.sha256((HERE.parent / "include" / "mjpbr.h").read_bytes()).hexdigest()[:16]
+      if lib.oracle_header_hash().decode() != want:
+        raise RuntimeError(f"{LIB} was built against another include/mjpbr.h: rebuild it (make -C oracle)")
       vp = ctypes.c_void_p
       lib.oracle_rt_render.argtypes = [vp, vp, vp, ctypes.c_int32, ctypes.c_int, ctypes.c_int]
       lib.oracle_rt_render.restype = ctypes.c_int

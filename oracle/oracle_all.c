@@ -3,3 +3,6 @@
 #include "rt_oracle.c"
 #include "pt_oracle.c"
 #include "atrous_oracle.c"
+
+/* The ABI header's hash this library was built against (oracle/__init__.py compares it). */
+const char* oracle_header_hash(void) { return MJPBR_HEADER_HASH; }

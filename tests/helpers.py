@@ -138,3 +138,19 @@ class Shot:
   @property
   def exposure(self) -> float:
     return float(self.rc.exposure.reshape(-1)[0])
+
+
+def many_cameras_scene(ncam: int):
+  """Plane, a sphere, a box and a capsule watched by `ncam` cameras on a circle (their sizes vary:
+  8x8 to 16x12), a sun and a point light."""
+  b = _Builder()
+  b.geom(mdl.GEOM_PLANE, (2, 2, 0.1), (0, 0, 0), matid=b.material(rgba=(0.8, 0.8, 0.8, 1), roughness=0.7))
+  b.geom(mdl.GEOM_SPHERE, (0.2, 0, 0), (0.0, 0.0, 0.2), matid=b.material(rgba=(0.9, 0.3, 0.2, 1), roughness=0.4))
+  b.geom(mdl.GEOM_BOX, (0.15, 0.1, 0.1), (0.4, 0.1, 0.1), matid=b.material(rgba=(0.2, 0.6, 0.9, 1), roughness=0.5, metallic=1.0))
+  b.geom(mdl.GEOM_CAPSULE, (0.08, 0.15, 0), (-0.3, -0.2, 0.25), matid=b.material(rgba=(0.3, 0.9, 0.3, 1), roughness=0.3))
+  b.light(mdl.LIGHT_DIRECTIONAL, (0, 0, 3), (-0.4, 0.3, -0.86))
+  b.light(mdl.LIGHT_POINT, (0.6, -0.5, 1.5), (0, 0, -1))
+  for i in range(ncam):
+    a = 2 * np.pi * i / ncam
+    b.camera((2.0 * np.cos(a), 2.0 * np.sin(a), 1.2), (0, 0, 0.15), 40.0 + i, (8 + 8 * (i % 2), 8 + 4 * (i % 3 == 0)))
+  return b.build()

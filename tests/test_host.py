@@ -11,6 +11,7 @@ import pytest
 import torch
 
 import mujoco_warp_pbr_amd as mj
+from tests.helpers import many_cameras_scene
 from oracle import tables as oracle_tables
 from mujoco_warp_pbr_amd import bvh
 from mujoco_warp_pbr_amd import envmap
@@ -244,3 +245,13 @@ def test_env_id_range():
   rc.env_id = torch.tensor([0, 2], dtype=torch.int32)
   with pytest.raises(ValueError, match=r"env_id must be in \[0, 2\)"):
     check_env_id(rc)
+
+
+def test_rendering_camera_count_limit():
+  """Up to MJPBR_MAX_CAMS (32) rendering cameras a context; more are refused with a clear error."""
+  m20 = many_cameras_scene(20)
+  rc = mj.create_render_context(m20, nworld=1, cam_res=(8, 8), device="cpu", precompute=False)
+  assert rc.base.nrender == 20 and rc.base.total_rays == 20 * 64
+  m33 = many_cameras_scene(33)
+  with pytest.raises(ValueError, match="at most 32 rendering cameras"):
+    mj.create_render_context(m33, nworld=1, cam_res=(8, 8), device="cpu", precompute=False)

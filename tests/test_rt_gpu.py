@@ -18,6 +18,7 @@ from tests.helpers import EV100_UNIT
 from tests.helpers import compare_rt
 from tests.helpers import gpu_outputs
 from tests.helpers import make
+from tests.helpers import many_cameras_scene
 from tests.helpers import plane_and_light
 
 pytestmark = pytest.mark.gpu
@@ -193,3 +194,15 @@ def test_every_kernel_instance_matches_oracle(cuda, mask, scene, kw):
   gpu, ref, (m, d, rc) = _render_both(mjm, 2, (32, 32), setup=setup, envmap=scenes.procedural_sky(mask % 5, 64, 128), **kw)
   assert _abi.library().mjpbr_rt_kernel_features(_abi.byref(settings_struct(rc))) == mask
   compare_rt(gpu, ref)
+
+
+def test_many_cameras_match_oracle(cuda):
+  """A context with 20 rendering cameras (past the 16 of the first builds; 32 at most) of mixed
+  resolutions: every camera's images follow the oracle."""
+  mjm = many_cameras_scene(20)
+  res = [(8 + 8 * (i % 2), 8 + 4 * (i % 3 == 0)) for i in range(20)]
+  gpu, ref, (_, _, rc) = _render_both(mjm, 2, res, envmap=scenes.constant_sky(), samples_per_pixel=2)
+  assert rc.base.nrender == 20
+  compare_rt(gpu, ref)
+  seg = gpu["seg"][..., 0]
+  assert len(np.unique(seg)) >= 5  # plane, three primitives, background
