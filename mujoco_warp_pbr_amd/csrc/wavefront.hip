@@ -84,8 +84,10 @@ __global__ void wf_reset_kernel(int* counter) {
   if (threadIdx.x == 0) counter[0] = 0;
 }
 
-// kCount: adds the rays traced to ray_counts (the same image).
-template <int kWavesPerEu, bool kCount>
+// kCount: adds the rays traced to ray_counts (the same image). kGlobal: some group reads its
+// worlds' records in place from world_stage (a per-item branch; 3 % slower, so only batches with
+// such a group take this instance).
+template <int kWavesPerEu, bool kCount, bool kGlobal>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kWavesPerEu, kWavesPerEu)))
 wf_kernel(const WfHeader* __restrict__ hdr, int* counter, float* slots, MjpbrSettings s, int seed, int refill, int64_t item_limit,
           uint64_t* ray_counts) {
@@ -117,8 +119,8 @@ wf_kernel(const WfHeader* __restrict__ hdr, int* counter, float* slots, MjpbrSet
     const int tl = t - G.tm.tile_base[cidx];
     const int tx = tl % G.tm.tiles_x[cidx], ty = tl / G.tm.tiles_x[cidx];
     // (G is the same for the whole workgroup, so both branches keep their barriers uniform)
-    SceneView sv = G.global ? global_world(sc, w, lds, G.tm.mesh_cache_nodes, G.tm.mesh_cache_n)
-                            : stage_world(sc, w, lds, G.tm.mesh_cache_nodes, G.tm.mesh_cache_n);
+    SceneView sv = kGlobal && G.global ? global_world(sc, w, lds, G.tm.mesh_cache_nodes, G.tm.mesh_cache_n)
+                                       : stage_world(sc, w, lds, G.tm.mesh_cache_nodes, G.tm.mesh_cache_n);
     const float exposure = sc.env.exposure[w % sc.env.exposure_n];
     PtCtx c{sc, s, sv, w, exposure, &rays};
     const int total_rays = sc.cams.total_rays;
@@ -338,13 +340,16 @@ extern "C" int mjpbr_pt_wavefront_launch(void* workspace, const MjpbrWavefrontPl
   const MjpbrSettings& s = plan->settings;
   using WfFn = void (*)(const WfHeader*, int*, float*, MjpbrSettings, int, int, int64_t, uint64_t*);
   const bool count = cfg->ray_counts != nullptr;
+  const bool global = plan->nstaged > 0;
   WfFn fn;
   switch (cfg->waves_per_eu) {
     case 4:
-      fn = count ? wf_kernel<4, true> : wf_kernel<4, false>;
+      fn = global ? (count ? wf_kernel<4, true, true> : wf_kernel<4, false, true>)
+                  : (count ? wf_kernel<4, true, false> : wf_kernel<4, false, false>);
       break;
     case 8:
-      fn = count ? wf_kernel<8, true> : wf_kernel<8, false>;
+      fn = global ? (count ? wf_kernel<8, true, true> : wf_kernel<8, false, true>)
+                  : (count ? wf_kernel<8, true, false> : wf_kernel<8, false, false>);
       break;
     default:
       return set_error(MJPBR_EINVAL, "wavefront: waves_per_eu must be 4 or 8, got %d", cfg->waves_per_eu);
