@@ -29,8 +29,9 @@ extern "C" {
 #define MJPBR_EINVAL (-1)  /* bad argument (shape, size, enum) */
 #define MJPBR_ELAUNCH (-2) /* HIP launch or runtime error */
 
-/* Rendering cameras of one context at most; their layout is passed by value (kernel arguments of
- * the render kernels stay under 4 KB). */
+/* Rendering cameras of one launch at most; their layout is passed by value (kernel arguments of
+ * the render kernels stay under 4 KB). A context with more cameras is rendered in launches of at
+ * most this many consecutive cameras (MjpbrCameras.ray_base). */
 #define MJPBR_MAX_CAMS 32
 /* Texture roles (MuJoCo mjtTextureRole). */
 #define MJPBR_NTEXROLE 10
@@ -163,8 +164,10 @@ typedef struct MjpbrCameras {
   int32_t ncam;
   int32_t cam_fovy_n;
   int32_t cam_intrinsic_n;
-  int32_t total_rays;               /* sum of the rendering cameras' pixels */
-  int32_t pad;
+  int32_t total_rays;               /* sum of every rendering camera's pixels, this launch's or not */
+  int32_t ray_base;                 /* ray id of this launch's first camera's first pixel: the pixels of
+                                       the context's cameras before it (0 when one launch renders them
+                                       all); ray ids seed the per-pixel random numbers */
   int32_t cam_res[MJPBR_MAX_CAMS][2];
   int32_t cam_id_map[MJPBR_MAX_CAMS];
   const int32_t* cam_projection;    /* [ncam] 0 perspective, 1 orthographic */

@@ -2,6 +2,8 @@
 // tile maps of the render kernels.
 #pragma once
 #include <hip/hip_runtime.h>
+#include <algorithm>
+#include <climits>
 #include <stdarg.h>
 #include <stdio.h>
 #include "../../include/mjpbr.h"
@@ -59,18 +61,24 @@ inline int make_tile_map(const MjpbrScene* sc, TileMap* tm, int* nlaunch) {
     return set_error(MJPBR_EINVAL, "ndirect must be in [0, ngeom] with direct_geom set, got %d", sc->geo.ndirect);
   tm->nrender = c.nrender;
   tm->mesh_cache_nodes = tm->mesh_cache_n = 0;
-  int base = 0, rays = 0;
+  if (c.ray_base < 0) return set_error(MJPBR_EINVAL, "ray_base must not be negative, got %d", c.ray_base);
+  int base = 0;
+  int64_t rays = c.ray_base;
   for (int i = 0; i < c.nrender; ++i) {
     int w = c.cam_res[i][0], h = c.cam_res[i][1];
     if (w < 1 || h < 1) return set_error(MJPBR_EINVAL, "camera %d has resolution %dx%d", i, w, h);
     tm->tile_base[i] = base;
     tm->tiles_x[i] = (w + kTile - 1) / kTile;
-    tm->ray_offset[i] = rays;
+    tm->ray_offset[i] = (int)std::min<int64_t>(rays, INT32_MAX);
     base += tm->tiles_x[i] * ((h + kTile - 1) / kTile);
-    rays += w * h;
+    rays += (int64_t)w * h;
   }
   tm->tile_base[c.nrender] = base;
-  if (rays != c.total_rays) return set_error(MJPBR_EINVAL, "total_rays %d != sum of camera pixels %d", c.total_rays, rays);
+  // this launch's cameras end at or before the context's last pixel (at it when ray_base is 0 and
+  // one launch renders every camera)
+  if (rays > c.total_rays || (c.ray_base == 0 && rays != c.total_rays && c.nrender < MJPBR_MAX_CAMS))
+    return set_error(MJPBR_EINVAL, "total_rays %d does not fit ray_base %d + the cameras' %lld pixels", c.total_rays, c.ray_base,
+                     (long long)(rays - c.ray_base));
   tm->tiles_per_world = base;
   int64_t nb = (int64_t)base * sc->geo.nworld;
   if (nb > (1ll << 30)) return set_error(MJPBR_EINVAL, "too many tiles (%lld)", (long long)nb);

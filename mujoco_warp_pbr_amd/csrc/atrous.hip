@@ -71,6 +71,8 @@ __global__ void demodulate_kernel(MjpbrAtrousParams p, MjpbrOutputs out, float4*
   if (i >= (int64_t)p.nworld * p.stride) return;
   int w = (int)(i / p.stride), index = (int)(i - (int64_t)w * p.stride);
   if (p.world_select && !p.world_select[w]) return;
+  // the pixels of this launch's cameras only (a context's cameras may be filtered in several launches)
+  if (locate(p, index).adr < 0) return;
   int64_t a = (int64_t)w * out.aov_stride + index;
   if (!finish) guides[i] = pack_guides(out.aov_normal + 3 * a, out.aov_albedo + 3 * a, out.aov_depth[a]);
   V3 albedo = ld_half3(out.aov_albedo + 3 * a);

@@ -6,17 +6,19 @@ from mujoco_warp_pbr_amd import _abi
 from mujoco_warp_pbr_amd.types import PbrRenderContext
 
 
-def atrous_params(rc: PbrRenderContext, nworld: int):
-  """MjpbrAtrousParams of a context: camera layout of the RGB buffer and the filter sigmas."""
+def atrous_params(rc: PbrRenderContext, nworld: int, cams: tuple[int, int] | None = None):
+  """MjpbrAtrousParams of a context: camera layout of the RGB buffer and the filter sigmas;
+  `cams` = (first, end) limits it to one launch's cameras (render.camera_chunks)."""
   S = _abi.structs()
   p = S["MjpbrAtrousParams"]()
   base = rc.base
+  first, end = (0, int(base.nrender)) if cams is None else cams
   p.nworld = nworld
-  p.nrender = base.nrender
-  for i in range(base.nrender):
-    p.cam_res[i][0] = int(base.cam_res[i][0])
-    p.cam_res[i][1] = int(base.cam_res[i][1])
-    p.rgb_adr[i] = int(base.rgb_adr[i])
+  p.nrender = end - first
+  for k, i in enumerate(range(first, end)):
+    p.cam_res[k][0] = int(base.cam_res[i][0])
+    p.cam_res[k][1] = int(base.cam_res[i][1])
+    p.rgb_adr[k] = int(base.rgb_adr[i])
   p.stride = int(base.rgb_data.shape[1])
   p.iterations = int(rc.denoise_iterations)
   p.sigma_color = float(rc.denoise_sigma_color)
@@ -30,9 +32,17 @@ def atrous_params(rc: PbrRenderContext, nworld: int):
   return p
 
 
-def denoise(rc: PbrRenderContext, nworld: int, out, stream: int) -> None:
+def denoise(rc: PbrRenderContext, nworld: int, stream: int, world_select=None) -> None:
   """Filters the noisy HDR in `rc.hdr_data` (with the first-hit AOVs) into the RGB (and HDR)
-  outputs, in place, on `stream`."""
-  p = atrous_params(rc, nworld)
+  outputs, in place, on `stream`, one launch per camera_chunks launch; `world_select` (int32
+  device tensor, one entry a world) filters only the worlds whose entry is non-zero."""
+  from mujoco_warp_pbr_amd.render import camera_chunks
+  from mujoco_warp_pbr_amd.render import outputs_struct
+
   a, b, guides = rc.denoise_buffers
-  _abi.call("mjpbr_atrous", _abi.byref(p), _abi.byref(out), _abi.ptr(a), _abi.ptr(b), _abi.ptr(guides), stream)
+  for cams in camera_chunks(rc):
+    p = atrous_params(rc, nworld, cams)
+    if world_select is not None:
+      p.world_select = _abi.ptr(world_select)
+    out = outputs_struct(rc, cams)
+    _abi.call("mjpbr_atrous", _abi.byref(p), _abi.byref(out), _abi.ptr(a), _abi.ptr(b), _abi.ptr(guides), stream)

@@ -245,8 +245,6 @@ def create_render_context(
       cams = [int(c) for c in cam_active]
   if not cams:
     raise ValueError("the model has no active camera to render")
-  if len(cams) > render_struct_max_cams():
-    raise ValueError(f"at most {render_struct_max_cams()} rendering cameras are supported, got {len(cams)}")
   nrender = len(cams)
   if cam_res is None:
     res = [tuple(int(x) for x in mjm.cam_resolution[c]) for c in cams]
@@ -429,11 +427,6 @@ def create_render_context(
 
     set_camera_rays(rc, camera_rays)
   return rc
-
-
-def render_struct_max_cams() -> int:
-  """Rendering cameras a context may have (MJPBR_MAX_CAMS of include/mjpbr.h)."""
-  return _abi._defines(_abi._header_text())["MJPBR_MAX_CAMS"]
 
 
 # Planes tested before the scene BVH when a model has at most this many (see create_render_context).

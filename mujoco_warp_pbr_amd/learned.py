@@ -404,7 +404,7 @@ class Denoiser:
     self.stats = torch.zeros((n, NSTATS), dtype=torch.float32, device=dev)
     self.fallback = torch.zeros((n,), dtype=torch.int32, device=dev)
 
-  def __call__(self, rc, nworld: int, out_struct, stream: int) -> None:
+  def __call__(self, rc, nworld: int, stream: int) -> None:
     """Filters the noisy HDR in rc.hdr_data into the RGB (and HDR) outputs: the U-Net for worlds
     whose guide statistics are in the trained range, the a-trous filter for the others."""
     from mujoco_warp_pbr_amd import denoise as atrous
@@ -419,10 +419,7 @@ class Denoiser:
         "mjpbr_unet_guide_stats", _abi.ptr(x), nworld, h, w, IN_CHANNELS, _abi.ptr(self.range), _abi.ptr(self.stats),
         _abi.ptr(self.fallback), int(k > 0), stream,
       )
-    p = atrous.atrous_params(rc, nworld)
-    p.world_select = _abi.ptr(self.fallback)
-    a, b, guides = rc.denoise_buffers
-    _abi.call("mjpbr_atrous", _abi.byref(p), _abi.byref(out_struct), _abi.ptr(a), _abi.ptr(b), _abi.ptr(guides), stream)
+    atrous.denoise(rc, nworld, stream, world_select=self.fallback)
     for cam in cams:
       x = self.inputs[cam]
       w, h = int(base.cam_res[cam][0]), int(base.cam_res[cam][1])
@@ -443,8 +440,8 @@ def denoiser_for(rc) -> Denoiser:
   return d
 
 
-def denoise(rc, nworld: int, out_struct, stream: int) -> None:
-  denoiser_for(rc)(rc, nworld, out_struct, stream)
+def denoise(rc, nworld: int, stream: int) -> None:
+  denoiser_for(rc)(rc, nworld, stream)
 
 
 def guide_range_from_stats(stats: np.ndarray, margin: float = 0.1) -> list[float]:

@@ -83,8 +83,19 @@ def geometry_struct(m, d, rc: PbrRenderContext):
   return g
 
 
-def scene_struct(m, d, rc: PbrRenderContext):
-  """The MjpbrScene of the current frame."""
+def camera_chunks(rc: PbrRenderContext) -> list[tuple[int, int]]:
+  """The rendering cameras as launches of at most MJPBR_MAX_CAMS consecutive ones: [(first, end)]."""
+  n, step = int(rc.base.nrender), _max_cams()
+  return [(a, min(a + step, n)) for a in range(0, n, step)]
+
+
+def _max_cams() -> int:
+  return _abi._defines(_abi._header_text())["MJPBR_MAX_CAMS"]
+
+
+def scene_struct(m, d, rc: PbrRenderContext, cams: tuple[int, int] | None = None):
+  """The MjpbrScene of the current frame; `cams` = (first, end) limits its cameras to one launch's
+  (camera_chunks), all of them by default."""
   S = _abi.structs()
   sc = S["MjpbrScene"]()
   sc.geo = geometry_struct(m, d, rc)
@@ -147,15 +158,17 @@ def scene_struct(m, d, rc: PbrRenderContext):
   # cameras
   base = rc.base
   cm = sc.cams
-  cm.nrender = base.nrender
+  first, end = _chunk(rc, cams)
+  cm.nrender = end - first
   cm.ncam = m.ncam
   cm.cam_fovy_n = _n(m.cam_fovy)
   cm.cam_intrinsic_n = _n(m.cam_intrinsic)
   cm.total_rays = base.total_rays
-  for i in range(base.nrender):
-    cm.cam_res[i][0] = int(base.cam_res[i][0])
-    cm.cam_res[i][1] = int(base.cam_res[i][1])
-    cm.cam_id_map[i] = int(base.cam_id_map[i])
+  cm.ray_base = sum(int(base.cam_res[i][0]) * int(base.cam_res[i][1]) for i in range(first))
+  for k, i in enumerate(range(first, end)):
+    cm.cam_res[k][0] = int(base.cam_res[i][0])
+    cm.cam_res[k][1] = int(base.cam_res[i][1])
+    cm.cam_id_map[k] = int(base.cam_id_map[i])
   cm.cam_projection = _abi.ptr(m.cam_projection)
   cm.cam_fovy = _abi.ptr(_check(m.cam_fovy, torch.float32, "cam_fovy"))
   cm.cam_sensorsize = _abi.ptr(m.cam_sensorsize)
@@ -258,18 +271,28 @@ def _check_output_buffers(rc: PbrRenderContext) -> None:
       raise ValueError(f"rc.{flag} is set but its output buffers were not allocated for it: create the context with {flag}=True")
 
 
-def outputs_struct(rc: PbrRenderContext):
+def _chunk(rc: PbrRenderContext, cams: tuple[int, int] | None) -> tuple[int, int]:
+  n = int(rc.base.nrender)
+  first, end = (0, n) if cams is None else (int(cams[0]), int(cams[1]))
+  if not 0 <= first < end <= n or end - first > _max_cams():
+    raise ValueError(f"a launch renders 1 to {_max_cams()} of the context's {n} cameras, got [{first}, {end})")
+  return first, end
+
+
+def outputs_struct(rc: PbrRenderContext, cams: tuple[int, int] | None = None):
+  """The MjpbrOutputs of the context; `cams` as in scene_struct."""
   _check_output_buffers(rc)
   S = _abi.structs()
   o = S["MjpbrOutputs"]()
   base = rc.base
-  for i in range(base.nrender):
-    o.render_rgb[i] = int(bool(base.render_rgb[i]))
-    o.render_depth[i] = int(bool(base.render_depth[i]))
-    o.render_seg[i] = int(bool(base.render_seg[i]))
-    o.rgb_adr[i] = int(base.rgb_adr[i])
-    o.depth_adr[i] = int(base.depth_adr[i])
-    o.seg_adr[i] = int(base.seg_adr[i])
+  first, end = _chunk(rc, cams)
+  for k, i in enumerate(range(first, end)):
+    o.render_rgb[k] = int(bool(base.render_rgb[i]))
+    o.render_depth[k] = int(bool(base.render_depth[i]))
+    o.render_seg[k] = int(bool(base.render_seg[i]))
+    o.rgb_adr[k] = int(base.rgb_adr[i])
+    o.depth_adr[k] = int(base.depth_adr[i])
+    o.seg_adr[k] = int(base.seg_adr[i])
   o.rgb_stride = int(base.rgb_data.shape[1])
   o.depth_stride = int(base.depth_data.shape[1])
   o.seg_stride = int(base.seg_data.shape[1])
@@ -359,10 +382,12 @@ def render_path_samples(m, d, rc: PbrRenderContext, seed: int = 0) -> None:
     raise ValueError("render_path_samples needs a path-traced context (Tier.PATH)")
   check_env_id(rc)
   warn_ignored_overrides(m, rc)
-  sc = scene_struct(m, d, rc)
   s = settings_struct(rc)
-  out = outputs_struct(rc)
-  _abi.call("mjpbr_pt_render", _abi.byref(sc), _abi.byref(s), _abi.byref(out), ctypes.c_int32(int(seed)), _stream_of(d.geom_xpos))
+  stream = _stream_of(d.geom_xpos)
+  for cams in camera_chunks(rc):
+    sc = scene_struct(m, d, rc, cams)
+    out = outputs_struct(rc, cams)
+    _abi.call("mjpbr_pt_render", _abi.byref(sc), _abi.byref(s), _abi.byref(out), ctypes.c_int32(int(seed)), stream)
 
 
 def render(m, d, rc: PbrRenderContext, seed: int = 0) -> None:
@@ -374,20 +399,21 @@ def render(m, d, rc: PbrRenderContext, seed: int = 0) -> None:
     raise ValueError(f"data has {d.nworld} worlds, the context {rc.base.nworld}")
   check_env_id(rc)
   warn_ignored_overrides(m, rc)
-  sc = scene_struct(m, d, rc)
   s = settings_struct(rc)
-  out = outputs_struct(rc)
   stream = _stream_of(d.geom_xpos)
-  if rc.tier == Tier.PATH:
-    _abi.call("mjpbr_pt_render", _abi.byref(sc), _abi.byref(s), _abi.byref(out), ctypes.c_int32(int(seed)), stream)
-    if rc.denoise:
-      if rc.denoiser == "learned":
-        from mujoco_warp_pbr_amd import learned
+  # contexts with more cameras than one launch takes render them in consecutive launches
+  chunks = camera_chunks(rc)
+  entry = "mjpbr_pt_render" if rc.tier == Tier.PATH else "mjpbr_rt_render"
+  for cams in chunks:
+    sc = scene_struct(m, d, rc, cams)
+    out = outputs_struct(rc, cams)
+    _abi.call(entry, _abi.byref(sc), _abi.byref(s), _abi.byref(out), ctypes.c_int32(int(seed)), stream)
+  if rc.tier == Tier.PATH and rc.denoise:
+    if rc.denoiser == "learned":
+      from mujoco_warp_pbr_amd import learned
 
-        learned.denoise(rc, d.nworld, out, stream)
-      else:
-        from mujoco_warp_pbr_amd import denoise
+      learned.denoise(rc, d.nworld, stream)
+    else:
+      from mujoco_warp_pbr_amd import denoise
 
-        denoise.denoise(rc, d.nworld, out, stream)
-    return
-  _abi.call("mjpbr_rt_render", _abi.byref(sc), _abi.byref(s), _abi.byref(out), ctypes.c_int32(int(seed)), stream)
+      denoise.denoise(rc, d.nworld, stream)

@@ -63,6 +63,8 @@ class WavefrontBatch:
         raise ValueError(f"group {i}: denoising is done by render(); create the context with denoise=False")
       if d.nworld != rc.base.nworld:
         raise ValueError(f"group {i}: data has {d.nworld} worlds, the context {rc.base.nworld}")
+      if int(rc.base.nrender) > _abi._defines(_abi._header_text())["MJPBR_MAX_CAMS"]:
+        raise ValueError(f"group {i}: a wavefront group renders at most MJPBR_MAX_CAMS cameras, got {int(rc.base.nrender)} (use render())")
     s0 = bytes(settings_struct(groups[0][2]))
     for i, (_, _, rc) in enumerate(groups[1:], 1):
       if bytes(settings_struct(rc)) != s0:

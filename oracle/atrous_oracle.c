@@ -6,17 +6,29 @@ static v3 half3_o(const uint16_t* p) { return mk3(half2f(p[0]), half2f(p[1]), ha
 
 static float b3_o(int i) { return i == 2 ? 0.375f : ((i == 1 || i == 3) ? 0.25f : 0.0625f); }
 
+/* The camera of RGB pixel `index` among the params' cameras: its first pixel (-1: none of them),
+ * size and the pixel's index in it. A context's cameras may be filtered in several calls. */
+static int locate_o(const MjpbrAtrousParams* p, int index, int* W, int* H, int* pid) {
+  for (int c = 0; c < p->nrender; ++c) {
+    int ca = p->rgb_adr[c], np_ = p->cam_res[c][0] * p->cam_res[c][1];
+    if (ca >= 0 && index >= ca && index < ca + np_) {
+      *W = p->cam_res[c][0];
+      *H = p->cam_res[c][1];
+      *pid = index - ca;
+      return ca;
+    }
+  }
+  return -1;
+}
+
 /* One edge-avoiding pass of footprint `step` (reference denoise.py:105-178): src -> dst, both
  * [nworld, stride, 4] (demodulated colour, variance). */
 static void atrous_pass_o(const MjpbrAtrousParams* p, const MjpbrOutputs* out, const float* src, float* dst, int step) {
   int64_t n = (int64_t)p->nworld * p->stride;
   for (int64_t i = 0; i < n; ++i) {
     int w = (int)(i / p->stride), index = (int)(i - (int64_t)w * p->stride);
-    int adr = -1, W = 0, H = 0, pid = 0;
-    for (int c = 0; c < p->nrender; ++c) {
-      int ca = p->rgb_adr[c], np_ = p->cam_res[c][0] * p->cam_res[c][1];
-      if (ca >= 0 && index >= ca && index < ca + np_) { adr = ca; W = p->cam_res[c][0]; H = p->cam_res[c][1]; pid = index - ca; break; }
-    }
+    int W = 0, H = 0, pid = 0;
+    int adr = locate_o(p, index, &W, &H, &pid);
     if (adr < 0) continue;
     const float* sw = src + 4 * (int64_t)w * p->stride;
     int64_t fa = (int64_t)w * out->aov_stride;
@@ -73,9 +85,11 @@ int oracle_atrous(const MjpbrAtrousParams* p, MjpbrOutputs* out, float* work0, f
   int64_t n = (int64_t)p->nworld * p->stride;
   float* src = work0;
   float* dst = work1;
-  /* _demodulate */
+  int W, H, pid;
+  /* _demodulate (the pixels of this call's cameras) */
   for (int64_t i = 0; i < n; ++i) {
     int w = (int)(i / p->stride), index = (int)(i - (int64_t)w * p->stride);
+    if (locate_o(p, index, &W, &H, &pid) < 0) continue;
     int64_t a = (int64_t)w * out->aov_stride + index;
     v3 albedo = half3_o(out->aov_albedo + 3 * a);
     v3 c = sub3(mul3(load3(out->hdr + 3 * ((int64_t)w * out->hdr_stride + index)), p->exposure[w % p->exposure_n]),
@@ -93,9 +107,10 @@ int oracle_atrous(const MjpbrAtrousParams* p, MjpbrOutputs* out, float* work0, f
     src = dst;
     dst = t;
   }
-  /* _finish */
+  /* _finish (the pixels of this call's cameras) */
   for (int64_t i = 0; i < n; ++i) {
     int w = (int)(i / p->stride), index = (int)(i - (int64_t)w * p->stride);
+    if (locate_o(p, index, &W, &H, &pid) < 0) continue;
     int64_t a = (int64_t)w * out->aov_stride + index;
     v3 albedo = half3_o(out->aov_albedo + 3 * a);
     v3 exposed = add3(load3(out->aov_emission + 3 * a),

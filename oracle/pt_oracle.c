@@ -412,7 +412,7 @@ static void pt_pixel_o(const MjpbrScene* sc, const MjpbrSettings* s, MjpbrOutput
 int oracle_pt_render(const MjpbrScene* sc, const MjpbrSettings* s, MjpbrOutputs* out, int32_t seed, int w0, int w1,
                      float* samples) {
   for (int w = w0; w < w1; ++w) {
-    int ray_offset = 0;
+    int ray_offset = sc->cams.ray_base;  /* a launch of the context's later cameras */
     for (int c = 0; c < sc->cams.nrender; ++c) {
       int W = sc->cams.cam_res[c][0], H = sc->cams.cam_res[c][1];
       for (int py = 0; py < H; ++py)

@@ -1255,7 +1255,7 @@ static void render_pixel_o(const MjpbrScene* sc, const MjpbrSettings* s, MjpbrOu
 /* Renders worlds [w0, w1) of the real-time tier. Returns 0. */
 int oracle_rt_render(const MjpbrScene* sc, const MjpbrSettings* s, MjpbrOutputs* out, int32_t seed, int w0, int w1) {
   for (int w = w0; w < w1; ++w) {
-    int ray_offset = 0;
+    int ray_offset = sc->cams.ray_base;  /* a launch of the context's later cameras */
     for (int c = 0; c < sc->cams.nrender; ++c) {
       int W = sc->cams.cam_res[c][0], H = sc->cams.cam_res[c][1];
       for (int py = 0; py < H; ++py)

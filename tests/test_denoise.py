@@ -97,10 +97,9 @@ def test_gpu_atrous_matches_oracle_on_same_inputs(cuda, layout):
   # filtered HDR too: run the GPU filter again writing it
   from mujoco_warp_pbr_amd import denoise as dn
   from mujoco_warp_pbr_amd.render import _stream_of
-  from mujoco_warp_pbr_amd.render import outputs_struct
 
   rc.render_hdr = True
-  dn.denoise(rc, 2, outputs_struct(rc), _stream_of(rc.hdr_data))
+  dn.denoise(rc, 2, _stream_of(rc.hdr_data))
   torch.cuda.synchronize()
   gpu_hdr = rc.hdr_data.cpu().numpy()
   ref = oracle.atrous(rc, noisy, aov, render_hdr=True)

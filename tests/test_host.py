@@ -247,11 +247,26 @@ def test_env_id_range():
     check_env_id(rc)
 
 
-def test_rendering_camera_count_limit():
-  """Up to MJPBR_MAX_CAMS (32) rendering cameras a context; more are refused with a clear error."""
-  m20 = many_cameras_scene(20)
-  rc = mj.create_render_context(m20, nworld=1, cam_res=(8, 8), device="cpu", precompute=False)
-  assert rc.base.nrender == 20 and rc.base.total_rays == 20 * 64
-  m33 = many_cameras_scene(33)
-  with pytest.raises(ValueError, match="at most 32 rendering cameras"):
-    mj.create_render_context(m33, nworld=1, cam_res=(8, 8), device="cpu", precompute=False)
+def test_many_cameras_render_in_launches_of_at_most_32():
+  """Any number of rendering cameras: the renderer launches at most MJPBR_MAX_CAMS (32)
+  consecutive cameras at a time, each launch's ray ids continuing the context's (ray_base)."""
+  from mujoco_warp_pbr_amd.render import camera_chunks
+  from mujoco_warp_pbr_amd.render import outputs_struct
+  from mujoco_warp_pbr_amd.render import scene_struct
+
+  mjm = many_cameras_scene(40)
+  res = [(8 + 8 * (i % 2), 8 + 4 * (i % 3 == 0)) for i in range(40)]
+  rc = mj.create_render_context(mjm, nworld=1, cam_res=res, device="cpu", precompute=False)
+  assert rc.base.nrender == 40
+  assert camera_chunks(rc) == [(0, 32), (32, 40)]
+  m, d = mj.put_model(mjm, "cpu"), mj.put_data(mjm, 1, "cpu")
+  pixels = [w * h for w, h in res]
+  for first, end in camera_chunks(rc):
+    sc = scene_struct(m, d, rc, (first, end))
+    out = outputs_struct(rc, (first, end))
+    assert sc.cams.nrender == end - first and sc.cams.total_rays == sum(pixels)
+    assert sc.cams.ray_base == sum(pixels[:first])
+    assert [tuple(sc.cams.cam_res[k]) for k in range(end - first)] == res[first:end]
+    assert [out.rgb_adr[k] for k in range(end - first)] == [int(a) for a in rc.base.rgb_adr[first:end]]
+  with pytest.raises(ValueError, match="1 to 32"):
+    scene_struct(m, d, rc)

@@ -196,13 +196,43 @@ def test_every_kernel_instance_matches_oracle(cuda, mask, scene, kw):
   compare_rt(gpu, ref)
 
 
-def test_many_cameras_match_oracle(cuda):
-  """A context with 20 rendering cameras (past the 16 of the first builds; 32 at most) of mixed
-  resolutions: every camera's images follow the oracle."""
-  mjm = many_cameras_scene(20)
-  res = [(8 + 8 * (i % 2), 8 + 4 * (i % 3 == 0)) for i in range(20)]
-  gpu, ref, (_, _, rc) = _render_both(mjm, 2, res, envmap=scenes.constant_sky(), samples_per_pixel=2)
-  assert rc.base.nrender == 20
+@pytest.mark.parametrize("ncam", [20, 40])
+def test_many_cameras_match_oracle(cuda, ncam):
+  """Contexts with 20 and 40 rendering cameras of mixed resolutions (40: two launches of 32 and 8
+  cameras, the second's ray ids continuing the first's), soft shadows and supersampling seeded by
+  the ray ids: every camera's images follow the oracle, and get_rgb reads the last camera."""
+  mjm = many_cameras_scene(ncam)
+  res = [(8 + 8 * (i % 2), 8 + 4 * (i % 3 == 0)) for i in range(ncam)]
+
+  def soft(m, d, rc):
+    rc.light_bulbradius = torch.full_like(rc.light_bulbradius, 0.05)
+
+  gpu, ref, (_, _, rc) = _render_both(mjm, 2, res, setup=soft, envmap=scenes.constant_sky(), samples_per_pixel=2, shadow_samples=4)
+  assert rc.base.nrender == ncam
   compare_rt(gpu, ref)
   seg = gpu["seg"][..., 0]
   assert len(np.unique(seg)) >= 5  # plane, three primitives, background
+  w, h = res[-1]
+  img = torch.zeros((2, h, w, 3), dtype=torch.float32, device="cuda")
+  mj.get_rgb(rc, ncam - 1, img)
+  adr = int(rc.base.rgb_adr[ncam - 1])
+  packed = gpu["rgb"][:, adr : adr + w * h].astype(np.int64).reshape(2, h, w)
+  want = np.stack([(packed >> 16) & 255, (packed >> 8) & 255, packed & 255], axis=-1)
+  np.testing.assert_array_equal(np.rint(img.cpu().numpy() * 255.0).astype(np.int64), want)
+
+
+def test_many_cameras_path_traced_with_atrous_match_oracle(cuda):
+  """40 cameras on the path-traced tier with a-trous denoising: both launches' samples and the
+  per-launch filter follow the oracle."""
+  mjm = many_cameras_scene(40)
+  res = [(8 + 8 * (i % 2), 8 + 4 * (i % 3 == 0)) for i in range(40)]
+  m, d, rc = make(mjm, nworld=2, device="cuda", cam_res=res, envmap=scenes.constant_sky(), tier=mj.Tier.PATH, path_samples=2, max_depth=3,
+                  denoise=True, render_hdr=True)
+  mj.refit_bvh(m, d, rc)
+  mj.render(m, d, rc, seed=3)
+  gpu = gpu_outputs(rc)
+  ref = oracle.pt_render(m, d, rc, seed=3, threads=8)
+  np.testing.assert_array_equal(gpu["seg"], ref["seg"])
+  np.testing.assert_allclose(gpu["depth"], ref["depth"], rtol=1e-5, atol=1e-7)
+  diff = np.abs(gpu["hdr"] - ref["hdr"]).max(axis=-1)
+  assert np.mean(diff <= 1e-3) >= 0.999, f"{np.mean(diff > 1e-3):.4%} of pixels differ, max {diff.max()}"
