@@ -8,6 +8,7 @@ current torch stream of the data's device, so render() performs no host synchron
 from __future__ import annotations
 
 import ctypes
+import functools
 import warnings
 
 import numpy as np
@@ -89,7 +90,9 @@ def camera_chunks(rc: PbrRenderContext) -> list[tuple[int, int]]:
   return [(a, min(a + step, n)) for a in range(0, n, step)]
 
 
+@functools.lru_cache(maxsize=None)
 def _max_cams() -> int:
+  """MJPBR_MAX_CAMS of include/mjpbr.h (read once: render() runs on the hot path)."""
   return _abi._defines(_abi._header_text())["MJPBR_MAX_CAMS"]
 
 

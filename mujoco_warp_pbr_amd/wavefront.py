@@ -18,6 +18,7 @@ import dataclasses
 import torch
 
 from mujoco_warp_pbr_amd import _abi
+from mujoco_warp_pbr_amd.render import _max_cams
 from mujoco_warp_pbr_amd.render import outputs_struct
 from mujoco_warp_pbr_amd.render import scene_struct
 from mujoco_warp_pbr_amd.render import settings_struct
@@ -63,7 +64,7 @@ class WavefrontBatch:
         raise ValueError(f"group {i}: denoising is done by render(); create the context with denoise=False")
       if d.nworld != rc.base.nworld:
         raise ValueError(f"group {i}: data has {d.nworld} worlds, the context {rc.base.nworld}")
-      if int(rc.base.nrender) > _abi._defines(_abi._header_text())["MJPBR_MAX_CAMS"]:
+      if int(rc.base.nrender) > _max_cams():
         raise ValueError(f"group {i}: a wavefront group renders at most MJPBR_MAX_CAMS cameras, got {int(rc.base.nrender)} (use render())")
     s0 = bytes(settings_struct(groups[0][2]))
     for i, (_, _, rc) in enumerate(groups[1:], 1):
