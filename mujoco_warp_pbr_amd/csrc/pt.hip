@@ -12,12 +12,14 @@
 
 namespace mjpbr {
 
-// 8 waves per SIMD with spills beat 2 without: the path loop is latency bound (measured 1.8x on
-// the full-feature scene at 1024 x 128^2, 1 sample).
+// W waves per SIMD. With meshes, 8 with spills beat fewer without: the path loop is latency bound
+// (measured 1.8x over 2 waves on the full-feature scene at 1024 x 128^2, 1 sample; 8 waves 103.9
+// ms against 4 waves 109.1 at 4 samples). Without meshes 4 waves win (the reference suite's
+// scenes at 16 samples, depth 4: material_grid 115.3 -> 94.5 ms, primitives 153.8 -> 118.4 ms).
 // kGlobal: the world's records are read in place from geo.world_stage (too large for the LDS).
 // kCount: adds the rays traced to out.ray_counts (the same image).
-template <bool kGlobal, bool kCount>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8)))
+template <bool kGlobal, bool kCount, int W>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W, W)))
 pt_kernel(MjpbrScene sc, MjpbrSettings s, MjpbrOutputs out, int seed, TileMap tm) {
   extern __shared__ float lds[];
   int b = logical_block(tm);
@@ -126,7 +128,11 @@ extern "C" int mjpbr_pt_render(const MjpbrScene* scene, const MjpbrSettings* s, 
   if ((rc = launch_stage_prep(scene, (hipStream_t)stream))) return rc;
   using PtFn = void (*)(MjpbrScene, MjpbrSettings, MjpbrOutputs, int, TileMap);
   const bool count = out->ray_counts != nullptr;
-  PtFn fn = global ? (count ? pt_kernel<true, true> : pt_kernel<true, false>) : (count ? pt_kernel<false, true> : pt_kernel<false, false>);
+  PtFn fn;
+  if (s->has_meshes)
+    fn = global ? (count ? pt_kernel<true, true, 8> : pt_kernel<true, false, 8>) : (count ? pt_kernel<false, true, 8> : pt_kernel<false, false, 8>);
+  else
+    fn = global ? (count ? pt_kernel<true, true, 4> : pt_kernel<true, false, 4>) : (count ? pt_kernel<false, true, 4> : pt_kernel<false, false, 4>);
   hipLaunchKernelGGL(fn, dim3(nlaunch), dim3(256), lds, (hipStream_t)stream, *scene, *s, *out, seed, tm);
   return check_launch("pt_kernel");
 }
