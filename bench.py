@@ -341,22 +341,34 @@ def pt_suite_section(device, nworld: int = 1024, res: int = 128, reps: int = 3) 
                                   envmap=scenes.procedural_sky(3, 512, 1024), tier=mj.Tier.PATH, path_samples=16, denoise=True,
                                   device=device)
     mj.refit_bvh(m, d, rc)
-    mj.render(m, d, rc)
-    torch.cuda.synchronize()
-    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    a.record(stream)
-    for _ in range(reps):
+
+    def timed():
       mj.render(m, d, rc)
-    b.record(stream)
-    b.synchronize()
-    ms = a.elapsed_time(b) / reps
+      torch.cuda.synchronize()
+      a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+      a.record(stream)
+      for _ in range(reps):
+        mj.render(m, d, rc)
+      b.record(stream)
+      b.synchronize()
+      return a.elapsed_time(b) / reps
+
+    rc.path_kernel = "megakernel"  # the per-pixel kernel (the reference's scheme), for comparison
+    mega_ms = timed()
+    mega_rgb = rc.base.rgb_data.clone()
+    rc.path_kernel = "wavefront"  # render()'s default
+    ms = timed()
+    identical = bool(torch.equal(mega_rgb, rc.base.rgb_data))
+    del mega_rgb
     mpx = nworld * res * res / ms / 1e3
     rays = mj.count_path_rays(m, d, rc)  # one untimed render through the counting kernel instance
     pub = PT_SUITE_PUBLISHED_MPX[name]
-    log(f"suite PT {name}: {ms:.1f} ms = {mpx:.1f} Mpx/s, {rays['total'] / ms / 1e3:.0f} Mrays/s "
-        f"(reference RTX PRO 6000 {pub['rtx_pro_6000']:.1f} Mpx/s)")
+    log(f"suite PT {name}: {ms:.1f} ms = {mpx:.1f} Mpx/s (megakernel {mega_ms:.1f} ms, identical: {identical}), "
+        f"{rays['total'] / ms / 1e3:.0f} Mrays/s (reference RTX PRO 6000 {pub['rtx_pro_6000']:.1f} Mpx/s)")
     out[name] = {
       "ms": round(ms, 3),
+      "megakernel_ms": round(mega_ms, 3),
+      "identical_to_megakernel": identical,
       "Mpixels_s": round(mpx, 2),
       "Mpaths_s": round(mpx * 16, 1),
       "rays_per_render": rays,
@@ -367,7 +379,8 @@ def pt_suite_section(device, nworld: int = 1024, res: int = 128, reps: int = 3) 
     }
     del rc, m, d
     torch.cuda.empty_cache()
-  out["workload"] = f"reference suite scenes, path-traced tier, {nworld} worlds x {res}^2, 16 spp, depth 4, a-trous (3 iterations), rgb"
+  out["workload"] = (f"reference suite scenes, path-traced tier, {nworld} worlds x {res}^2, 16 spp, depth 4, a-trous (3 iterations), rgb; "
+                     "render() through the wavefront kernel, the per-pixel megakernel timed beside it")
   return out
 
 
@@ -378,8 +391,8 @@ def pt_wavefront_section(device) -> dict:
   """Batched path tracing: 4096 worlds of the full-feature scene, 2048 at 64x64, 1536 at 128x128
   and 512 at 256x256, 4 samples, depth 8, Russian roulette from 3, per-world materials, cameras,
   sky rotation and light intensities. The persistent wavefront tracer (one launch over the three
-  groups, autotuned shape, graph replay) against the per-pixel megakernel (one launch a group);
-  images must be identical."""
+  groups, autotuned shape, graph replay) against the per-pixel megakernel (one render() a group,
+  the contexts created with path_kernel="megakernel"); images must be identical."""
   import numpy as np
   import torch
 
@@ -395,7 +408,7 @@ def pt_wavefront_section(device) -> dict:
     d = mj.put_data(mjm, nworld, device)
     rc = mj.create_render_context(
       mjm, nworld=nworld, cam_res=(res, res), envmap=scenes.procedural_sky(0, 256, 512), tier=mj.Tier.PATH, path_samples=4,
-      max_depth=8, rr_depth=3, ev100=math.log2(1.0 / 1.2), device=device,
+      max_depth=8, rr_depth=3, ev100=math.log2(1.0 / 1.2), device=device, path_kernel="megakernel",
     )
     scenes.randomize_worlds(m, rc, nworld, seed=10 * i + 1)
     scenes.jitter_cameras(d, seed=10 * i + 2)

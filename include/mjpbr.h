@@ -324,6 +324,7 @@ typedef struct MjpbrWavefrontPlan {
   int64_t npixels;  /* pixels over every world of every group */
   int64_t nstaged;  /* worlds of the groups whose records exceed the LDS (staged into their
                        geo.world_stage by each launch and read from there) */
+  int64_t nregions; /* per-workgroup sample regions the workspace holds: the launch's grid at most */
   int32_t lds_bytes;
   int32_t ngroup;
   MjpbrSettings settings;
@@ -383,14 +384,18 @@ int mjpbr_pt_render(const MjpbrScene* scene, const MjpbrSettings* s, MjpbrOutput
                     void* stream);
 
 /* Persistent wavefront path tracer over up to MJPBR_WF_MAX_GROUPS world groups, each with its own cameras, world
- * count and outputs (mixed resolutions in one launch); images bit-identical to mjpbr_pt_render
- * of each group with the same seed (no counterpart in the reference, whose path tracer is the
- * per-pixel megakernel of pathtrace.py:340-438). Workspace bytes: mjpbr_pt_wavefront_workspace
- * (a descriptor header, the work counter, one vec3 slot per path sample). prepare validates,
- * writes the descriptors (pointers of scenes[g] and outs[g] as they are now) into the workspace
- * and synchronises `stream`; launch enqueues counter reset, the persistent kernel and the
- * per-pixel resolve on `stream` without allocating or synchronising (graph-capturable). */
-int64_t mjpbr_pt_wavefront_workspace(const MjpbrScene* scenes, int ngroup, const MjpbrSettings* s);
+ * count and outputs (mixed resolutions in one launch); images, and with s->denoise the noisy HDR
+ * and first-hit features, bit-identical to mjpbr_pt_render of each group with the same seed (no
+ * counterpart in the reference, whose path tracer is the per-pixel megakernel of
+ * pathtrace.py:340-438). Workspace: a descriptor header, the work counter and one sample region
+ * per persistent workgroup (a 16x16 tile's path samples: 3 floats each, 12 with s->denoise);
+ * mjpbr_pt_wavefront_workspace gives the bytes for launches of up to `groups_per_cu` (1..16)
+ * workgroups per compute unit, and prepare accepts any workspace holding at least one region (a
+ * launch's grid is capped at the regions the workspace holds). prepare validates, writes the descriptors (pointers of scenes[g] and outs[g] as they
+ * are now) into the workspace and synchronises `stream`; launch enqueues the counter reset, the
+ * staging of oversized worlds and the persistent kernel (which resolves each tile itself) on
+ * `stream` without allocating or synchronising (graph-capturable). */
+int64_t mjpbr_pt_wavefront_workspace(const MjpbrScene* scenes, int ngroup, const MjpbrSettings* s, int groups_per_cu);
 int mjpbr_pt_wavefront_prepare(const MjpbrScene* scenes, const MjpbrOutputs* outs, int ngroup, const MjpbrSettings* s,
                                void* workspace, int64_t workspace_bytes, MjpbrWavefrontPlan* plan, void* stream);
 int mjpbr_pt_wavefront_launch(void* workspace, const MjpbrWavefrontPlan* plan, const MjpbrWavefrontConfig* cfg, int32_t seed,

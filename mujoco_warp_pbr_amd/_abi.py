@@ -83,7 +83,7 @@ FUNCTIONS.update(
     "mjpbr_radiance_mips": (_I, [_VP, _I, _I, _VP, _VP, _I, _VP, _L, _VP]),
     "mjpbr_env_sh9": (_I, [_VP, _I, _I, _VP, _L, _VP, _VP]),
     "mjpbr_env_sampling_tables": (_I, [_VP, _I, _I, _I, _VP, _L, _VP, _VP, _VP, _VP]),
-    "mjpbr_pt_wavefront_workspace": (_L, [_VP, _I, _VP]),
+    "mjpbr_pt_wavefront_workspace": (_L, [_VP, _I, _VP, _I]),
     "mjpbr_pt_wavefront_prepare": (_I, [_VP, _VP, _I, _VP, _VP, _L, _VP, _VP]),
     "mjpbr_pt_wavefront_launch": (_I, [_VP, _VP, _VP, ctypes.c_int32, _VP]),
   }

@@ -11,15 +11,17 @@
 // (misses, Russian roulette, depth limit): after each segment a wave ballots its finished lanes
 // and, once at least `refill` lanes are free, claims that many new paths with one LDS atomic and
 // hands them to the free lanes by their prefix rank (mbcnt). Lanes stay busy with live paths
-// instead of idling until the wave's longest path ends. A path's radiance goes to its sample slot;
-// a resolve pass averages each pixel's samples in sample order and encodes the display colour,
-// so the images are bit-identical to the megakernel's.
+// instead of idling until the wave's longest path ends. A path's radiance (and with the denoiser
+// its first-hit features) goes to its sample slot in the workgroup's own region; once every path
+// of the tile has ended, the workgroup averages each pixel's samples in sample order and writes
+// the display colour (or the noisy HDR and features), so the results are bit-identical to the
+// megakernel's.
 //
 // Every launch is asynchronous and allocation-free on the caller's stream (the work counter is
 // cleared by a kernel, not a memset), so the launches (counter reset, the staging of oversized
-// worlds when a group has them, the persistent kernel, the resolve) can be captured in a graph.
-// A batch holds up to MJPBR_WF_MAX_GROUPS groups; items and pixels find their group by a binary
-// search over the groups' first indices.
+// worlds when a group has them, the persistent kernel) can be captured in a graph. A batch holds
+// up to MJPBR_WF_MAX_GROUPS groups; an item finds its group by a binary search over the groups'
+// first items.
 #pragma clang fp contract(off)
 #include "abi_util.hpp"
 #include "path.hpp"
@@ -34,7 +36,6 @@ struct WfGroup {
   MjpbrOutputs out;
   TileMap tm;
   int64_t item_base;   // first work item of the group
-  int64_t pixel_base;  // first resolve index (worlds x total_rays before this group)
   int64_t stage_base;  // global groups: first index of their worlds in wf_stage_kernel's grid
   int32_t global;      // 1: the world's records are read in place from sc.geo.world_stage
 };
@@ -67,14 +68,13 @@ MJ_DEV int lanes_below(uint64_t mask) {
   return (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
 }
 
-// The last group whose first work item (by_pixel: first resolve index) is at most `index`: a
-// binary search over the groups' non-decreasing bases.
-MJ_DEV int find_group(const WfHeader* h, int64_t index, bool by_pixel) {
+// The last group whose first work item is at most `item`: a binary search over the groups'
+// non-decreasing item bases.
+MJ_DEV int find_group(const WfHeader* h, int64_t item) {
   int lo = 0, hi = h->ngroup - 1;
   while (lo < hi) {
     const int mid = (lo + hi + 1) >> 1;
-    const int64_t base = by_pixel ? h->groups[mid].pixel_base : h->groups[mid].item_base;
-    if (index >= base) lo = mid;
+    if (item >= h->groups[mid].item_base) lo = mid;
     else hi = mid - 1;
   }
   return lo;
@@ -84,9 +84,66 @@ __global__ void wf_reset_kernel(int* counter) {
   if (threadIdx.x == 0) counter[0] = 0;
 }
 
+// Floats a path sample keeps until its tile is resolved: the radiance, and with the denoiser also
+// the first hit's albedo, normal and emission (the per-pixel kernel's per-sample sums).
+MJ_HOST_DEV inline int slot_floats(const MjpbrSettings& s) { return s.denoise ? 12 : 3; }
+
+MJ_HOST_DEV inline int64_t region_floats(const MjpbrSettings& s) { return (int64_t)kTilePixels * s.path_samples * slot_floats(s); }
+
+// One pixel of a finished tile from the workgroup's sample region (slot lp * spp + sample): the
+// per-pixel kernel's arithmetic after its sample loop (pathtrace.py:340-438, pt.hip), sums in
+// sample order; the display colour and HDR, or with the denoiser the noisy HDR and the features.
+MJ_DEV void resolve_pixel(const WfGroup& G, const MjpbrSettings& s, int w, int cidx, int tx, int ty, const float* region) {
+  const int lp = threadIdx.x;
+  const int blk = lp >> 6, ln = lp & 63;
+  const int px = tx * kTile + (blk & 1) * 8 + (ln & 7), py = ty * kTile + (blk >> 1) * 8 + (ln >> 3);
+  Pixel pix;
+  if (!setup_pixel(G.sc.cams, G.out, s, w, cidx, px, py, G.tm.ray_offset[cidx], &pix) || !pix.want_rgb) return;
+  const int spp = s.path_samples, sf = slot_floats(s);
+  const float exposure = G.sc.env.exposure[w % G.sc.env.exposure_n];
+  const MjpbrOutputs& out = G.out;
+  const int64_t adr = out.rgb_adr[cidx] + pix.pixelid;
+  const float* sl = region + (int64_t)lp * spp * sf;
+  V3 hdr = v3s(0.0f);
+  const float fn = (float)spp;
+  if (!s.denoise) {
+    for (int k = 0; k < spp; ++k) hdr += ld3(sl + 3 * k);
+    hdr = hdr / fn;
+    if (s.render_hdr) st3(out.hdr + 3 * ((int64_t)w * out.hdr_stride + adr), hdr);
+    out.rgb[(int64_t)w * out.rgb_stride + adr] = encode_display(display_transform(hdr * exposure, s.tone_mapper));
+    return;
+  }
+  V3 albedo_sum = v3s(0.0f), normal_sum = v3s(0.0f), emission_sum = v3s(0.0f);
+  float lum_sq = 0.0f;
+  for (int k = 0; k < spp; ++k, sl += 12) {
+    const V3 radiance = ld3(sl);
+    hdr += radiance;
+    albedo_sum += ld3(sl + 3);
+    normal_sum += ld3(sl + 6);
+    emission_sum += ld3(sl + 9);
+    const float lum = luminance(radiance) * exposure;
+    lum_sq += lum * lum;
+  }
+  hdr = hdr / fn;
+  const int64_t a = (int64_t)w * out.aov_stride + adr;
+  st3(out.hdr + 3 * ((int64_t)w * out.hdr_stride + adr), hdr);
+  const V3 alb = albedo_sum / fn;
+  out.aov_albedo[3 * a] = float_to_half(alb.x);
+  out.aov_albedo[3 * a + 1] = float_to_half(alb.y);
+  out.aov_albedo[3 * a + 2] = float_to_half(alb.z);
+  st3(out.aov_emission + 3 * a, emission_sum * (exposure / fn));
+  if (len_sq(normal_sum) > 0.0f) normal_sum = normalize(normal_sum);
+  out.aov_normal[3 * a] = float_to_half(normal_sum.x);
+  out.aov_normal[3 * a + 1] = float_to_half(normal_sum.y);
+  out.aov_normal[3 * a + 2] = float_to_half(normal_sum.z);
+  const float mean_lum = luminance(hdr) * exposure;
+  out.aov_variance[a] = fmaxf(lum_sq / fn - mean_lum * mean_lum, 0.0f) / fn;
+}
+
 // kCount: adds the rays traced to ray_counts (the same image). kGlobal: some group reads its
 // worlds' records in place from world_stage (a per-item branch; 3 % slower, so only batches with
-// such a group take this instance).
+// such a group take this instance). Workgroup b keeps its tile's samples in region b of `slots`
+// and resolves the tile at the top of the next item iteration, after the barrier that ends it.
 template <int kWavesPerEu, bool kCount, bool kGlobal>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kWavesPerEu, kWavesPerEu)))
 wf_kernel(const WfHeader* __restrict__ hdr, int* counter, float* slots, MjpbrSettings s, int seed, int refill, int64_t item_limit,
@@ -98,17 +155,21 @@ wf_kernel(const WfHeader* __restrict__ hdr, int* counter, float* slots, MjpbrSet
   const int64_t nitems = item_limit > 0 && item_limit < hdr->nitems ? item_limit : hdr->nitems;
   const int spp = s.path_samples;
   const int pool = kTilePixels * spp;
+  const int sf = slot_floats(s);
+  float* region = slots + (int64_t)blockIdx.x * region_floats(s);
   RayCount rays{0u, 0u, 0u};
+  int done_g = -1, done_w = 0, done_cidx = 0, done_tx = 0, done_ty = 0;  // the tile to resolve
   for (;;) {
-    __syncthreads();  // every wave is done with the previous item's LDS and pool
+    __syncthreads();  // every wave is done with the previous item's LDS, pool and sample slots
+    if (done_g >= 0) resolve_pixel(hdr->groups[done_g], s, done_w, done_cidx, done_tx, done_ty, region);
     if (threadIdx.x == 0) {
       s_item = atomicAdd(counter, 1);
       s_pool = 0;
     }
-    __syncthreads();
+    __syncthreads();  // (also: the resolve has read every slot before new paths overwrite them)
     const int item = __builtin_amdgcn_readfirstlane(s_item);
     if (item >= nitems) break;
-    const int g = __builtin_amdgcn_readfirstlane(find_group(hdr, item, false));
+    const int g = __builtin_amdgcn_readfirstlane(find_group(hdr, item));
     const WfGroup& G = hdr->groups[g];
     const MjpbrScene& sc = G.sc;
     const int64_t local = item - G.item_base;
@@ -118,6 +179,11 @@ wf_kernel(const WfHeader* __restrict__ hdr, int* counter, float* slots, MjpbrSet
     while (cidx + 1 < G.tm.nrender && t >= G.tm.tile_base[cidx + 1]) ++cidx;
     const int tl = t - G.tm.tile_base[cidx];
     const int tx = tl % G.tm.tiles_x[cidx], ty = tl / G.tm.tiles_x[cidx];
+    done_g = g;
+    done_w = w;
+    done_cidx = cidx;
+    done_tx = tx;
+    done_ty = ty;
     // (G is the same for the whole workgroup, so both branches keep their barriers uniform)
     SceneView sv = kGlobal && G.global ? global_world(sc, w, lds, G.tm.mesh_cache_nodes, G.tm.mesh_cache_n)
                                        : stage_world(sc, w, lds, G.tm.mesh_cache_nodes, G.tm.mesh_cache_n);
@@ -128,12 +194,11 @@ wf_kernel(const WfHeader* __restrict__ hdr, int* counter, float* slots, MjpbrSet
     const float* xm = sc.cams.cam_xmat + ((int64_t)w * sc.cams.ncam + sc.cams.cam_id_map[cidx]) * 9;
     const V3 head_dir = -V3{xm[2], xm[5], xm[8]};
     const bool want_depth = G.out.render_depth[cidx] != 0, want_seg = G.out.render_seg[cidx] != 0;
-    const int64_t world_slot = ((int64_t)G.pixel_base + (int64_t)w * total_rays + ray_offset) * spp;
 
     PathState st;
     Primary first;
     bool alive = false, pool_done = false;
-    int smp = 0, pseed = 0, pixelid = 0;
+    int smp = 0, pseed = 0, pixelid = 0, slot = 0;
     float dcz = 0.0f;
     for (;;) {
       const uint64_t dead = __ballot(!alive);
@@ -168,6 +233,7 @@ wf_kernel(const WfHeader* __restrict__ hdr, int* counter, float* slots, MjpbrSet
                 path_begin(&st, &first, o, d, md);
                 smp = sample;
                 pixelid = pix.pixelid;
+                slot = lp * spp + sample;
                 dcz = dc.z;
                 alive = true;
               }
@@ -178,49 +244,31 @@ wf_kernel(const WfHeader* __restrict__ hdr, int* counter, float* slots, MjpbrSet
       if (ndead == 64 && pool_done) break;
       if (alive && !path_step(c, &st, head_dir, pseed, smp, &first)) {
         alive = false;
-        st3(slots + 3 * (world_slot + (int64_t)pixelid * spp + smp), st.radiance);
+        float* sl = region + (int64_t)slot * sf;
+        st3(sl, st.radiance);
+        if (s.denoise) {
+          st3(sl + 3, first.albedo);
+          st3(sl + 6, first.normal);
+          st3(sl + 9, first.emission);
+        }
         if (smp == 0) {
+          const float depth = first.hit.geom_id != -1 ? first.hit.dist * -dcz : 0.0f;
           if (want_seg) {
             int2 seg = first.hit.geom_id != -1 ? make_int2(first.hit.geom_id, 5) : make_int2(-1, -1);
             reinterpret_cast<int2*>(G.out.seg)[(int64_t)w * G.out.seg_stride + G.out.seg_adr[cidx] + pixelid] = seg;
           }
-          if (want_depth) {
-            float depth = first.hit.geom_id != -1 ? first.hit.dist * -dcz : 0.0f;
-            G.out.depth[(int64_t)w * G.out.depth_stride + G.out.depth_adr[cidx] + pixelid] = depth;
-          }
+          if (want_depth) G.out.depth[(int64_t)w * G.out.depth_stride + G.out.depth_adr[cidx] + pixelid] = depth;
           if (s.render_guides) {
             int64_t a = (int64_t)w * G.out.guide_stride + G.out.rgb_adr[cidx] + pixelid;
             st3(G.out.albedo + 3 * a, first.base_color);
             st3(G.out.normal + 3 * a, first.normal);
           }
+          if (s.denoise) G.out.aov_depth[(int64_t)w * G.out.aov_stride + G.out.rgb_adr[cidx] + pixelid] = depth;
         }
       }
     }
   }
   if (kCount) flush_ray_counts(ray_counts, rays);  // the item loop's exit is uniform: every lane is here
-}
-
-// Mean of each pixel's samples in sample order, HDR and display colour (pathtrace.py:420-438).
-__global__ void wf_resolve_kernel(const WfHeader* __restrict__ hdr, const float* slots, MjpbrSettings s) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= hdr->npixels) return;
-  const WfGroup& G = hdr->groups[find_group(hdr, i, true)];
-  const int64_t local = i - G.pixel_base;
-  const int total_rays = G.sc.cams.total_rays;
-  const int w = (int)(local / total_rays);
-  const int r = (int)(local - (int64_t)w * total_rays);
-  int cidx = 0;
-  while (cidx + 1 < G.tm.nrender && r >= G.tm.ray_offset[cidx + 1]) ++cidx;
-  if (!(s.any_rgb && G.out.render_rgb[cidx])) return;
-  const int pixelid = r - G.tm.ray_offset[cidx];
-  const int spp = s.path_samples;
-  V3 sum = v3s(0.0f);
-  for (int k = 0; k < spp; ++k) sum += ld3(slots + 3 * (i * spp + k));
-  V3 h = sum / (float)spp;
-  const float exposure = G.sc.env.exposure[w % G.sc.env.exposure_n];
-  const int64_t adr = G.out.rgb_adr[cidx] + pixelid;
-  if (s.render_hdr) st3(G.out.hdr + 3 * ((int64_t)w * G.out.hdr_stride + adr), h);
-  G.out.rgb[(int64_t)w * G.out.rgb_stride + adr] = encode_display(display_transform(h * exposure, s.tone_mapper));
 }
 
 int device_cus() {
@@ -235,20 +283,31 @@ int device_cus() {
   return cached[dev];
 }
 
-int64_t slot_count(const MjpbrScene* scenes, int ngroup, int spp) {
-  int64_t n = 0;
-  for (int g = 0; g < ngroup; ++g) n += (int64_t)scenes[g].geo.nworld * scenes[g].cams.total_rays;
-  return n * spp;
+constexpr int kMaxGroupsPerCu = 16;
+
+// Work items (16x16 tiles) of the batch, or a negative code when a group's cameras are invalid.
+int64_t count_items(const MjpbrScene* scenes, int ngroup) {
+  int64_t items = 0;
+  for (int g = 0; g < ngroup; ++g) {
+    TileMap tm;
+    int nlaunch;
+    if (make_tile_map(&scenes[g], &tm, &nlaunch)) return -1;
+    items += (int64_t)tm.tiles_per_world * scenes[g].geo.nworld;
+  }
+  return items;
 }
 
 }  // namespace mjpbr
 
 using namespace mjpbr;
 
-extern "C" int64_t mjpbr_pt_wavefront_workspace(const MjpbrScene* scenes, int ngroup, const MjpbrSettings* s) {
-  if (ngroup < 1 || ngroup > kWfMaxGroups || s->path_samples < 1) return -1;
-  size_t slots = (size_t)slot_count(scenes, ngroup, s->path_samples) * 12;
-  return (int64_t)(kHeaderBytes + kCounterBytes + (slots + 255) / 256 * 256);
+extern "C" int64_t mjpbr_pt_wavefront_workspace(const MjpbrScene* scenes, int ngroup, const MjpbrSettings* s, int groups_per_cu) {
+  if (ngroup < 1 || ngroup > kWfMaxGroups || s->path_samples < 1 || groups_per_cu < 1 || groups_per_cu > kMaxGroupsPerCu) return -1;
+  const int64_t items = count_items(scenes, ngroup);
+  if (items < 1) return -1;
+  int64_t regions = (int64_t)device_cus() * groups_per_cu;
+  if (regions > items) regions = items;
+  return (int64_t)(kHeaderBytes + kCounterBytes) + regions * region_floats(*s) * (int64_t)sizeof(float);
 }
 
 extern "C" int mjpbr_pt_wavefront_prepare(const MjpbrScene* scenes, const MjpbrOutputs* outs, int ngroup, const MjpbrSettings* s,
@@ -257,10 +316,11 @@ extern "C" int mjpbr_pt_wavefront_prepare(const MjpbrScene* scenes, const MjpbrO
   if (s->path_samples < 1) return set_error(MJPBR_EINVAL, "path_samples must be at least 1");
   if (s->max_depth < 0 || s->max_depth + 4 * s->max_layers >= kJitterDepth)
     return set_error(MJPBR_EINVAL, "max_depth + 4 * max_layers must be below %d", kJitterDepth);
-  if (s->denoise) return set_error(MJPBR_EINVAL, "wavefront: the denoiser features are written by the per-pixel kernel only");
-  int64_t need = mjpbr_pt_wavefront_workspace(scenes, ngroup, s);
-  if (!workspace || workspace_bytes < need)
-    return set_error(MJPBR_EINVAL, "wavefront: workspace of %lld bytes, need %lld", (long long)workspace_bytes, (long long)need);
+  const int64_t region_bytes = region_floats(*s) * (int64_t)sizeof(float);
+  const int64_t fixed = (int64_t)(kHeaderBytes + kCounterBytes);
+  if (!workspace || workspace_bytes < fixed + region_bytes)
+    return set_error(MJPBR_EINVAL, "wavefront: workspace of %lld bytes, need at least %lld (one sample region)", (long long)workspace_bytes,
+                     (long long)(fixed + region_bytes));
   WfHeader* h = new WfHeader();
   h->ngroup = ngroup;
   h->spp = s->path_samples;
@@ -290,7 +350,6 @@ extern "C" int mjpbr_pt_wavefront_prepare(const MjpbrScene* scenes, const MjpbrO
     b += plan_mesh_cache(&scenes[g], s->has_meshes != 0, b, &G.tm);
     lds = b > lds ? b : lds;
     G.item_base = items;
-    G.pixel_base = pixels;
     items += (int64_t)G.tm.tiles_per_world * scenes[g].geo.nworld;
     pixels += (int64_t)scenes[g].geo.nworld * scenes[g].cams.total_rays;
   }
@@ -308,6 +367,7 @@ extern "C" int mjpbr_pt_wavefront_prepare(const MjpbrScene* scenes, const MjpbrO
   plan->nitems = items;
   plan->npixels = pixels;
   plan->nstaged = staged;
+  plan->nregions = (workspace_bytes - fixed) / region_bytes;
   plan->lds_bytes = (int32_t)lds;
   plan->ngroup = ngroup;
   plan->settings = *s;
@@ -316,11 +376,11 @@ extern "C" int mjpbr_pt_wavefront_prepare(const MjpbrScene* scenes, const MjpbrO
 
 extern "C" int mjpbr_pt_wavefront_launch(void* workspace, const MjpbrWavefrontPlan* plan, const MjpbrWavefrontConfig* cfg, int32_t seed,
                                          void* stream) {
-  if (!workspace || plan->nitems < 1) return set_error(MJPBR_EINVAL, "wavefront: not prepared");
+  if (!workspace || plan->nitems < 1 || plan->nregions < 1) return set_error(MJPBR_EINVAL, "wavefront: not prepared");
   int refill = cfg->refill_threshold;
   if (refill < 1 || refill > 64) return set_error(MJPBR_EINVAL, "wavefront: refill_threshold must be in [1, 64], got %d", refill);
-  if (cfg->groups_per_cu < 1 || cfg->groups_per_cu > 16)
-    return set_error(MJPBR_EINVAL, "wavefront: groups_per_cu must be in [1, 16], got %d", cfg->groups_per_cu);
+  if (cfg->groups_per_cu < 1 || cfg->groups_per_cu > kMaxGroupsPerCu)
+    return set_error(MJPBR_EINVAL, "wavefront: groups_per_cu must be in [1, %d], got %d", kMaxGroupsPerCu, cfg->groups_per_cu);
   const WfHeader* hdr = reinterpret_cast<const WfHeader*>(workspace);
   int* counter = reinterpret_cast<int*>(reinterpret_cast<char*>(workspace) + kHeaderBytes);
   float* slots = reinterpret_cast<float*>(reinterpret_cast<char*>(workspace) + kHeaderBytes + kCounterBytes);
@@ -335,8 +395,10 @@ extern "C" int mjpbr_pt_wavefront_launch(void* workspace, const MjpbrWavefrontPl
     if (rc) return rc;
   }
   int64_t items = cfg->item_limit > 0 && cfg->item_limit < plan->nitems ? cfg->item_limit : plan->nitems;
+  // workgroup b owns sample region b: the grid never exceeds the regions the workspace holds
   int64_t grid = (int64_t)device_cus() * cfg->groups_per_cu;
   if (grid > items) grid = items;
+  if (grid > plan->nregions) grid = plan->nregions;
   const MjpbrSettings& s = plan->settings;
   using WfFn = void (*)(const WfHeader*, int*, float*, MjpbrSettings, int, int, int64_t, uint64_t*);
   const bool count = cfg->ray_counts != nullptr;
@@ -356,8 +418,5 @@ extern "C" int mjpbr_pt_wavefront_launch(void* workspace, const MjpbrWavefrontPl
   }
   hipLaunchKernelGGL(fn, dim3((unsigned)grid), dim3(256), plan->lds_bytes, st, hdr, counter, slots, s, seed, refill, (int64_t)cfg->item_limit,
                      cfg->ray_counts);
-  rc = check_launch("wf_kernel");
-  if (rc) return rc;
-  hipLaunchKernelGGL(wf_resolve_kernel, dim3((unsigned)((plan->npixels + 255) / 256)), dim3(256), 0, st, hdr, slots, s);
-  return check_launch("wf_resolve_kernel");
+  return check_launch("wf_kernel");
 }

@@ -61,6 +61,7 @@ _ARGUMENT_RULES = (
   ("denoise_iterations",) + _at_least(0),
   ("shadow_light_fraction", lambda v: 0.0 <= v <= 1.0, "must be in [0, 1], got {v}"),
   ("denoiser", lambda v: v in ("atrous", "learned"), 'must be "atrous" or "learned", got {v!r}'),
+  ("path_kernel", lambda v: v in ("wavefront", "megakernel"), 'must be "wavefront" or "megakernel", got {v!r}'),
   ("mesh_bvh_frame", lambda v: v in _MESH_BVH_FRAMES, 'must be "auto" or "mujoco", got {v!r}'),
   ("mesh_bvh_constructor", lambda v: v in _MESH_BVH_CONSTRUCTORS, f"must be one of {_MESH_BVH_CONSTRUCTORS}, got {{v!r}}"),
   ("traversal", lambda v: v in _TRAVERSALS, 'must be "software", "optix" or "auto", got {v!r}'),
@@ -182,6 +183,7 @@ def create_render_context(
   precompute: bool = True,
   denoiser: str = "atrous",
   denoiser_weights: str | None = None,
+  path_kernel: str = "wavefront",
 ) -> PbrRenderContext:
   """Creates a PBR render context on `device` (reference io.py:331-634; same arguments).
 
@@ -198,6 +200,11 @@ def create_render_context(
   a-trous filter, or the learned U-Net (weights `denoiser_weights`, default the packaged ones)
   with the a-trous filter as fallback for worlds outside its training range. The learned
   denoiser needs camera sides that are multiples of 64.
+
+  `path_kernel` picks the path tracer's kernel: "wavefront" (a persistent kernel that refills
+  lanes as paths end, the default) or "megakernel" (one thread per pixel, the reference's
+  scheme). Both give identical images; contexts with more cameras than one launch takes use the
+  megakernel.
 
   `precompute=False` leaves the GPU-built tables (DFG LUT, texture and radiance mip chains,
   prefiltered environment stacks, SH9 coefficients, sampling tables) zero; it exists so host-side
@@ -421,6 +428,7 @@ def create_render_context(
     traversal=traversal,
     denoiser=denoiser,
     denoiser_weights=denoiser_weights,
+    path_kernel=path_kernel,
   )
   if camera_rays is not None:
     from mujoco_warp_pbr_amd.rays import set_camera_rays

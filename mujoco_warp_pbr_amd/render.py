@@ -385,9 +385,19 @@ def render_path_samples(m, d, rc: PbrRenderContext, seed: int = 0) -> None:
     raise ValueError("render_path_samples needs a path-traced context (Tier.PATH)")
   check_env_id(rc)
   warn_ignored_overrides(m, rc)
-  s = settings_struct(rc)
-  stream = _stream_of(d.geom_xpos)
-  for cams in camera_chunks(rc):
+  _path_trace(m, d, rc, settings_struct(rc), seed, _stream_of(d.geom_xpos))
+
+
+def _path_trace(m, d, rc: PbrRenderContext, s, seed: int, stream: int) -> None:
+  """The path tracer's launches of render(): the wavefront kernel (rc.path_kernel "wavefront",
+  cameras in one launch), else the per-pixel megakernel a launch per camera_chunks entry."""
+  chunks = camera_chunks(rc)
+  if rc.path_kernel == "wavefront" and len(chunks) == 1:
+    from mujoco_warp_pbr_amd import wavefront
+
+    if wavefront.render_context(m, d, rc, s, seed, stream):
+      return
+  for cams in chunks:
     sc = scene_struct(m, d, rc, cams)
     out = outputs_struct(rc, cams)
     _abi.call("mjpbr_pt_render", _abi.byref(sc), _abi.byref(s), _abi.byref(out), ctypes.c_int32(int(seed)), stream)
@@ -404,13 +414,14 @@ def render(m, d, rc: PbrRenderContext, seed: int = 0) -> None:
   warn_ignored_overrides(m, rc)
   s = settings_struct(rc)
   stream = _stream_of(d.geom_xpos)
-  # contexts with more cameras than one launch takes render them in consecutive launches
-  chunks = camera_chunks(rc)
-  entry = "mjpbr_pt_render" if rc.tier == Tier.PATH else "mjpbr_rt_render"
-  for cams in chunks:
-    sc = scene_struct(m, d, rc, cams)
-    out = outputs_struct(rc, cams)
-    _abi.call(entry, _abi.byref(sc), _abi.byref(s), _abi.byref(out), ctypes.c_int32(int(seed)), stream)
+  if rc.tier == Tier.PATH:
+    _path_trace(m, d, rc, s, seed, stream)
+  else:
+    # contexts with more cameras than one launch takes render them in consecutive launches
+    for cams in camera_chunks(rc):
+      sc = scene_struct(m, d, rc, cams)
+      out = outputs_struct(rc, cams)
+      _abi.call("mjpbr_rt_render", _abi.byref(sc), _abi.byref(s), _abi.byref(out), ctypes.c_int32(int(seed)), stream)
   if rc.tier == Tier.PATH and rc.denoise:
     if rc.denoiser == "learned":
       from mujoco_warp_pbr_amd import learned

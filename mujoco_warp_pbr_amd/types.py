@@ -194,6 +194,7 @@ class PbrRenderContext:
   ray_direction: Any = None  # and directions; None renders the pinhole cameras
   denoiser: str = "atrous"  # "atrous" (the reference's filter) or "learned" (U-Net, a-trous fallback)
   denoiser_weights: str | None = None  # learned denoiser weights (None: the packaged ones)
+  path_kernel: str = "wavefront"  # path tracer: "wavefront" (persistent) or "megakernel" (per pixel)
   denoise_sigma_color: float = 4.0
   denoise_sigma_normal: float = 128.0
   denoise_sigma_depth: float = 0.02

@@ -4,7 +4,8 @@ A batch is a list of path-traced (model, data, context) groups that share the re
 each group has its own worlds, camera resolutions and output buffers, so one launch renders,
 say, 2048 worlds at 64x64, 1536 at 128x128 and 512 at 256x256. The images are bit-identical to
 `render` of each group with the same seed: only the scheduling differs (dynamic tile queue,
-lanes refilled with new paths as old ones end, one resolve pass per pixel).
+lanes refilled with new paths as old ones end, each tile resolved by its workgroup). render()
+itself takes this kernel for path-traced contexts (render_context), as a batch of one group.
 
 No counterpart in the reference (mujoco_warp_pbr renders one context per `render` call with the
 per-pixel megakernel, pathtrace.py:426-438); the API mirrors `render(m, d, rc, seed)`.
@@ -50,6 +51,45 @@ class WavefrontConfig:
     return c
 
 
+class _Prepared:
+  """A workspace holding the descriptors of one set of groups (mjpbr_pt_wavefront_prepare) and the
+  plan of its launches; `key` is the descriptor bytes it was written from."""
+
+  def __init__(self, device, groups_per_cu: int):
+    self.device = device
+    self.groups_per_cu = groups_per_cu
+    self.workspace = None
+    self.plan = None
+    self.key = None
+
+  @staticmethod
+  def descriptor_key(scenes, outs, settings) -> bytes:
+    return bytes(scenes) + bytes(outs) + bytes(settings)
+
+  def prepare(self, scenes, outs, settings, stream: int) -> None:
+    """Sizes the workspace for launches of up to `groups_per_cu` workgroups per CU and writes the
+    descriptors into it; synchronises `stream`."""
+    lib = _abi.library()
+    n = len(scenes)
+    need = lib.mjpbr_pt_wavefront_workspace(_abi.byref(scenes), n, _abi.byref(settings), self.groups_per_cu)
+    if need < 0:
+      raise ValueError("invalid wavefront batch (groups, cameras or samples)")
+    if self.workspace is None or self.workspace.numel() < need:
+      self.workspace = None  # the old one is released before the new one is allocated
+      self.workspace = torch.empty(int(need), dtype=torch.uint8, device=self.device)
+    plan = _abi.structs()["MjpbrWavefrontPlan"]()
+    _abi.call(
+      "mjpbr_pt_wavefront_prepare", _abi.byref(scenes), _abi.byref(outs), n, _abi.byref(settings), _abi.ptr(self.workspace),
+      int(self.workspace.numel()), _abi.byref(plan), stream,
+    )
+    self.plan = plan
+    self.key = self.descriptor_key(scenes, outs, settings)
+
+  def launch(self, seed: int, config: WavefrontConfig, stream: int) -> None:
+    cfg = config.to_c()
+    _abi.call("mjpbr_pt_wavefront_launch", _abi.ptr(self.workspace), _abi.byref(self.plan), _abi.byref(cfg), ctypes.c_int32(int(seed)), stream)
+
+
 class WavefrontBatch:
   """One persistent launch over several path-traced world groups."""
 
@@ -79,9 +119,8 @@ class WavefrontBatch:
     self.groups = groups
     self.device = dev
     self.config = config or WavefrontConfig()
-    self._workspace = None
-    self._plan = None
-    self._key = None
+    # sized for the largest grid, so every launch shape the autotuner tries gets its full grid
+    self._state = _Prepared(dev, 16)
 
   @property
   def npixels(self) -> int:
@@ -102,30 +141,16 @@ class WavefrontBatch:
     settings = settings_struct(self.groups[0][2])
     return scenes, outs, settings
 
+  def _stream(self) -> int:
+    return torch.cuda.current_stream(self.device).cuda_stream
+
   def prepare(self) -> None:
     """(Re)writes the group descriptors into the workspace; synchronises the current stream.
     render() calls it whenever a group's buffers changed (a replaced per-world array)."""
-    scenes, outs, settings = self._structs()
-    lib = _abi.library()
-    n = len(self.groups)
-    need = lib.mjpbr_pt_wavefront_workspace(_abi.byref(scenes), n, _abi.byref(settings))
-    if need < 0:
-      raise ValueError("invalid wavefront batch (groups or samples)")
-    if self._workspace is None or self._workspace.numel() < need:
-      self._workspace = torch.empty(int(need), dtype=torch.uint8, device=self.device)
-    plan = _abi.structs()["MjpbrWavefrontPlan"]()
-    stream = torch.cuda.current_stream(self.device).cuda_stream
-    _abi.call(
-      "mjpbr_pt_wavefront_prepare", _abi.byref(scenes), _abi.byref(outs), n, _abi.byref(settings), _abi.ptr(self._workspace),
-      int(self._workspace.numel()), _abi.byref(plan), stream,
-    )
-    self._plan = plan
-    self._key = bytes(scenes) + bytes(outs) + bytes(settings)
+    self._state.prepare(*self._structs(), self._stream())
 
   def _launch(self, seed: int, config: WavefrontConfig) -> None:
-    cfg = config.to_c()
-    stream = torch.cuda.current_stream(self.device).cuda_stream
-    _abi.call("mjpbr_pt_wavefront_launch", _abi.ptr(self._workspace), _abi.byref(self._plan), _abi.byref(cfg), ctypes.c_int32(int(seed)), stream)
+    self._state.launch(seed, config, self._stream())
 
   def render(self, seed: int = 0, config: WavefrontConfig | None = None) -> None:
     """Renders every world of every group (as `render(m, d, rc, seed)` of each group would)."""
@@ -134,8 +159,8 @@ class WavefrontBatch:
     for _, _, rc in self.groups:
       check_env_id(rc)
     scenes, outs, settings = self._structs()
-    if self._key != bytes(scenes) + bytes(outs) + bytes(settings):
-      self.prepare()
+    if self._state.key != _Prepared.descriptor_key(scenes, outs, settings):
+      self._state.prepare(scenes, outs, settings, self._stream())
     self._launch(seed, config or self.config)
 
   def count_rays(self, seed: int = 0, config: WavefrontConfig | None = None) -> dict:
@@ -150,8 +175,8 @@ class WavefrontBatch:
     return out
 
   def capture(self, seed: int = 0, config: WavefrontConfig | None = None) -> torch.cuda.CUDAGraph:
-    """A graph of one render (counter reset, persistent kernel, resolve); replay() re-renders with
-    the same seed and buffers."""
+    """A graph of one render (counter reset, staging of oversized worlds, persistent kernel);
+    replay() re-renders with the same seed and buffers."""
     self.render(seed, config)  # prepares and warms up outside the capture
     torch.cuda.synchronize(self.device)
     graph = torch.cuda.CUDAGraph()
@@ -165,3 +190,31 @@ def render_batch(groups, seed: int = 0, config: WavefrontConfig | None = None) -
   batch = WavefrontBatch(groups, config)
   batch.render(seed)
   return batch
+
+
+# Launch shape of render()'s wavefront launches (a path-traced context as a batch of one group).
+RENDER_CONFIG = WavefrontConfig(waves_per_eu=4, groups_per_cu=4, refill_threshold=32)
+
+
+def render_context(m, d, rc, settings, seed: int, stream: int) -> bool:
+  """The path tracer of render(m, d, rc, seed) through the wavefront kernel: the noisy HDR and
+  features with `rc.denoise`, else the images (bit-identical to the per-pixel kernel's). The
+  workspace and descriptors are kept in the context and rewritten when a buffer changed.
+  Returns False, having done nothing, under graph capture before the context's current buffers
+  were prepared (preparing synchronises): the caller then launches the per-pixel kernel."""
+  S = _abi.structs()
+  scenes = (S["MjpbrScene"] * 1)()
+  outs = (S["MjpbrOutputs"] * 1)()
+  scenes[0] = scene_struct(m, d, rc)
+  outs[0] = outputs_struct(rc)
+  counts = rc._host_cache.get("ray_counts")
+  outs[0].ray_counts = None  # the launch counts through its config
+  state = rc._host_cache.get("wavefront")
+  if state is None or state.key != _Prepared.descriptor_key(scenes, outs, settings):
+    if torch.cuda.is_current_stream_capturing():
+      return False
+    if state is None or state.device != rc.hdr_data.device:
+      state = rc._host_cache["wavefront"] = _Prepared(rc.hdr_data.device, RENDER_CONFIG.groups_per_cu)
+    state.prepare(scenes, outs, settings, stream)
+  state.launch(seed, dataclasses.replace(RENDER_CONFIG, ray_counts=counts), stream)
+  return True

@@ -63,7 +63,7 @@ def test_oversized_world_without_world_stage_is_refused_without_a_gpu():
   plan = S["MjpbrWavefrontPlan"]()
   scenes_arr = (S["MjpbrScene"] * 1)(sc)
   outs_arr = (S["MjpbrOutputs"] * 1)(out)
-  need = lib.mjpbr_pt_wavefront_workspace(ctypes.addressof(scenes_arr), 1, ctypes.addressof(s))
+  need = lib.mjpbr_pt_wavefront_workspace(ctypes.addressof(scenes_arr), 1, ctypes.addressof(s), 16)
   assert need > 0
   rc = lib.mjpbr_pt_wavefront_prepare(ctypes.addressof(scenes_arr), ctypes.addressof(outs_arr), 1, ctypes.addressof(s), ctypes.c_void_p(256),
                                       need, ctypes.addressof(plan), None)

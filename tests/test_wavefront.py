@@ -1,8 +1,11 @@
 """Persistent wavefront path tracer: batches of world groups with mixed camera resolutions (edge
 tiles, two cameras per world) must reproduce the per-pixel megakernel bit for bit, for every
-launch shape the autotuner can pick, eagerly and as a replayed graph."""
+launch shape the autotuner can pick, eagerly and as a replayed graph; render() of a path-traced
+context takes it, denoiser features included."""
 
 from __future__ import annotations
+
+import contextlib
 
 import numpy as np
 import pytest
@@ -12,6 +15,7 @@ import mujoco_warp_pbr_amd as mj
 from mujoco_warp_pbr_amd import autotune
 from mujoco_warp_pbr_amd import model as mdl
 from mujoco_warp_pbr_amd import scenes
+from mujoco_warp_pbr_amd.render import render_path_samples
 from mujoco_warp_pbr_amd.scenes import _Builder
 from mujoco_warp_pbr_amd.wavefront import WavefrontBatch
 from mujoco_warp_pbr_amd.wavefront import WavefrontConfig
@@ -69,6 +73,22 @@ def _clear(rc):
     t.fill_(7)
 
 
+@contextlib.contextmanager
+def _path_kernel(rc, name):
+  old = rc.path_kernel
+  rc.path_kernel = name
+  try:
+    yield
+  finally:
+    rc.path_kernel = old
+
+
+def _mega_render(m, d, rc, seed):
+  """render() through the per-pixel megakernel (the reference's scheme)."""
+  with _path_kernel(rc, "megakernel"):
+    mj.render(m, d, rc, seed=seed)
+
+
 def _assert_same(a, b, what):
   for k in a:
     assert np.array_equal(a[k], b[k]), f"{what}: {k} differs in {np.sum(a[k] != b[k])} entries"
@@ -93,7 +113,7 @@ def test_gpu_wavefront_matches_megakernel(cuda):
   groups = [_group(cuda, n, res, 10 * i) for i, (n, res) in enumerate(GROUPS)]
   ref = []
   for m, d, rc in groups:
-    mj.render(m, d, rc, seed=11)
+    _mega_render(m, d, rc, 11)
     ref.append(_outputs(rc))
   batch = WavefrontBatch(groups)
   for cfg in (WavefrontConfig(), WavefrontConfig(4, 2, 1), WavefrontConfig(8, 8, 64), WavefrontConfig(8, 1, 32)):
@@ -113,7 +133,7 @@ def test_gpu_wavefront_batch_of_many_groups_matches_megakernel(cuda):
   groups = [_group(cuda, n, res, 31 + 7 * i) for i, (n, res) in enumerate(shapes)]
   ref = []
   for m, d, rc in groups:
-    mj.render(m, d, rc, seed=5)
+    _mega_render(m, d, rc, 5)
     ref.append(_outputs(rc))
   batch = WavefrontBatch(groups)
   for _, _, rc in groups:
@@ -168,21 +188,27 @@ def test_gpu_ray_counts_agree_between_kernels_and_leave_images_unchanged(cuda):
   camera segment, and each light and the sky add shadow rays."""
   groups = [_group(cuda, n, res, 10 * i) for i, (n, res) in enumerate(GROUPS)]
   mega = {k: 0 for k in ("segments", "light_shadow", "env_shadow", "total")}
+  single = dict(mega)
   ref = []
   for m, d, rc in groups:
-    mj.render(m, d, rc, seed=5)
+    _mega_render(m, d, rc, 5)
     ref.append(_outputs(rc))
     _clear(rc)
-    for k, v in mj.count_path_rays(m, d, rc, seed=5).items():
-      mega[k] += v
-  for i, (_, _, rc) in enumerate(groups):
+    with _path_kernel(rc, "megakernel"):
+      for k, v in mj.count_path_rays(m, d, rc, seed=5).items():
+        mega[k] += v
+  for i, (m, d, rc) in enumerate(groups):
     _assert_same(_outputs(rc), ref[i], f"counting megakernel, group {i}")
+    _clear(rc)
+    for k, v in mj.count_path_rays(m, d, rc, seed=5).items():  # render()'s wavefront launch
+      single[k] += v
+    _assert_same(_outputs(rc), ref[i], f"counting render() wavefront, group {i}")
     _clear(rc)
   batch = WavefrontBatch(groups)
   wf = batch.count_rays(seed=5)
   for i, (_, _, rc) in enumerate(groups):
     _assert_same(_outputs(rc), ref[i], f"counting wavefront, group {i}")
-  assert wf == mega
+  assert wf == mega and single == mega
   paths = sum(rc.base.nworld * rc.base.total_rays for _, _, rc in groups) * groups[0][2].path_samples
   assert mega["segments"] >= paths
   assert mega["light_shadow"] > 0 and mega["env_shadow"] > 0
@@ -201,3 +227,79 @@ def test_gpu_ray_counts_of_camera_segments_only(cuda):
   with pytest.raises(ValueError, match="path-traced"):
     m2, d2, rc2 = make(_scene(), 1, device=cuda, cam_res=(32, 32))
     mj.count_path_rays(m2, d2, rc2)
+
+
+def _denoise_outputs(rc):
+  torch.cuda.synchronize()
+  out = {"rgb": rc.base.rgb_data.cpu().numpy().copy(), "hdr": rc.hdr_data.cpu().numpy().copy()}
+  for k in ("aov_albedo", "aov_normal", "aov_depth", "aov_emission", "aov_variance"):
+    out[k] = getattr(rc, k).cpu().numpy().copy()
+  out["depth"] = rc.base.depth_data.cpu().numpy().copy()
+  out["seg"] = rc.base.seg_data.cpu().numpy().copy()
+  return out
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("spp", [1, 5])
+def test_gpu_render_takes_wavefront_with_denoiser_features(cuda, spp):
+  """render() of a path-traced context with the a-trous denoiser launches the wavefront kernel
+  (its workspace appears in the context), whose noisy HDR, first-hit features and filtered image
+  equal the megakernel's bit for bit; edge tiles (40x24) and two cameras per world."""
+  m, d, rc = make(_scene(), 3, device=cuda, cam_res=[(40, 24), (16, 16)], envmap=scenes.procedural_sky(1, 32, 64), envmap_intensity=1.0,
+                  **{**PT, "path_samples": spp, "denoise": True, "render_hdr": True})
+  scenes.randomize_worlds(m, rc, 3, seed=3)
+  mj.refit_bvh(m, d, rc)
+  _mega_render(m, d, rc, 9)
+  ref = _denoise_outputs(rc)
+  assert "wavefront" not in rc._host_cache
+  for t in (rc.base.rgb_data, rc.hdr_data, rc.aov_albedo, rc.aov_normal, rc.aov_depth, rc.aov_emission, rc.aov_variance):
+    t.fill_(3)
+  mj.render(m, d, rc, seed=9)
+  assert "wavefront" in rc._host_cache
+  _assert_same(_denoise_outputs(rc), ref, f"render() wavefront, {spp} spp")
+  # only the noisy HDR and features (no filter): the same through both kernels
+  render_path_samples(m, d, rc, seed=4)
+  a = _denoise_outputs(rc)
+  with _path_kernel(rc, "megakernel"):
+    render_path_samples(m, d, rc, seed=4)
+  _assert_same(_denoise_outputs(rc), a, "render_path_samples")
+
+
+@pytest.mark.gpu
+def test_gpu_render_wavefront_follows_replaced_buffers_and_graph_capture(cuda):
+  """A replaced per-world array re-prepares the context's wavefront descriptors; render() under
+  graph capture replays the eager images (prepared before the capture), and a capture of a
+  context never rendered eagerly falls back to the megakernel (the same images)."""
+  m, d, rc = make(_scene(), 2, device=cuda, cam_res=(24, 24), envmap=scenes.procedural_sky(1, 32, 64), envmap_intensity=1.0, **PT)
+  mj.refit_bvh(m, d, rc)
+  mj.render(m, d, rc, seed=1)
+  key = rc._host_cache["wavefront"].key
+  m.mat_rgba = m.mat_rgba.clone() * 0.5
+  _mega_render(m, d, rc, 1)
+  ref = _outputs(rc)
+  _clear(rc)
+  mj.render(m, d, rc, seed=1)
+  assert rc._host_cache["wavefront"].key != key
+  _assert_same(_outputs(rc), ref, "after a replaced array")
+  torch.cuda.synchronize()
+  graph = torch.cuda.CUDAGraph()
+  with torch.cuda.graph(graph):
+    mj.render(m, d, rc, seed=1)
+  for rep in range(3):
+    _clear(rc)
+    torch.cuda.synchronize()
+    graph.replay()
+    _assert_same(_outputs(rc), ref, f"replay {rep}")
+    mj.render(m, d, rc, seed=2)  # eager work between replays
+    torch.cuda.synchronize()
+  m2, d2, rc2 = make(_scene(), 2, device=cuda, cam_res=(24, 24), envmap=scenes.procedural_sky(1, 32, 64), envmap_intensity=1.0, **PT)
+  mj.refit_bvh(m2, d2, rc2)
+  torch.cuda.synchronize()
+  graph2 = torch.cuda.CUDAGraph()
+  with torch.cuda.graph(graph2):
+    mj.render(m2, d2, rc2, seed=1)
+  assert "wavefront" not in rc2._host_cache
+  graph2.replay()
+  b = _outputs(rc2)
+  _mega_render(m2, d2, rc2, 1)
+  _assert_same(_outputs(rc2), b, "captured megakernel fallback")
