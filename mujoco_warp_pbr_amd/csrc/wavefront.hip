@@ -90,24 +90,37 @@ MJ_HOST_DEV inline int slot_floats(const MjpbrSettings& s) { return s.denoise ? 
 
 MJ_HOST_DEV inline int64_t region_floats(const MjpbrSettings& s) { return (int64_t)kTilePixels * s.path_samples * slot_floats(s); }
 
-// One pixel of a finished tile from the workgroup's sample region (slot lp * spp + sample): the
+// One pixel (tile pixel threadIdx.x) of a finished tile from the workgroup's sample region, where
+// sample k of tile pixel lp sits in slot k * 256 + lp (a wave's lanes read adjacent slots): the
 // per-pixel kernel's arithmetic after its sample loop (pathtrace.py:340-438, pt.hip), sums in
-// sample order; the display colour and HDR, or with the denoiser the noisy HDR and the features.
+// sample order; the display colour and HDR, or with the features (s.denoise) the noisy HDR and
+// the features.
+template <bool kFeatures>
 MJ_DEV void resolve_pixel(const WfGroup& G, const MjpbrSettings& s, int w, int cidx, int tx, int ty, const float* region) {
   const int lp = threadIdx.x;
   const int blk = lp >> 6, ln = lp & 63;
   const int px = tx * kTile + (blk & 1) * 8 + (ln & 7), py = ty * kTile + (blk >> 1) * 8 + (ln >> 3);
   Pixel pix;
   if (!setup_pixel(G.sc.cams, G.out, s, w, cidx, px, py, G.tm.ray_offset[cidx], &pix) || !pix.want_rgb) return;
-  const int spp = s.path_samples, sf = slot_floats(s);
+  const int spp = s.path_samples, sf = kFeatures ? 12 : 3;
   const float exposure = G.sc.env.exposure[w % G.sc.env.exposure_n];
   const MjpbrOutputs& out = G.out;
   const int64_t adr = out.rgb_adr[cidx] + pix.pixelid;
-  const float* sl = region + (int64_t)lp * spp * sf;
+  const float* sl = region + (int64_t)lp * sf;
+  const int step = kTilePixels * sf;
   V3 hdr = v3s(0.0f);
   const float fn = (float)spp;
-  if (!s.denoise) {
-    for (int k = 0; k < spp; ++k) hdr += ld3(sl + 3 * k);
+  if (!kFeatures) {
+    // eight samples' loads in flight at a time, summed in sample order
+    int k = 0;
+    for (; k + 8 <= spp; k += 8, sl += 8 * step) {
+      V3 r[8];
+#pragma unroll
+      for (int j = 0; j < 8; ++j) r[j] = ld3(sl + j * step);
+#pragma unroll
+      for (int j = 0; j < 8; ++j) hdr += r[j];
+    }
+    for (; k < spp; ++k, sl += step) hdr += ld3(sl);
     hdr = hdr / fn;
     if (s.render_hdr) st3(out.hdr + 3 * ((int64_t)w * out.hdr_stride + adr), hdr);
     out.rgb[(int64_t)w * out.rgb_stride + adr] = encode_display(display_transform(hdr * exposure, s.tone_mapper));
@@ -115,15 +128,27 @@ MJ_DEV void resolve_pixel(const WfGroup& G, const MjpbrSettings& s, int w, int c
   }
   V3 albedo_sum = v3s(0.0f), normal_sum = v3s(0.0f), emission_sum = v3s(0.0f);
   float lum_sq = 0.0f;
-  for (int k = 0; k < spp; ++k, sl += 12) {
-    const V3 radiance = ld3(sl);
+  auto add = [&](const float* v) {
+    const V3 radiance = ld3(v);
     hdr += radiance;
-    albedo_sum += ld3(sl + 3);
-    normal_sum += ld3(sl + 6);
-    emission_sum += ld3(sl + 9);
+    albedo_sum += ld3(v + 3);
+    normal_sum += ld3(v + 6);
+    emission_sum += ld3(v + 9);
     const float lum = luminance(radiance) * exposure;
     lum_sq += lum * lum;
+  };
+  // four samples' loads in flight at a time, summed in sample order
+  int k = 0;
+  for (; k + 4 <= spp; k += 4, sl += 4 * step) {
+    float v[4][12];
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+      for (int q = 0; q < 12; ++q) v[j][q] = sl[j * step + q];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) add(v[j]);
   }
+  for (; k < spp; ++k, sl += step) add(sl);
   hdr = hdr / fn;
   const int64_t a = (int64_t)w * out.aov_stride + adr;
   st3(out.hdr + 3 * ((int64_t)w * out.hdr_stride + adr), hdr);
@@ -142,9 +167,12 @@ MJ_DEV void resolve_pixel(const WfGroup& G, const MjpbrSettings& s, int w, int c
 
 // kCount: adds the rays traced to ray_counts (the same image). kGlobal: some group reads its
 // worlds' records in place from world_stage (a per-item branch; 3 % slower, so only batches with
-// such a group take this instance). Workgroup b keeps its tile's samples in region b of `slots`
-// and resolves the tile at the top of the next item iteration, after the barrier that ends it.
-template <int kWavesPerEu, bool kCount, bool kGlobal>
+// such a group take this instance). kFeatures (= s.denoise): the first-hit features are kept per
+// sample; as a runtime flag they stayed live, and spilled, in the path loop of every launch
+// (material_grid at 16 spp 74.3 against 78.4 ms without them). Workgroup b keeps its tile's
+// samples in region b of `slots` and resolves the tile at the top of the next item iteration,
+// after the barrier that ends it.
+template <int kWavesPerEu, bool kCount, bool kGlobal, bool kFeatures>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kWavesPerEu, kWavesPerEu)))
 wf_kernel(const WfHeader* __restrict__ hdr, int* counter, float* slots, MjpbrSettings s, int seed, int refill, int64_t item_limit,
           uint64_t* ray_counts) {
@@ -155,13 +183,13 @@ wf_kernel(const WfHeader* __restrict__ hdr, int* counter, float* slots, MjpbrSet
   const int64_t nitems = item_limit > 0 && item_limit < hdr->nitems ? item_limit : hdr->nitems;
   const int spp = s.path_samples;
   const int pool = kTilePixels * spp;
-  const int sf = slot_floats(s);
+  const int sf = kFeatures ? 12 : 3;
   float* region = slots + (int64_t)blockIdx.x * region_floats(s);
   RayCount rays{0u, 0u, 0u};
   int done_g = -1, done_w = 0, done_cidx = 0, done_tx = 0, done_ty = 0;  // the tile to resolve
   for (;;) {
     __syncthreads();  // every wave is done with the previous item's LDS, pool and sample slots
-    if (done_g >= 0) resolve_pixel(hdr->groups[done_g], s, done_w, done_cidx, done_tx, done_ty, region);
+    if (done_g >= 0) resolve_pixel<kFeatures>(hdr->groups[done_g], s, done_w, done_cidx, done_tx, done_ty, region);
     if (threadIdx.x == 0) {
       s_item = atomicAdd(counter, 1);
       s_pool = 0;
@@ -198,7 +226,7 @@ wf_kernel(const WfHeader* __restrict__ hdr, int* counter, float* slots, MjpbrSet
     PathState st;
     Primary first;
     bool alive = false, pool_done = false;
-    int smp = 0, pseed = 0, pixelid = 0, slot = 0;
+    int pseed = 0, slot = 0;  // slot: sample * 256 + tile pixel (the path's sample and pixel)
     float dcz = 0.0f;
     for (;;) {
       const uint64_t dead = __ballot(!alive);
@@ -231,9 +259,7 @@ wf_kernel(const WfHeader* __restrict__ hdr, int* counter, float* slots, MjpbrSet
                 pseed = seed + w * total_rays + pix.rayid;
                 path_primary(pix, s, pseed, sample, &o, &d, &md, &dc);
                 path_begin(&st, &first, o, d, md);
-                smp = sample;
-                pixelid = pix.pixelid;
-                slot = lp * spp + sample;
+                slot = pid;
                 dcz = dc.z;
                 alive = true;
               }
@@ -242,16 +268,18 @@ wf_kernel(const WfHeader* __restrict__ hdr, int* counter, float* slots, MjpbrSet
         }
       }
       if (ndead == 64 && pool_done) break;
-      if (alive && !path_step(c, &st, head_dir, pseed, smp, &first)) {
+      if (alive && !path_step(c, &st, head_dir, pseed, slot >> 8, &first)) {
         alive = false;
         float* sl = region + (int64_t)slot * sf;
         st3(sl, st.radiance);
-        if (s.denoise) {
+        if (kFeatures) {
           st3(sl + 3, first.albedo);
           st3(sl + 6, first.normal);
           st3(sl + 9, first.emission);
         }
-        if (smp == 0) {
+        if (slot < kTilePixels) {  // sample 0
+          const int blk = slot >> 6, ln = slot & 63;
+          const int pixelid = (ty * kTile + (blk >> 1) * 8 + (ln >> 3)) * sc.cams.cam_res[cidx][0] + tx * kTile + (blk & 1) * 8 + (ln & 7);
           const float depth = first.hit.geom_id != -1 ? first.hit.dist * -dcz : 0.0f;
           if (want_seg) {
             int2 seg = first.hit.geom_id != -1 ? make_int2(first.hit.geom_id, 5) : make_int2(-1, -1);
@@ -263,7 +291,7 @@ wf_kernel(const WfHeader* __restrict__ hdr, int* counter, float* slots, MjpbrSet
             st3(G.out.albedo + 3 * a, first.base_color);
             st3(G.out.normal + 3 * a, first.normal);
           }
-          if (s.denoise) G.out.aov_depth[(int64_t)w * G.out.aov_stride + G.out.rgb_adr[cidx] + pixelid] = depth;
+          if (kFeatures) G.out.aov_depth[(int64_t)w * G.out.aov_stride + G.out.rgb_adr[cidx] + pixelid] = depth;
         }
       }
     }
@@ -403,19 +431,15 @@ extern "C" int mjpbr_pt_wavefront_launch(void* workspace, const MjpbrWavefrontPl
   using WfFn = void (*)(const WfHeader*, int*, float*, MjpbrSettings, int, int, int64_t, uint64_t*);
   const bool count = cfg->ray_counts != nullptr;
   const bool global = plan->nstaged > 0;
-  WfFn fn;
-  switch (cfg->waves_per_eu) {
-    case 4:
-      fn = global ? (count ? wf_kernel<4, true, true> : wf_kernel<4, false, true>)
-                  : (count ? wf_kernel<4, true, false> : wf_kernel<4, false, false>);
-      break;
-    case 8:
-      fn = global ? (count ? wf_kernel<8, true, true> : wf_kernel<8, false, true>)
-                  : (count ? wf_kernel<8, true, false> : wf_kernel<8, false, false>);
-      break;
-    default:
-      return set_error(MJPBR_EINVAL, "wavefront: waves_per_eu must be 4 or 8, got %d", cfg->waves_per_eu);
-  }
+  if (cfg->waves_per_eu != 4 && cfg->waves_per_eu != 8)
+    return set_error(MJPBR_EINVAL, "wavefront: waves_per_eu must be 4 or 8, got %d", cfg->waves_per_eu);
+  // [waves 8][count][global][features]
+  static const WfFn kFns[2][2][2][2] = {
+      {{{wf_kernel<4, false, false, false>, wf_kernel<4, false, false, true>}, {wf_kernel<4, false, true, false>, wf_kernel<4, false, true, true>}},
+       {{wf_kernel<4, true, false, false>, wf_kernel<4, true, false, true>}, {wf_kernel<4, true, true, false>, wf_kernel<4, true, true, true>}}},
+      {{{wf_kernel<8, false, false, false>, wf_kernel<8, false, false, true>}, {wf_kernel<8, false, true, false>, wf_kernel<8, false, true, true>}},
+       {{wf_kernel<8, true, false, false>, wf_kernel<8, true, false, true>}, {wf_kernel<8, true, true, false>, wf_kernel<8, true, true, true>}}}};
+  const WfFn fn = kFns[cfg->waves_per_eu == 8][count][global][s.denoise != 0];
   hipLaunchKernelGGL(fn, dim3((unsigned)grid), dim3(256), plan->lds_bytes, st, hdr, counter, slots, s, seed, refill, (int64_t)cfg->item_limit,
                      cfg->ray_counts);
   return check_launch("wf_kernel");
