@@ -61,7 +61,7 @@ _ARGUMENT_RULES = (
   ("denoise_iterations",) + _at_least(0),
   ("shadow_light_fraction", lambda v: 0.0 <= v <= 1.0, "must be in [0, 1], got {v}"),
   ("denoiser", lambda v: v in ("atrous", "learned"), 'must be "atrous" or "learned", got {v!r}'),
-  ("path_kernel", lambda v: v in ("wavefront", "megakernel"), 'must be "wavefront" or "megakernel", got {v!r}'),
+  ("path_kernel", lambda v: v in ("auto", "wavefront", "megakernel"), 'must be "auto", "wavefront" or "megakernel", got {v!r}'),
   ("mesh_bvh_frame", lambda v: v in _MESH_BVH_FRAMES, 'must be "auto" or "mujoco", got {v!r}'),
   ("mesh_bvh_constructor", lambda v: v in _MESH_BVH_CONSTRUCTORS, f"must be one of {_MESH_BVH_CONSTRUCTORS}, got {{v!r}}"),
   ("traversal", lambda v: v in _TRAVERSALS, 'must be "software", "optix" or "auto", got {v!r}'),
@@ -69,6 +69,17 @@ _ARGUMENT_RULES = (
   ("max_registers", lambda v: v is None or v >= 0, "must be at least 0, got {v}"),
   ("nworld", lambda v: v >= 1, "must be at least 1, got {v}"),
 )
+
+
+def _rendered_mesh_triangles(mjm, rendered) -> int:
+  """Triangles of the rendered mesh geoms, counted once a geom (instances of one mesh add up)."""
+  ids = np.asarray(rendered)
+  if ids.dtype == bool:
+    ids = np.nonzero(ids)[0]
+  mesh = ids[mjm.geom_type[ids] == mdl.GEOM_MESH]
+  dataid = mjm.geom_dataid[mesh]
+  dataid = dataid[dataid >= 0]
+  return int(np.asarray(mjm.mesh_facenum, np.int64)[dataid].sum()) if dataid.size else 0
 
 
 def _check_arguments(args: dict) -> None:
@@ -183,7 +194,7 @@ def create_render_context(
   precompute: bool = True,
   denoiser: str = "atrous",
   denoiser_weights: str | None = None,
-  path_kernel: str = "wavefront",
+  path_kernel: str = "auto",
 ) -> PbrRenderContext:
   """Creates a PBR render context on `device` (reference io.py:331-634; same arguments).
 
@@ -202,9 +213,11 @@ def create_render_context(
   denoiser needs camera sides that are multiples of 64.
 
   `path_kernel` picks the path tracer's kernel: "wavefront" (a persistent kernel that refills
-  lanes as paths end, the default) or "megakernel" (one thread per pixel, the reference's
-  scheme). Both give identical images; contexts with more cameras than one launch takes use the
-  megakernel.
+  lanes as paths end), "megakernel" (one thread per pixel, the reference's scheme) or "auto"
+  (the default: the wavefront kernel where it measured faster, at two or more samples a pixel on
+  scenes of at most 65,536 rendered mesh triangles). Both give identical images; contexts with
+  more cameras than one launch takes, and renders of hundreds of samples a pixel (sample regions
+  beyond 2 GiB), use the megakernel.
 
   `precompute=False` leaves the GPU-built tables (DFG LUT, texture and radiance mip chains,
   prefiltered environment stacks, SH9 coefficients, sampling tables) zero; it exists so host-side
@@ -336,6 +349,7 @@ def create_render_context(
     headlight_specular=tuple(float(x) for x in mjm.headlight_specular),
     has_spot_lights=bool(mjm.nlight and np.any(mjm.light_type == mdl.LIGHT_SPOT)),
     has_meshes=bool(np.any(np.isin(mjm.geom_type[rendered], (mdl.GEOM_MESH, mdl.GEOM_SDF, mdl.GEOM_HFIELD)))),
+    mesh_triangles=_rendered_mesh_triangles(mjm, rendered),
   )
 
   base.bvh_geom_ids = torch.as_tensor(bvh_ids).to(dev)

@@ -113,6 +113,8 @@ class RenderBase:
   has_spot_lights: bool = False
   # an enabled geom is a mesh, SDF or height field (reference io.py:317 `has_meshes`)
   has_meshes: bool = True
+  # triangles of the rendered mesh geoms, counted once a geom (the path tracer's kernel choice)
+  mesh_triangles: int = 0
   # [nworld, floats] per-world records the render kernels resolve once a render and copy into
   # LDS (MjpbrGeometry.world_stage); None: each workgroup resolves its world itself
   world_stage: Any = None
@@ -194,7 +196,7 @@ class PbrRenderContext:
   ray_direction: Any = None  # and directions; None renders the pinhole cameras
   denoiser: str = "atrous"  # "atrous" (the reference's filter) or "learned" (U-Net, a-trous fallback)
   denoiser_weights: str | None = None  # learned denoiser weights (None: the packaged ones)
-  path_kernel: str = "wavefront"  # path tracer: "wavefront" (persistent) or "megakernel" (per pixel)
+  path_kernel: str = "auto"  # path tracer: "wavefront" (persistent), "megakernel" (per pixel) or "auto"
   denoise_sigma_color: float = 4.0
   denoise_sigma_normal: float = 128.0
   denoise_sigma_depth: float = 0.02

@@ -66,14 +66,18 @@ class _Prepared:
   def descriptor_key(scenes, outs, settings) -> bytes:
     return bytes(scenes) + bytes(outs) + bytes(settings)
 
+  def workspace_bytes(self, scenes, settings) -> int:
+    """Workspace for launches of up to `groups_per_cu` workgroups per CU."""
+    need = _abi.library().mjpbr_pt_wavefront_workspace(_abi.byref(scenes), len(scenes), _abi.byref(settings), self.groups_per_cu)
+    if need < 0:
+      raise ValueError("invalid wavefront batch (groups, cameras or samples)")
+    return int(need)
+
   def prepare(self, scenes, outs, settings, stream: int) -> None:
     """Sizes the workspace for launches of up to `groups_per_cu` workgroups per CU and writes the
     descriptors into it; synchronises `stream`."""
-    lib = _abi.library()
     n = len(scenes)
-    need = lib.mjpbr_pt_wavefront_workspace(_abi.byref(scenes), n, _abi.byref(settings), self.groups_per_cu)
-    if need < 0:
-      raise ValueError("invalid wavefront batch (groups, cameras or samples)")
+    need = self.workspace_bytes(scenes, settings)
     if self.workspace is None or self.workspace.numel() < need:
       self.workspace = None  # the old one is released before the new one is allocated
       self.workspace = torch.empty(int(need), dtype=torch.uint8, device=self.device)
@@ -192,8 +196,14 @@ def render_batch(groups, seed: int = 0, config: WavefrontConfig | None = None) -
   return batch
 
 
-# Launch shape of render()'s wavefront launches (a path-traced context as a batch of one group).
+# Launch shape of render()'s wavefront launches (a path-traced context as a batch of one group):
+# the best of a sweep on the reference suite's scenes and the full-feature scene at 1024 x 128^2.
 RENDER_CONFIG = WavefrontConfig(waves_per_eu=4, groups_per_cu=4, refill_threshold=32)
+
+# Largest workspace render() gives the wavefront kernel (its sample regions grow with the samples
+# a pixel: 192 KiB a workgroup at 16 samples with the denoiser features); beyond it, hundreds of
+# samples a pixel, render() launches the per-pixel megakernel.
+WORKSPACE_BUDGET = 2 << 30
 
 
 def render_context(m, d, rc, settings, seed: int, stream: int) -> bool:
@@ -201,7 +211,8 @@ def render_context(m, d, rc, settings, seed: int, stream: int) -> bool:
   features with `rc.denoise`, else the images (bit-identical to the per-pixel kernel's). The
   workspace and descriptors are kept in the context and rewritten when a buffer changed.
   Returns False, having done nothing, under graph capture before the context's current buffers
-  were prepared (preparing synchronises): the caller then launches the per-pixel kernel."""
+  were prepared (preparing synchronises), or when the workspace would exceed WORKSPACE_BUDGET:
+  the caller then launches the per-pixel kernel."""
   S = _abi.structs()
   scenes = (S["MjpbrScene"] * 1)()
   outs = (S["MjpbrOutputs"] * 1)()
@@ -213,8 +224,11 @@ def render_context(m, d, rc, settings, seed: int, stream: int) -> bool:
   if state is None or state.key != _Prepared.descriptor_key(scenes, outs, settings):
     if torch.cuda.is_current_stream_capturing():
       return False
-    if state is None or state.device != rc.hdr_data.device:
-      state = rc._host_cache["wavefront"] = _Prepared(rc.hdr_data.device, RENDER_CONFIG.groups_per_cu)
+    fresh = _Prepared(rc.hdr_data.device, RENDER_CONFIG.groups_per_cu)
+    if fresh.workspace_bytes(scenes, settings) > WORKSPACE_BUDGET:
+      return False
+    if state is None or state.device != fresh.device:
+      state = rc._host_cache["wavefront"] = fresh
     state.prepare(scenes, outs, settings, stream)
   state.launch(seed, dataclasses.replace(RENDER_CONFIG, ray_counts=counts), stream)
   return True

@@ -6,6 +6,7 @@ context takes it, denoiser features included."""
 from __future__ import annotations
 
 import contextlib
+import ctypes
 
 import numpy as np
 import pytest
@@ -242,11 +243,11 @@ def _denoise_outputs(rc):
 @pytest.mark.gpu
 @pytest.mark.parametrize("spp", [1, 5])
 def test_gpu_render_takes_wavefront_with_denoiser_features(cuda, spp):
-  """render() of a path-traced context with the a-trous denoiser launches the wavefront kernel
-  (its workspace appears in the context), whose noisy HDR, first-hit features and filtered image
+  """render() of a path-traced context with the a-trous denoiser and path_kernel="wavefront"
+  launches the wavefront kernel (its workspace appears in the context), whose noisy HDR, first-hit features and filtered image
   equal the megakernel's bit for bit; edge tiles (40x24) and two cameras per world."""
   m, d, rc = make(_scene(), 3, device=cuda, cam_res=[(40, 24), (16, 16)], envmap=scenes.procedural_sky(1, 32, 64), envmap_intensity=1.0,
-                  **{**PT, "path_samples": spp, "denoise": True, "render_hdr": True})
+                  **{**PT, "path_samples": spp, "denoise": True, "render_hdr": True, "path_kernel": "wavefront"})
   scenes.randomize_worlds(m, rc, 3, seed=3)
   mj.refit_bvh(m, d, rc)
   _mega_render(m, d, rc, 9)
@@ -303,3 +304,86 @@ def test_gpu_render_wavefront_follows_replaced_buffers_and_graph_capture(cuda):
   b = _outputs(rc2)
   _mega_render(m2, d2, rc2, 1)
   _assert_same(_outputs(rc2), b, "captured megakernel fallback")
+
+
+def _bare_scene(S, nworld, res):
+  sc = S["MjpbrScene"]()
+  sc.cams.nrender = 1
+  sc.cams.cam_res[0][0], sc.cams.cam_res[0][1] = res
+  sc.cams.total_rays = res[0] * res[1]
+  sc.geo.nworld = nworld
+  sc.geo.tlas_depth = 1
+  return sc
+
+
+def test_workspace_holds_one_sample_region_a_workgroup():
+  """mjpbr_pt_wavefront_workspace: a fixed header plus one region a persistent workgroup (at most
+  one a tile) of 256 pixels x samples x 3 floats (12 with the denoiser features)."""
+  from mujoco_warp_pbr_amd import _abi
+
+  lib, S = _abi.library(), _abi.structs()
+  scenes_arr = (S["MjpbrScene"] * 1)(_bare_scene(S, 3, (40, 24)))  # 3 x 2 tiles a world: 18 items
+  s = S["MjpbrSettings"]()
+
+  def size(spp, denoise=0, gpc=16):
+    s.path_samples, s.denoise = spp, denoise
+    return lib.mjpbr_pt_wavefront_workspace(ctypes.addressof(scenes_arr), 1, ctypes.addressof(s), gpc)
+
+  region = size(2) - size(1)
+  assert region == 18 * 256 * 3 * 4
+  fixed = size(1) - region
+  assert fixed > 0 and size(7) == fixed + 7 * region
+  assert size(5, denoise=1) == fixed + 4 * 5 * region
+  assert size(1, gpc=1) == size(1)  # 18 tiles: fewer regions than any grid
+  for bad in (0, 17):
+    assert size(1, gpc=bad) == -1
+  assert size(0) == -1
+
+
+@pytest.mark.gpu
+def test_gpu_render_beyond_the_workspace_budget_takes_the_megakernel(cuda, monkeypatch):
+  from mujoco_warp_pbr_amd import wavefront
+
+  m, d, rc = make(_scene(), 2, device=cuda, cam_res=(24, 24), envmap=scenes.procedural_sky(1, 32, 64), envmap_intensity=1.0, **PT)
+  mj.refit_bvh(m, d, rc)
+  monkeypatch.setattr(wavefront, "WORKSPACE_BUDGET", 1 << 10)
+  mj.render(m, d, rc, seed=6)
+  assert "wavefront" not in rc._host_cache
+  a = _outputs(rc)
+  monkeypatch.undo()
+  _clear(rc)
+  mj.render(m, d, rc, seed=6)
+  assert "wavefront" in rc._host_cache
+  _assert_same(_outputs(rc), a, "megakernel beyond the budget")
+
+
+def test_auto_path_kernel_choice():
+  """path_kernel "auto": the wavefront kernel at 2+ samples a pixel on scenes of at most 65,536
+  rendered mesh triangles (counted once a geom), else the per-pixel kernel; "wavefront" and
+  "megakernel" force one."""
+  from mujoco_warp_pbr_amd import io
+  from mujoco_warp_pbr_amd.render import AUTO_WAVEFRONT_MAX_MESH_TRIANGLES
+  from mujoco_warp_pbr_amd.render import _wavefront_pays
+
+  mjm = _scene()
+  tris = io._rendered_mesh_triangles(mjm, np.ones(mjm.ngeom, bool))
+  mesh_geom = int(np.nonzero(mjm.geom_type == mdl.GEOM_MESH)[0][0])
+  assert tris == int(mjm.mesh_facenum[mjm.geom_dataid[mesh_geom]]) > 0
+  hidden = np.ones(mjm.ngeom, bool)
+  hidden[mesh_geom] = False
+  assert io._rendered_mesh_triangles(mjm, hidden) == 0
+  m, d, rc = make(mjm, 1, device="cpu", cam_res=(16, 16), **PT)
+  assert rc.path_kernel == "auto" and rc.base.mesh_triangles == tris
+  assert _wavefront_pays(rc)
+  rc.path_samples = 1
+  assert not _wavefront_pays(rc)
+  rc.path_kernel = "wavefront"
+  assert _wavefront_pays(rc)
+  rc.path_kernel, rc.path_samples = "auto", 4
+  rc.base.mesh_triangles = AUTO_WAVEFRONT_MAX_MESH_TRIANGLES + 1
+  assert not _wavefront_pays(rc)
+  rc.path_kernel = "megakernel"
+  rc.base.mesh_triangles = 0
+  assert not _wavefront_pays(rc)
+  with pytest.raises(ValueError, match="path_kernel"):
+    make(mjm, 1, device="cpu", cam_res=(16, 16), **{**PT, "path_kernel": "fast"})
