@@ -30,6 +30,7 @@ namespace mjpbr {
 
 constexpr int kWfMaxGroups = MJPBR_WF_MAX_GROUPS;
 constexpr int kTilePixels = kTile * kTile;
+static_assert(kTilePixels == 256, "a path's slot keeps its sample above bit 8 and its tile pixel below");
 
 struct WfGroup {
   MjpbrScene sc;

@@ -53,7 +53,11 @@ class WavefrontConfig:
 
 class _Prepared:
   """A workspace holding the descriptors of one set of groups (mjpbr_pt_wavefront_prepare) and the
-  plan of its launches; `key` is the descriptor bytes it was written from."""
+  plan of its launches; `key` is the descriptor bytes it was written from.
+
+  A launch captured into a graph keeps reading its workspace's descriptors at every replay, so a
+  captured workspace is never rewritten or freed: the next prepare moves to a new workspace and
+  keeps the captured one alive (`retired`) for as long as this object lives."""
 
   def __init__(self, device, groups_per_cu: int):
     self.device = device
@@ -61,6 +65,8 @@ class _Prepared:
     self.workspace = None
     self.plan = None
     self.key = None
+    self.captured = False
+    self.retired = []
 
   @staticmethod
   def descriptor_key(scenes, outs, settings) -> bytes:
@@ -75,11 +81,18 @@ class _Prepared:
 
   def prepare(self, scenes, outs, settings, stream: int) -> None:
     """Sizes the workspace for launches of up to `groups_per_cu` workgroups per CU and writes the
-    descriptors into it; synchronises `stream`."""
+    descriptors into it; synchronises `stream` (and the device, before it rewrites or frees a
+    workspace that launches on another stream may still read)."""
     n = len(scenes)
     need = self.workspace_bytes(scenes, settings)
-    if self.workspace is None or self.workspace.numel() < need:
-      self.workspace = None  # the old one is released before the new one is allocated
+    if self.captured:
+      self.retired.append(self.workspace)
+      self.workspace, self.captured = None, False
+    elif self.workspace is not None:
+      torch.cuda.synchronize(self.device)
+      if self.workspace.numel() < need:
+        self.workspace = None  # released before the larger one is allocated
+    if self.workspace is None:
       self.workspace = torch.empty(int(need), dtype=torch.uint8, device=self.device)
     plan = _abi.structs()["MjpbrWavefrontPlan"]()
     _abi.call(
@@ -90,6 +103,8 @@ class _Prepared:
     self.key = self.descriptor_key(scenes, outs, settings)
 
   def launch(self, seed: int, config: WavefrontConfig, stream: int) -> None:
+    if torch.cuda.is_current_stream_capturing():
+      self.captured = True
     cfg = config.to_c()
     _abi.call("mjpbr_pt_wavefront_launch", _abi.ptr(self.workspace), _abi.byref(self.plan), _abi.byref(cfg), ctypes.c_int32(int(seed)), stream)
 

@@ -387,3 +387,27 @@ def test_auto_path_kernel_choice():
   assert not _wavefront_pays(rc)
   with pytest.raises(ValueError, match="path_kernel"):
     make(mjm, 1, device="cpu", cam_res=(16, 16), **{**PT, "path_kernel": "fast"})
+
+
+@pytest.mark.gpu
+def test_gpu_captured_render_keeps_its_buffers_after_a_reprepare(cuda):
+  """A graph of render() keeps rendering the buffers it was captured with after render() moved to
+  new ones (a larger sample count needs a larger workspace): the captured workspace stays alive
+  and unmodified."""
+  m, d, rc = make(_scene(), 2, device=cuda, cam_res=(24, 24), envmap=scenes.procedural_sky(1, 32, 64), envmap_intensity=1.0, **PT)
+  mj.refit_bvh(m, d, rc)
+  mj.render(m, d, rc, seed=3)
+  ref = _outputs(rc)
+  graph = torch.cuda.CUDAGraph()
+  with torch.cuda.graph(graph):
+    mj.render(m, d, rc, seed=3)
+  old_rgb = rc.base.rgb_data
+  rc.base.rgb_data = torch.zeros_like(old_rgb)
+  rc.path_samples = 6  # a new settings key and a larger workspace
+  mj.render(m, d, rc, seed=3)
+  torch.cuda.synchronize()
+  assert len(rc._host_cache["wavefront"].retired) == 1
+  old_rgb.fill_(7)
+  graph.replay()
+  torch.cuda.synchronize()
+  assert torch.equal(old_rgb.cpu(), torch.as_tensor(ref["rgb"]))
