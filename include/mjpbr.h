@@ -374,7 +374,8 @@ int mjpbr_rt_render_noalloc(const MjpbrScene* scene, const MjpbrSettings* s, Mjp
 
 /* Feature mask of the real-time kernel instance mjpbr_rt_render launches for these settings
  * (bits: 1 meshes, 2 supersampling, 4 soft shadows, 8 mirror, 16 layers, 32 textures): the
- * reference's kernel specialisation key (kernel_inputs.py:321-337) as compiled instances. */
+ * reference's kernel specialisation key (kernel_inputs.py:321-337) as compiled instances; bit 128
+ * marks the instances of 2x2 supersampling that run one lane a sample (same images). */
 int mjpbr_rt_kernel_features(const MjpbrSettings* s);
 
 /* Path-traced tier megakernel: one thread per (world, pixel), `path_samples` paths each

@@ -105,11 +105,28 @@ MJ_DEV V3 trace_pixel(const RtCtx& c, V3 origin, V3 dir, float max_dist, float f
 // Occupancy over registers: W waves per SIMD (W = 8: 64 VGPRs a lane) with spills to scratch
 // beat fewer waves without, since waves mostly wait on dependent BVH, texture and spill loads;
 // W is chosen per instance from measurements (rt_kernel instances table below).
+//
+// kFQuad instances (2x2 supersampling) run one lane a sample: four workgroups a 16x16 tile (8x8
+// pixels each, on the tile's XCD), a wave a 4x4 pixel block with a pixel's four samples on
+// adjacent lanes (sample = lane % 4). A wave then spans 16 pixels instead of 64, and the four
+// lanes of a pixel trace nearly the same rays, so far fewer lanes idle in divergent branches
+// (full-feature scene at 1024 x 128^2: 57.1 -> 43.4 ms; as 1024-thread workgroups a tile, 64.4 ms:
+// a workgroup of 16 waves holds its compute unit's slots until its slowest wave ends). The
+// pixel's sums are gathered on its first lane in sample order, as the per-pixel loop adds them,
+// so the images are bit-identical.
 template <unsigned F, int W>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W, W)))
 rt_kernel(MjpbrScene sc, MjpbrSettings s, MjpbrOutputs out, int seed, TileMap tm) {
   extern __shared__ float lds[];
-  int b = logical_block(tm);
+  constexpr bool kQuad = (F & kFQuad) != 0;
+  int b, quarter = 0;
+  if constexpr (kQuad) {  // four workgroups a tile, on the tile's XCD: 8 x 8 pixels each
+    const int xcd = blockIdx.x & 7, k4 = blockIdx.x >> 3;
+    quarter = tm.xcd_chunk > 0 ? (k4 & 3) : (int)(blockIdx.x & 3);
+    b = tm.xcd_chunk > 0 ? xcd * tm.xcd_chunk + (k4 >> 2) : (int)(blockIdx.x >> 2);
+  } else {
+    b = logical_block(tm);
+  }
   if (b >= tm.nblocks) return;
   int w = b / tm.tiles_per_world;
   int t = b - w * tm.tiles_per_world;
@@ -118,8 +135,15 @@ rt_kernel(MjpbrScene sc, MjpbrSettings s, MjpbrOutputs out, int seed, TileMap tm
   int tl = t - tm.tile_base[cidx];
   int tx = tl % tm.tiles_x[cidx], ty = tl / tm.tiles_x[cidx];
   int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  int px = tx * kTile + (wave & 1) * 8 + (lane & 7);
-  int py = ty * kTile + (wave >> 1) * 8 + (lane >> 3);
+  int px, py;
+  if constexpr (kQuad) {
+    const int p = lane >> 2;  // pixel of the wave's 4x4 block; lane & 3 is the sample
+    px = tx * kTile + (quarter & 1) * 8 + (wave & 1) * 4 + (p & 3);
+    py = ty * kTile + (quarter >> 1) * 8 + (wave >> 1) * 4 + (p >> 2);
+  } else {
+    px = tx * kTile + (wave & 1) * 8 + (lane & 7);
+    py = ty * kTile + (wave >> 1) * 8 + (lane >> 3);
+  }
 
   SceneView sv = (F & kFGlobal) ? global_world(sc, w, lds, tm.mesh_cache_nodes, tm.mesh_cache_n)
                                  : stage_world(sc, w, lds, tm.mesh_cache_nodes, tm.mesh_cache_n);
@@ -130,6 +154,7 @@ rt_kernel(MjpbrScene sc, MjpbrSettings s, MjpbrOutputs out, int seed, TileMap tm
   RtCtx c{sc, s, sv, w, 0, exposure};
   bool cull = s.cull_backfaces != 0;
   if (!pix.want_rgb) {
+    if (kQuad && (lane & 3) != 0) return;
     V3 o, d, dc;
     float md;
     primary_ray(pix, s, 1, 0, 0, &o, &d, &md, &dc);
@@ -145,13 +170,30 @@ rt_kernel(MjpbrScene sc, MjpbrSettings s, MjpbrOutputs out, int seed, TileMap tm
   V3 head_dir = -V3{pix.xmat[2], pix.xmat[5], pix.xmat[8]};
   c.seed = seed + w * sc.cams.total_rays + pix.rayid;
   V3 display = v3s(0.0f), hdr = v3s(0.0f);
-  for (int smp = 0; smp < n_aa * n_aa; ++smp) {
+  if constexpr (kQuad) {  // the launch guarantees samples_per_axis == 2
+    const int smp = lane & 3;
     V3 o, d, dc;
     float md;
-    primary_ray(pix, s, n_aa, smp % n_aa, smp / n_aa, &o, &d, &md, &dc);
-    V3 radiance = trace_pixel<F>(c, o, d, md, width, spread, head_dir, 2 * smp, smp == 0, out, pix, dc);
-    display += display_transform(radiance * exposure, s.tone_mapper);
-    hdr += radiance;
+    primary_ray(pix, s, 2, smp & 1, smp >> 1, &o, &d, &md, &dc);
+    const V3 radiance = trace_pixel<F>(c, o, d, md, width, spread, head_dir, 2 * smp, smp == 0, out, pix, dc);
+    const V3 shown = display_transform(radiance * exposure, s.tone_mapper);
+    // a pixel's four lanes are all here (they return together): its sums in sample order
+    const int first = lane & ~3;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      display += V3{pull_float(first + k, shown.x), pull_float(first + k, shown.y), pull_float(first + k, shown.z)};
+      hdr += V3{pull_float(first + k, radiance.x), pull_float(first + k, radiance.y), pull_float(first + k, radiance.z)};
+    }
+    if (smp != 0) return;
+  } else {
+    for (int smp = 0; smp < n_aa * n_aa; ++smp) {
+      V3 o, d, dc;
+      float md;
+      primary_ray(pix, s, n_aa, smp % n_aa, smp / n_aa, &o, &d, &md, &dc);
+      V3 radiance = trace_pixel<F>(c, o, d, md, width, spread, head_dir, 2 * smp, smp == 0, out, pix, dc);
+      display += display_transform(radiance * exposure, s.tone_mapper);
+      hdr += radiance;
+    }
   }
   float inv2 = inv_aa * inv_aa;
   int64_t adr = out.rgb_adr[pix.camid] + pix.pixelid;
@@ -189,6 +231,9 @@ const RtInstance kRtInstances[] = {
     MJPBR_RT_INSTANCE(kFMesh | kFSS | kFSoft | kFMirror | kFTex, 8),
     MJPBR_RT_INSTANCE(kFAll, 8),
     MJPBR_RT_INSTANCE(kFAll | kFGlobal, 8),
+    MJPBR_RT_INSTANCE(kFMesh | kFSS | kFSoft | kFMirror | kFTex | kFQuad, 8),
+    MJPBR_RT_INSTANCE(kFAll | kFQuad, 8),
+    MJPBR_RT_INSTANCE(kFAll | kFGlobal | kFQuad, 8),
 };
 #undef MJPBR_RT_INSTANCE
 
@@ -196,6 +241,7 @@ unsigned required_features(const MjpbrSettings& s) {
   unsigned f = 0;
   if (s.has_meshes) f |= kFMesh;
   if (s.samples_per_axis > 1) f |= kFSS;
+  if (s.samples_per_axis == 2) f |= kFQuad;
   if (s.use_shadows && s.shadow_samples > 1) f |= kFSoft;
   if (s.has_reflection) f |= kFMirror;
   if (s.has_transparency) f |= kFLayer;
@@ -203,11 +249,14 @@ unsigned required_features(const MjpbrSettings& s) {
   return f;
 }
 
-// The instance with the fewest features among those covering `need` (kFAll covers everything).
+// The instance with the fewest features among those covering `need` (kFAll covers everything) and
+// of the launch form it asks for: one lane a sample exactly when samples_per_axis is 2.
 const RtInstance& pick_instance(unsigned need) {
   const RtInstance* best = nullptr;
   for (const RtInstance& k : kRtInstances)
-    if ((k.mask & need) == need && (!best || __builtin_popcount(k.mask) < __builtin_popcount(best->mask))) best = &k;
+    if ((k.mask & need) == need && (k.mask & kFQuad) == (need & kFQuad) &&
+        (!best || __builtin_popcount(k.mask) < __builtin_popcount(best->mask)))
+      best = &k;
   return *best;
 }
 
@@ -235,8 +284,10 @@ extern "C" int mjpbr_rt_render(const MjpbrScene* scene, const MjpbrSettings* s, 
   if ((k.mask & kFMesh) && scene->geo.ngeom >= (1 << kPendShift))
     return set_error(MJPBR_EINVAL, "ngeom must be below %d, got %d", 1 << kPendShift, scene->geo.ngeom);
   lds += plan_mesh_cache(scene, (k.mask & kFMesh) && s->has_meshes, lds + job_lds, &tm);
+  const int64_t grid = (int64_t)nlaunch * ((k.mask & kFQuad) ? 4 : 1);
+  if (grid > INT32_MAX) return set_error(MJPBR_EINVAL, "too many workgroups (%lld)", (long long)grid);
   if ((rc = launch_stage_prep(scene, (hipStream_t)stream))) return rc;
-  hipLaunchKernelGGL(k.fn, dim3(nlaunch), dim3(256), lds, (hipStream_t)stream, *scene, *s, *out, seed, tm);
+  hipLaunchKernelGGL(k.fn, dim3((unsigned)grid), dim3(256), lds, (hipStream_t)stream, *scene, *s, *out, seed, tm);
   return check_launch("rt_kernel");
 }
 

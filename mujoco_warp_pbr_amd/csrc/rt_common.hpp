@@ -17,7 +17,10 @@ constexpr float kMinThroughput = 1.0f / 255.0f;  // render.py:62
 //   kFMirror  has_reflection
 //   kFLayer   has_transparency
 //   kFTex     textures
-enum : unsigned { kFMesh = 1, kFSS = 2, kFSoft = 4, kFMirror = 8, kFLayer = 16, kFTex = 32, kFAll = 63, kFGlobal = 64 };
+// and two launch forms: kFGlobal (world records read in place from geo.world_stage) and kFQuad
+// (samples_per_axis == 2: one lane a sample, a pixel's four samples on adjacent lanes, four
+// workgroups a 16x16 tile).
+enum : unsigned { kFMesh = 1, kFSS = 2, kFSoft = 4, kFMirror = 8, kFLayer = 16, kFTex = 32, kFAll = 63, kFGlobal = 64, kFQuad = 128 };
 
 struct RtCtx {
   const MjpbrScene& sc;

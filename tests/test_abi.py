@@ -43,7 +43,8 @@ def test_errors_are_reported_without_a_gpu():
 def test_realtime_kernel_instance_covers_the_settings():
   """mjpbr_rt_kernel_features names the compiled real-time instance a launch uses: it covers
   every feature the settings need (mesh 1, supersampling 2, soft shadows 4, mirror 8, layers 16,
-  textures 32) and is the smallest such instance for the common cases."""
+  textures 32), runs one lane a sample (128) exactly at 2x2 supersampling, and is the smallest
+  such instance for the common cases."""
   lib = _abi.library()
   S = _abi.structs()
 
@@ -62,8 +63,11 @@ def test_realtime_kernel_instance_covers_the_settings():
   assert mask(has_meshes=1) == 1
   assert mask(has_meshes=1, textures=1) == 33
   full = dict(has_meshes=1, samples_per_axis=2, shadow_samples=4, has_reflection=1, textures=1)
-  assert mask(**full) == 47
-  assert mask(**full, has_transparency=1) == 63
+  assert mask(**full) == 47 | 128
+  assert mask(**full, has_transparency=1) == 63 | 128
+  assert mask(**{**full, "samples_per_axis": 3}) == 47
+  assert mask(**{**full, "samples_per_axis": 3}, has_transparency=1) == 63
+  assert mask(samples_per_axis=2) == 47 | 128  # the smallest one-lane-a-sample instance
   # soft shadows only count with shadows on; any needed bit is covered
   assert mask(shadow_samples=4, use_shadows=0) == 0
   for bits in range(64):
@@ -75,6 +79,7 @@ def test_realtime_kernel_instance_covers_the_settings():
     if bits & 16: kw["has_transparency"] = 1
     if bits & 32: kw["textures"] = 1
     assert mask(**kw) & bits == bits
+    assert not mask(**kw) & 128  # 3x3 supersampling runs the per-pixel sample loop
 
 
 def test_library_built_against_another_header_is_refused(tmp_path, monkeypatch):

@@ -170,16 +170,19 @@ def _instance_cases():
     (40, lambda: scenes.suite_primitives(32), {}),
     (1, lambda: scenes.mesh_scene(1, res=32, nmesh=4, instances=2, ntri=600), {"use_textures": False}),
     (33, lambda: scenes.mesh_scene(1, res=32, nmesh=4, instances=2, ntri=600), {}),
-    (47, lambda: scenes.full_scene(1, res=32), {"samples_per_pixel": 2, "shadow_samples": 4}),
-    (63, translucent_full, {"samples_per_pixel": 2, "shadow_samples": 4}),
+    (47, lambda: scenes.full_scene(1, res=32), {"samples_per_pixel": 3, "shadow_samples": 4}),
+    (63, translucent_full, {"samples_per_pixel": 3, "shadow_samples": 4}),
+    (47 | 128, lambda: scenes.full_scene(1, res=32), {"samples_per_pixel": 2, "shadow_samples": 4}),
+    (63 | 128, translucent_full, {"samples_per_pixel": 2, "shadow_samples": 4}),
   ]
 
 
 @pytest.mark.parametrize("mask,scene,kw", _instance_cases(), ids=[str(c[0]) for c in _instance_cases()])
 def test_every_kernel_instance_matches_oracle(cuda, mask, scene, kw):
   """Each compiled real-time kernel instance (feature mask: 1 meshes, 2 supersampling, 4 soft
-  shadows, 8 mirror, 16 layers, 32 textures) renders its own scene like the oracle: instances are
-  separately compiled code, so each needs its own parity check."""
+  shadows, 8 mirror, 16 layers, 32 textures, 128 one lane a sample at 2x2 supersampling) renders
+  its own scene like the oracle: instances are separately compiled code, so each needs its own
+  parity check."""
   from mujoco_warp_pbr_amd import _abi
   from mujoco_warp_pbr_amd.render import settings_struct
 
@@ -187,7 +190,7 @@ def test_every_kernel_instance_matches_oracle(cuda, mask, scene, kw):
 
   def setup(m, d, rc):
     scenes.randomize_worlds(m, rc, 2, seed=mask + 1)
-    if mask == 63:
+    if (mask & 63) == 63:
       m.geom_rgba[..., 3, 3] = 0.5  # randomize_worlds made every geom opaque
     rc.light_bulbradius = torch.full_like(rc.light_bulbradius, 0.05)
 
