@@ -388,20 +388,18 @@ def render_path_samples(m, d, rc: PbrRenderContext, seed: int = 0) -> None:
   _path_trace(m, d, rc, settings_struct(rc), seed, _stream_of(d.geom_xpos))
 
 
-# rc.path_kernel "auto" takes the wavefront kernel at 2+ samples a pixel on scenes of at most this
-# many rendered mesh triangles. Measured at 1024 worlds (ms, per-pixel kernel vs wavefront): the
-# reference suite's scenes at 16 spp + a-trous 97.6 vs 82.7 and 121.7 vs 112.4; the full-feature
-# scene (4k triangles) at 4 spp 256^2 280.3 vs 247.9, at 1 spp 128^2 23.9 vs 24.5; the mesh-heavy
-# scene (200k triangles) at 4 spp 128^2 191.4 vs 209.8, at 1 spp 256^2 182.9 vs 226.8.
-AUTO_WAVEFRONT_MAX_MESH_TRIANGLES = 1 << 16
-
-
+# rc.path_kernel "auto" takes the wavefront kernel at 2+ samples a pixel (where lanes refill with
+# new paths, and a wave's new paths share few pixels) and the per-pixel kernel at 1. Measured at
+# 1024 worlds (ms, per-pixel kernel vs wavefront): the reference suite's scenes at 16 spp + a-trous
+# 97.1 vs 77.1 and 121.6 vs 108.7; the full-feature scene at 4 spp 256^2 280.2 vs 237.4, at 1 spp
+# 128^2 23.8 vs 24.4; the mesh-heavy scene at 4 spp 128^2 191.5 vs 173.9, at 1 spp 256^2 182.6 vs
+# 194.0 (its launch shape: wavefront.render_config).
 def _wavefront_pays(rc: PbrRenderContext) -> bool:
   if rc.path_kernel == "wavefront":
     return True
   if rc.path_kernel == "megakernel":
     return False
-  return int(rc.path_samples) >= 2 and int(rc.base.mesh_triangles) <= AUTO_WAVEFRONT_MAX_MESH_TRIANGLES
+  return int(rc.path_samples) >= 2
 
 
 def _path_trace(m, d, rc: PbrRenderContext, s, seed: int, stream: int) -> None:

@@ -211,9 +211,19 @@ def render_batch(groups, seed: int = 0, config: WavefrontConfig | None = None) -
   return batch
 
 
-# Launch shape of render()'s wavefront launches (a path-traced context as a batch of one group):
-# the best of a sweep on the reference suite's scenes and the full-feature scene at 1024 x 128^2.
+# Launch shapes of render()'s wavefront launches (a path-traced context as a batch of one group),
+# the best of sweeps at 1024 worlds: RENDER_CONFIG on the reference suite's scenes and the
+# full-feature scene (4k mesh triangles), MESH_HEAVY_CONFIG on scenes of more than
+# MESH_HEAVY_TRIANGLES rendered mesh triangles, whose mesh walks hide their load latency better
+# at 8 waves a SIMD (the 200k-triangle scene at 4 spp 128^2: 173.9 ms against 193.4 at 4 waves).
 RENDER_CONFIG = WavefrontConfig(waves_per_eu=4, groups_per_cu=4, refill_threshold=32)
+MESH_HEAVY_CONFIG = WavefrontConfig(waves_per_eu=8, groups_per_cu=8, refill_threshold=32)
+MESH_HEAVY_TRIANGLES = 1 << 16
+
+
+def render_config(rc) -> WavefrontConfig:
+  """The launch shape render() gives the context's wavefront launches."""
+  return MESH_HEAVY_CONFIG if int(rc.base.mesh_triangles) > MESH_HEAVY_TRIANGLES else RENDER_CONFIG
 
 # Largest workspace render() gives the wavefront kernel (its sample regions grow with the samples
 # a pixel: 192 KiB a workgroup at 16 samples with the denoiser features); beyond it, hundreds of
@@ -235,15 +245,16 @@ def render_context(m, d, rc, settings, seed: int, stream: int) -> bool:
   outs[0] = outputs_struct(rc)
   counts = rc._host_cache.get("ray_counts")
   outs[0].ray_counts = None  # the launch counts through its config
+  config = render_config(rc)
   state = rc._host_cache.get("wavefront")
   if state is None or state.key != _Prepared.descriptor_key(scenes, outs, settings):
     if torch.cuda.is_current_stream_capturing():
       return False
-    fresh = _Prepared(rc.hdr_data.device, RENDER_CONFIG.groups_per_cu)
+    fresh = _Prepared(rc.hdr_data.device, config.groups_per_cu)
     if fresh.workspace_bytes(scenes, settings) > WORKSPACE_BUDGET:
       return False
-    if state is None or state.device != fresh.device:
+    if state is None or state.device != fresh.device or state.groups_per_cu != fresh.groups_per_cu:
       state = rc._host_cache["wavefront"] = fresh
     state.prepare(scenes, outs, settings, stream)
-  state.launch(seed, dataclasses.replace(RENDER_CONFIG, ray_counts=counts), stream)
+  state.launch(seed, dataclasses.replace(config, ray_counts=counts), stream)
   return True

@@ -100,6 +100,25 @@ def test_gpu_mesh_heavy_path_traced_matches_oracle(cuda, mesh_heavy):
   _pt_parity(gpu, ref, "mesh-heavy PT")
 
 
+def test_gpu_mesh_heavy_wavefront_matches_megakernel(cuda, mesh_heavy):
+  """render() of the mesh-heavy scene at 2 paths a pixel takes the wavefront kernel at 8 waves a
+  SIMD (more than 65,536 mesh triangles): the images equal the per-pixel kernel's bit for bit."""
+  from mujoco_warp_pbr_amd import wavefront
+
+  m, d, rc = make(mesh_heavy, 2, device=cuda, cam_res=(64, 64), envmap=scenes.procedural_sky(1, 64, 128), tier=mj.Tier.PATH,
+                  path_samples=2, max_depth=8, rr_depth=3, render_hdr=True)
+  _randomized(m, d, rc, 2, seed=7)
+  assert wavefront.render_config(rc) == wavefront.MESH_HEAVY_CONFIG
+  mj.render(m, d, rc, seed=2)
+  assert "wavefront" in rc._host_cache
+  wave = gpu_outputs(rc)
+  rc.path_kernel = "megakernel"
+  mj.render(m, d, rc, seed=2)
+  mega = gpu_outputs(rc)
+  for k in ("rgb", "hdr", "depth", "seg"):
+    assert np.array_equal(mega[k], wave[k]), f"{k} differs from the megakernel"
+
+
 def _spot_scene():
   mjm = scenes.primitives_scene(4, 48)
   mjm.light_type[1] = mdl.LIGHT_SPOT

@@ -358,11 +358,11 @@ def test_gpu_render_beyond_the_workspace_budget_takes_the_megakernel(cuda, monke
 
 
 def test_auto_path_kernel_choice():
-  """path_kernel "auto": the wavefront kernel at 2+ samples a pixel on scenes of at most 65,536
-  rendered mesh triangles (counted once a geom), else the per-pixel kernel; "wavefront" and
-  "megakernel" force one."""
+  """path_kernel "auto": the wavefront kernel at 2+ samples a pixel, else the per-pixel kernel;
+  "wavefront" and "megakernel" force one. The wavefront launch shape: 8 waves a SIMD on scenes of
+  more than 65,536 rendered mesh triangles (counted once a geom), 4 otherwise."""
   from mujoco_warp_pbr_amd import io
-  from mujoco_warp_pbr_amd.render import AUTO_WAVEFRONT_MAX_MESH_TRIANGLES
+  from mujoco_warp_pbr_amd import wavefront
   from mujoco_warp_pbr_amd.render import _wavefront_pays
 
   mjm = _scene()
@@ -375,16 +375,16 @@ def test_auto_path_kernel_choice():
   m, d, rc = make(mjm, 1, device="cpu", cam_res=(16, 16), **PT)
   assert rc.path_kernel == "auto" and rc.base.mesh_triangles == tris
   assert _wavefront_pays(rc)
+  assert wavefront.render_config(rc) == wavefront.RENDER_CONFIG
   rc.path_samples = 1
   assert not _wavefront_pays(rc)
   rc.path_kernel = "wavefront"
   assert _wavefront_pays(rc)
-  rc.path_kernel, rc.path_samples = "auto", 4
-  rc.base.mesh_triangles = AUTO_WAVEFRONT_MAX_MESH_TRIANGLES + 1
+  rc.path_kernel, rc.path_samples = "megakernel", 4
   assert not _wavefront_pays(rc)
-  rc.path_kernel = "megakernel"
-  rc.base.mesh_triangles = 0
-  assert not _wavefront_pays(rc)
+  rc.base.mesh_triangles = wavefront.MESH_HEAVY_TRIANGLES + 1
+  assert wavefront.render_config(rc) == wavefront.MESH_HEAVY_CONFIG
+  assert wavefront.MESH_HEAVY_CONFIG.waves_per_eu == 8 and wavefront.RENDER_CONFIG.waves_per_eu == 4
   with pytest.raises(ValueError, match="path_kernel"):
     make(mjm, 1, device="cpu", cam_res=(16, 16), **{**PT, "path_kernel": "fast"})
 
