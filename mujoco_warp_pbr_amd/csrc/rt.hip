@@ -110,8 +110,9 @@ MJ_DEV V3 trace_pixel(const RtCtx& c, V3 origin, V3 dir, float max_dist, float f
 // pixels each, on the tile's XCD), a wave a 4x4 pixel block with a pixel's four samples on
 // adjacent lanes (sample = lane % 4). A wave then spans 16 pixels instead of 64, and the four
 // lanes of a pixel trace nearly the same rays, so far fewer lanes idle in divergent branches
-// (full-feature scene at 1024 x 128^2: 57.1 -> 43.4 ms; as 1024-thread workgroups a tile, 64.4 ms:
-// a workgroup of 16 waves holds its compute unit's slots until its slowest wave ends). The
+// (full-feature scene at 1024 x 128^2: 57.1 -> 43.4 ms at 8 waves a SIMD, 38.7 at 6; as
+// 1024-thread workgroups a tile, 64.4 ms: a workgroup of 16 waves holds its compute unit's slots
+// until its slowest wave ends). The
 // pixel's sums are gathered on its first lane in sample order, as the per-pixel loop adds them,
 // so the images are bit-identical.
 template <unsigned F, int W>
@@ -217,7 +218,9 @@ struct RtInstance {
 // Compiled instances, each launched for the settings whose required features it covers, at the
 // occupancy measured best for it (waves per SIMD at 1024 worlds x 128^2: the reflective-plane
 // textured scene runs 6 % faster at 6 than at 8 and 3 % slower at 5; the leanest instance 2 %
-// faster at 7 than at 8 or 6; the full-feature one 1 % slower at 7 than at 8). The textured mesh
+// faster at 7 than at 8 or 6; the full-feature one with its per-pixel sample loop 1 % slower at 7
+// than at 8, with one lane a sample 38.7 ms at 6 against 38.8 at 7, 44.3 at 8, 39.3 at 5: its
+// lanes diverge less, so fewer spills beat more waves). The textured mesh
 // instance runs at 6: compiled for 8 (64 VGPRs, about 270 bytes a lane of spills) it shaded mesh
 // hits wrongly after a source change in code it never executes, while the same source at 6 or 4
 // waves matches the CPU oracle - its spill code at 8 is not trusted.
@@ -231,9 +234,9 @@ const RtInstance kRtInstances[] = {
     MJPBR_RT_INSTANCE(kFMesh | kFSS | kFSoft | kFMirror | kFTex, 8),
     MJPBR_RT_INSTANCE(kFAll, 8),
     MJPBR_RT_INSTANCE(kFAll | kFGlobal, 8),
-    MJPBR_RT_INSTANCE(kFMesh | kFSS | kFSoft | kFMirror | kFTex | kFQuad, 8),
-    MJPBR_RT_INSTANCE(kFAll | kFQuad, 8),
-    MJPBR_RT_INSTANCE(kFAll | kFGlobal | kFQuad, 8),
+    MJPBR_RT_INSTANCE(kFMesh | kFSS | kFSoft | kFMirror | kFTex | kFQuad, 6),
+    MJPBR_RT_INSTANCE(kFAll | kFQuad, 6),
+    MJPBR_RT_INSTANCE(kFAll | kFGlobal | kFQuad, 6),
 };
 #undef MJPBR_RT_INSTANCE
 
