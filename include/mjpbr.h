@@ -248,6 +248,9 @@ typedef struct MjpbrSettings {
   /* 1 when an enabled geom is a mesh, SDF or height field (reference io.py:317 `has_meshes`);
    * 0 selects kernel instances without mesh traversal, which then skip such geoms. */
   int32_t has_meshes;
+  /* Triangles of the rendered mesh geoms, counted once a geom (instances add up): a scheduling
+   * hint only (the path tracer's occupancy), the images do not depend on it. */
+  int32_t mesh_triangles;
 } MjpbrSettings;
 
 /* Output buffers (reference: kernel_inputs.py:190-204 `Outputs`). Per-world flat images:

@@ -12,12 +12,16 @@
 
 namespace mjpbr {
 
-// W waves per SIMD. With meshes, 8 with spills beat fewer without: the path loop is latency bound
-// (measured 1.8x over 2 waves on the full-feature scene at 1024 x 128^2, 1 sample; 8 waves 103.9
-// ms against 4 waves 109.1 at 4 samples). Without meshes 4 waves win (the reference suite's
-// scenes at 16 samples, depth 4: material_grid 115.3 -> 94.5 ms, primitives 153.8 -> 118.4 ms).
+// W waves per SIMD, from measurements at 1024 worlds: on scenes of more than 64 k mesh triangles
+// 8 with spills (the mesh walks' dependent loads want the waves: the 200 k-triangle scene at 256^2,
+// 1 sample, 182.5 ms against 189.9 at 6); with lighter meshes 6 (the full-feature scene at 128^2,
+// 1 sample, 22.65 against 23.72 at 8; 6 and 8 both far ahead of 2); without meshes 4 (the
+// reference suite's scenes at 16 samples, depth 4: material_grid 115.3 -> 94.5 ms, primitives
+// 153.8 -> 118.4 ms against 8).
 // kGlobal: the world's records are read in place from geo.world_stage (too large for the LDS).
 // kCount: adds the rays traced to out.ray_counts (the same image).
+constexpr int kHeavyMeshTriangles = 1 << 16;
+
 template <bool kGlobal, bool kCount, int W>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W, W)))
 pt_kernel(MjpbrScene sc, MjpbrSettings s, MjpbrOutputs out, int seed, TileMap tm) {
@@ -128,11 +132,13 @@ extern "C" int mjpbr_pt_render(const MjpbrScene* scene, const MjpbrSettings* s, 
   if ((rc = launch_stage_prep(scene, (hipStream_t)stream))) return rc;
   using PtFn = void (*)(MjpbrScene, MjpbrSettings, MjpbrOutputs, int, TileMap);
   const bool count = out->ray_counts != nullptr;
-  PtFn fn;
-  if (s->has_meshes)
-    fn = global ? (count ? pt_kernel<true, true, 8> : pt_kernel<true, false, 8>) : (count ? pt_kernel<false, true, 8> : pt_kernel<false, false, 8>);
-  else
-    fn = global ? (count ? pt_kernel<true, true, 4> : pt_kernel<true, false, 4>) : (count ? pt_kernel<false, true, 4> : pt_kernel<false, false, 4>);
+  // [waves: 4 without meshes, 6 light meshes, 8 heavy meshes][global][count]
+  static const PtFn kFns[3][2][2] = {
+      {{pt_kernel<false, false, 4>, pt_kernel<false, true, 4>}, {pt_kernel<true, false, 4>, pt_kernel<true, true, 4>}},
+      {{pt_kernel<false, false, 6>, pt_kernel<false, true, 6>}, {pt_kernel<true, false, 6>, pt_kernel<true, true, 6>}},
+      {{pt_kernel<false, false, 8>, pt_kernel<false, true, 8>}, {pt_kernel<true, false, 8>, pt_kernel<true, true, 8>}}};
+  const int occupancy = !s->has_meshes ? 0 : s->mesh_triangles > kHeavyMeshTriangles ? 2 : 1;
+  const PtFn fn = kFns[occupancy][global][count];
   hipLaunchKernelGGL(fn, dim3(nlaunch), dim3(256), lds, (hipStream_t)stream, *scene, *s, *out, seed, tm);
   return check_launch("pt_kernel");
 }

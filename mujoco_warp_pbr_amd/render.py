@@ -255,6 +255,7 @@ def settings_struct(rc: PbrRenderContext):
   s.max_depth = int(rc.max_depth)
   s.rr_depth = int(rc.rr_depth)
   s.has_meshes = int(bool(base.has_meshes))
+  s.mesh_triangles = min(int(base.mesh_triangles), 2**31 - 1)
   return s
 
 

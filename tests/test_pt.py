@@ -153,6 +153,21 @@ def test_gpu_per_sample_radiance_matches_oracle(cuda, seed):
 
 
 @pytest.mark.gpu
+def test_gpu_per_sample_radiance_without_meshes_matches_oracle(cuda):
+  """One path per pixel on a scene without meshes: the per-pixel kernel's instance for such
+  scenes (4 waves a SIMD; the full-feature scene above runs the 6-wave one, the mesh-heavy
+  scene of test_configs_gpu.py the 8-wave one) follows the oracle's random decisions too."""
+  mjm = scenes.primitives_scene(3, 32)
+  gpu, ref, (_, _, rc) = _pt_both(mjm, 3, (32, 32), seed=4, envmap=scenes.procedural_sky(2, 64, 128), path_samples=1, max_depth=8,
+                                  **PATH)
+  assert not rc.base.has_meshes
+  np.testing.assert_array_equal(gpu["seg"], ref["seg"])
+  np.testing.assert_allclose(gpu["depth"], ref["depth"], rtol=1e-5, atol=1e-7)
+  diff = np.abs(gpu["hdr"] - ref["hdr"]).max(axis=-1)
+  assert np.mean(diff <= 1e-3) >= 0.999, f"{np.mean(diff > 1e-3):.4%} of samples differ, max {diff.max()}"
+
+
+@pytest.mark.gpu
 def test_gpu_sample_means_match_oracle(cuda):
   mjm = scenes.full_scene(1, res=24)
   gpu, ref, _ = _pt_both(mjm, 2, (24, 24), setup=_full_setup(2), envmap=scenes.procedural_sky(1, 64, 128), path_samples=4,
