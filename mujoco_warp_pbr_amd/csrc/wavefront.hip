@@ -2,19 +2,21 @@
 // resolutions, in one launch (no counterpart in the reference, whose path tracer is the per-pixel
 // megakernel of pathtrace.py:200-438; the per-path arithmetic here is that kernel's, path.hpp).
 //
-// Work items are 16x16 pixel tiles of one world of one group (every group has its own cameras,
-// world count and outputs), handed out by a global atomic counter to a persistent grid. A
+// Work items are runs of 16x16 pixel tiles of one camera of one world of one group (every group
+// has its own cameras, world count and outputs; four tiles a run at 1 sample a pixel, so that a
+// run holds more paths than a workgroup has lanes, else one), handed out by a global atomic
+// counter to a persistent grid. A
 // workgroup stages the item's world (scene BVH + geom frames) in LDS (a world whose records exceed
 // the LDS is staged into its group's world_stage by a launch before, and read from there); its
-// four waves then draw the tile's 256 x spp paths from an LDS counter, pixel by pixel (path id
-// pixel * spp + sample: the paths a wave claims together share few pixels, so their camera
+// four waves then draw the item's tiles x 256 x spp paths from an LDS counter, pixel by pixel
+// (path id pixel * spp + sample: the paths a wave claims together share few pixels, so their camera
 // segments trace nearly the same rays; sample-major ids measured 7-9 % slower). Paths end after
 // different numbers of segments (misses, Russian roulette, depth limit): after each segment a
 // wave ballots its finished lanes and, once at least `refill` lanes are free, claims that many
 // new paths with one LDS atomic and hands them to the free lanes by their prefix rank (mbcnt).
 // Lanes stay busy with live paths instead of idling until the wave's longest path ends. A path's radiance (and with the denoiser
 // its first-hit features) goes to its sample slot in the workgroup's own region; once every path
-// of the tile has ended, the workgroup averages each pixel's samples in sample order and writes
+// of the item has ended, the workgroup averages each pixel's samples in sample order and writes
 // the display colour (or the noisy HDR and features), so the results are bit-identical to the
 // megakernel's.
 //
@@ -40,6 +42,8 @@ struct WfGroup {
   int64_t item_base;   // first work item of the group
   int64_t stage_base;  // global groups: first index of their worlds in wf_stage_kernel's grid
   int32_t global;      // 1: the world's records are read in place from sc.geo.world_stage
+  int32_t items_per_world;                    // every camera's tiles in runs of tiles_per_item
+  int32_t item_cam_base[MJPBR_MAX_CAMS + 1];  // first item of each camera within a world
 };
 
 struct WfHeader {
@@ -90,26 +94,54 @@ __global__ void wf_reset_kernel(int* counter) {
 // the first hit's albedo, normal and emission (the per-pixel kernel's per-sample sums).
 MJ_HOST_DEV inline int slot_floats(const MjpbrSettings& s) { return s.denoise ? 12 : 3; }
 
-MJ_HOST_DEV inline int64_t region_floats(const MjpbrSettings& s) { return (int64_t)kTilePixels * s.path_samples * slot_floats(s); }
+// Tiles of one camera a work item holds: four at 1 sample a pixel, so that the lanes have more
+// paths than lanes to refill from (one tile would be one path a lane), else one. An item's pixels
+// q = tile * 256 + tile pixel fit `item_bits(spp)` bits.
+MJ_HOST_DEV inline int tiles_per_item(int spp) { return spp == 1 ? 4 : 1; }
+MJ_HOST_DEV inline int item_bits(int spp) { return spp == 1 ? 10 : 8; }
 
-// One pixel (tile pixel threadIdx.x) of a finished tile from the workgroup's sample region, where
-// sample k of tile pixel lp sits in slot k * 256 + lp (a wave's lanes read adjacent slots): the
-// per-pixel kernel's arithmetic after its sample loop (pathtrace.py:340-438, pt.hip), sums in
-// sample order; the display colour and HDR, or with the features (s.denoise) the noisy HDR and
-// the features.
-template <bool kFeatures>
-MJ_DEV void resolve_pixel(const WfGroup& G, const MjpbrSettings& s, int w, int cidx, int tx, int ty, const float* region) {
-  const int lp = threadIdx.x;
+MJ_HOST_DEV inline int64_t region_floats(const MjpbrSettings& s) {
+  return (int64_t)tiles_per_item(s.path_samples) * kTilePixels * s.path_samples * slot_floats(s);
+}
+
+// Work items of one world of a group (each camera's tiles in runs of tiles_per_item), and the
+// first item of every camera.
+MJ_HOST_DEV inline int item_layout(const TileMap& tm, int spp, int32_t* cam_base) {
+  const int t = tiles_per_item(spp);
+  int items = 0;
+  for (int c = 0; c < tm.nrender; ++c) {
+    if (cam_base) cam_base[c] = items;
+    items += (tm.tile_base[c + 1] - tm.tile_base[c] + t - 1) / t;
+  }
+  if (cam_base) cam_base[tm.nrender] = items;
+  return items;
+}
+
+// Pixel of item pixel q (tile q >> 8 of the item's run, from tile `first_tile` of its camera).
+MJ_DEV void item_pixel(int first_tile, int tiles_x, int q, int* px, int* py) {
+  const int tl = first_tile + (q >> 8), lp = q & 255;
+  const int tx = tl % tiles_x, ty = tl / tiles_x;
   const int blk = lp >> 6, ln = lp & 63;
-  const int px = tx * kTile + (blk & 1) * 8 + (ln & 7), py = ty * kTile + (blk >> 1) * 8 + (ln >> 3);
+  *px = tx * kTile + (blk & 1) * 8 + (ln & 7);
+  *py = ty * kTile + (blk >> 1) * 8 + (ln >> 3);
+}
+
+// Pixel q of a finished item from the workgroup's sample region, where sample k of item pixel q
+// sits in slot (k << item_bits) | q (a wave's lanes read adjacent slots): the per-pixel kernel's
+// arithmetic after its sample loop (pathtrace.py:340-438, pt.hip), sums in sample order; the
+// display colour and HDR, or with the features (s.denoise) the noisy HDR and the features.
+template <bool kFeatures>
+MJ_DEV void resolve_pixel(const WfGroup& G, const MjpbrSettings& s, int w, int cidx, int first_tile, int q, const float* region) {
+  int px, py;
+  item_pixel(first_tile, G.tm.tiles_x[cidx], q, &px, &py);
   Pixel pix;
   if (!setup_pixel(G.sc.cams, G.out, s, w, cidx, px, py, G.tm.ray_offset[cidx], &pix) || !pix.want_rgb) return;
   const int spp = s.path_samples, sf = kFeatures ? 12 : 3;
   const float exposure = G.sc.env.exposure[w % G.sc.env.exposure_n];
   const MjpbrOutputs& out = G.out;
   const int64_t adr = out.rgb_adr[cidx] + pix.pixelid;
-  const float* sl = region + (int64_t)lp * sf;
-  const int step = kTilePixels * sf;
+  const float* sl = region + (int64_t)q * sf;
+  const int step = (1 << item_bits(spp)) * sf;
   V3 hdr = v3s(0.0f);
   const float fn = (float)spp;
   if (!kFeatures) {
@@ -171,27 +203,34 @@ MJ_DEV void resolve_pixel(const WfGroup& G, const MjpbrSettings& s, int w, int c
 // worlds' records in place from world_stage (a per-item branch; 3 % slower, so only batches with
 // such a group take this instance). kFeatures (= s.denoise): the first-hit features are kept per
 // sample; as a runtime flag they stayed live, and spilled, in the path loop of every launch
-// (material_grid at 16 spp 74.3 against 78.4 ms without them). Workgroup b keeps its tile's
-// samples in region b of `slots` and resolves the tile at the top of the next item iteration,
+// (material_grid at 16 spp 74.3 against 78.4 ms without them). Workgroup b keeps its item's
+// samples in region b of `slots` and resolves the item at the top of the next item iteration,
 // after the barrier that ends it.
-template <int kWavesPerEu, bool kCount, bool kGlobal, bool kFeatures>
+//
+// kTiles: tiles a work item (tiles_per_item, 4 at 1 sample a pixel): its own instance, since the
+// single-tile instance keeps the tile's corner uniform where the general one looks each path's
+// tile up (1-3 % slower at 16 samples, 5 % with meshes at 8 waves).
+template <int kWavesPerEu, bool kCount, bool kGlobal, bool kFeatures, int kTiles>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kWavesPerEu, kWavesPerEu)))
 wf_kernel(const WfHeader* __restrict__ hdr, int* counter, float* slots, MjpbrSettings s, int seed, int refill, int64_t item_limit,
           uint64_t* ray_counts) {
   extern __shared__ float lds[];
   __shared__ int s_item;
   __shared__ int s_pool;
+  __shared__ int2 s_corner[kTiles];
   const int lane = threadIdx.x & 63;
   const int64_t nitems = item_limit > 0 && item_limit < hdr->nitems ? item_limit : hdr->nitems;
   const int spp = s.path_samples;
-  const int pool = kTilePixels * spp;
+  constexpr int bits = kTiles == 4 ? 10 : 8, per_item = kTiles;
   const int sf = kFeatures ? 12 : 3;
   float* region = slots + (int64_t)blockIdx.x * region_floats(s);
   RayCount rays{0u, 0u, 0u};
-  int done_g = -1, done_w = 0, done_cidx = 0, done_tx = 0, done_ty = 0;  // the tile to resolve
+  int done_g = -1, done_w = 0, done_cidx = 0, done_first = 0, done_tiles = 0;  // the item to resolve
   for (;;) {
     __syncthreads();  // every wave is done with the previous item's LDS, pool and sample slots
-    if (done_g >= 0) resolve_pixel<kFeatures>(hdr->groups[done_g], s, done_w, done_cidx, done_tx, done_ty, region);
+    if (done_g >= 0)
+      for (int t = 0; t < (kTiles == 1 ? 1 : done_tiles); ++t)
+        resolve_pixel<kFeatures>(hdr->groups[done_g], s, done_w, done_cidx, done_first, t * kTilePixels + threadIdx.x, region);
     if (threadIdx.x == 0) {
       s_item = atomicAdd(counter, 1);
       s_pool = 0;
@@ -203,17 +242,35 @@ wf_kernel(const WfHeader* __restrict__ hdr, int* counter, float* slots, MjpbrSet
     const WfGroup& G = hdr->groups[g];
     const MjpbrScene& sc = G.sc;
     const int64_t local = item - G.item_base;
-    const int w = (int)(local / G.tm.tiles_per_world);
-    const int t = (int)(local - (int64_t)w * G.tm.tiles_per_world);
+    const int w = (int)(local / G.items_per_world);
+    const int r = (int)(local - (int64_t)w * G.items_per_world);
     int cidx = 0;
-    while (cidx + 1 < G.tm.nrender && t >= G.tm.tile_base[cidx + 1]) ++cidx;
-    const int tl = t - G.tm.tile_base[cidx];
-    const int tx = tl % G.tm.tiles_x[cidx], ty = tl / G.tm.tiles_x[cidx];
+    while (cidx + 1 < G.tm.nrender && r >= G.item_cam_base[cidx + 1]) ++cidx;
+    // the item: tiles first_tile .. + ntiles of camera cidx, its paths pixel by pixel
+    const int first_tile = (r - G.item_cam_base[cidx]) * per_item;
+    const int ntiles = min(per_item, G.tm.tile_base[cidx + 1] - G.tm.tile_base[cidx] - first_tile);
+    const int pool = ntiles * kTilePixels * spp;
+    // the corner of each tile of the item: one uniform value, or with several tiles in LDS (read
+    // where a path starts and ends: registers held through the path loop would spill)
+    const int2 corner0 = make_int2((first_tile % G.tm.tiles_x[cidx]) * kTile, (first_tile / G.tm.tiles_x[cidx]) * kTile);
+    if (kTiles > 1) {
+      if (threadIdx.x < kTiles) {
+        const int tl = first_tile + min((int)threadIdx.x, ntiles - 1);
+        s_corner[threadIdx.x] = make_int2((tl % G.tm.tiles_x[cidx]) * kTile, (tl / G.tm.tiles_x[cidx]) * kTile);
+      }
+      __syncthreads();  // the corners before any path reads them (global_world need not wait)
+    }
+    auto pixel_of = [&](int q, int* px, int* py) {
+      const int lp = q & 255, blk = lp >> 6, ln = lp & 63;
+      const int2 c = kTiles == 1 ? corner0 : s_corner[q >> 8];
+      *px = c.x + (blk & 1) * 8 + (ln & 7);
+      *py = c.y + (blk >> 1) * 8 + (ln >> 3);
+    };
     done_g = g;
     done_w = w;
     done_cidx = cidx;
-    done_tx = tx;
-    done_ty = ty;
+    done_first = first_tile;
+    done_tiles = ntiles;
     // (G is the same for the whole workgroup, so both branches keep their barriers uniform)
     SceneView sv = kGlobal && G.global ? global_world(sc, w, lds, G.tm.mesh_cache_nodes, G.tm.mesh_cache_n)
                                        : stage_world(sc, w, lds, G.tm.mesh_cache_nodes, G.tm.mesh_cache_n);
@@ -228,7 +285,7 @@ wf_kernel(const WfHeader* __restrict__ hdr, int* counter, float* slots, MjpbrSet
     PathState st;
     Primary first;
     bool alive = false, pool_done = false;
-    int pseed = 0, slot = 0;  // slot: sample * 256 + tile pixel (the path's sample and pixel)
+    int pseed = 0, slot = 0;  // slot: (sample << bits) | item pixel (the path's sample and pixel)
     float dcz = 0.0f;
     for (;;) {
       const uint64_t dead = __ballot(!alive);
@@ -242,9 +299,9 @@ wf_kernel(const WfHeader* __restrict__ hdr, int* counter, float* slots, MjpbrSet
         } else if (!alive) {
           const int pid = base + lanes_below(dead);
           if (pid < pool) {
-            const int lp = pid / spp, sample = pid - lp * spp;  // a pixel's samples on adjacent lanes
-            const int blk = lp >> 6, ln = lp & 63;
-            const int px = tx * kTile + (blk & 1) * 8 + (ln & 7), py = ty * kTile + (blk >> 1) * 8 + (ln >> 3);
+            const int q = pid / spp, sample = pid - q * spp;  // a pixel's samples on adjacent lanes
+            int px, py;
+            pixel_of(q, &px, &py);
             Pixel pix;
             if (setup_pixel(sc.cams, G.out, s, w, cidx, px, py, ray_offset, &pix)) {
               V3 o, d, dc;
@@ -261,7 +318,7 @@ wf_kernel(const WfHeader* __restrict__ hdr, int* counter, float* slots, MjpbrSet
                 pseed = seed + w * total_rays + pix.rayid;
                 path_primary(pix, s, pseed, sample, &o, &d, &md, &dc);
                 path_begin(&st, &first, o, d, md);
-                slot = sample * kTilePixels + lp;
+                slot = (sample << bits) | q;
                 dcz = dc.z;
                 alive = true;
               }
@@ -270,7 +327,7 @@ wf_kernel(const WfHeader* __restrict__ hdr, int* counter, float* slots, MjpbrSet
         }
       }
       if (ndead == 64 && pool_done) break;
-      if (alive && !path_step(c, &st, head_dir, pseed, slot >> 8, &first)) {
+      if (alive && !path_step(c, &st, head_dir, pseed, slot >> bits, &first)) {
         alive = false;
         float* sl = region + (int64_t)slot * sf;
         st3(sl, st.radiance);
@@ -279,9 +336,10 @@ wf_kernel(const WfHeader* __restrict__ hdr, int* counter, float* slots, MjpbrSet
           st3(sl + 6, first.normal);
           st3(sl + 9, first.emission);
         }
-        if (slot < kTilePixels) {  // sample 0
-          const int blk = slot >> 6, ln = slot & 63;
-          const int pixelid = (ty * kTile + (blk >> 1) * 8 + (ln >> 3)) * sc.cams.cam_res[cidx][0] + tx * kTile + (blk & 1) * 8 + (ln & 7);
+        if ((slot >> bits) == 0) {  // sample 0
+          int px, py;
+          pixel_of(slot, &px, &py);
+          const int pixelid = py * sc.cams.cam_res[cidx][0] + px;
           const float depth = first.hit.geom_id != -1 ? first.hit.dist * -dcz : 0.0f;
           if (want_seg) {
             int2 seg = first.hit.geom_id != -1 ? make_int2(first.hit.geom_id, 5) : make_int2(-1, -1);
@@ -316,13 +374,13 @@ int device_cus() {
 constexpr int kMaxGroupsPerCu = 16;
 
 // Work items (16x16 tiles) of the batch, or a negative code when a group's cameras are invalid.
-int64_t count_items(const MjpbrScene* scenes, int ngroup) {
+int64_t count_items(const MjpbrScene* scenes, int ngroup, int spp) {
   int64_t items = 0;
   for (int g = 0; g < ngroup; ++g) {
     TileMap tm;
     int nlaunch;
     if (make_tile_map(&scenes[g], &tm, &nlaunch)) return -1;
-    items += (int64_t)tm.tiles_per_world * scenes[g].geo.nworld;
+    items += (int64_t)item_layout(tm, spp, nullptr) * scenes[g].geo.nworld;
   }
   return items;
 }
@@ -333,7 +391,7 @@ using namespace mjpbr;
 
 extern "C" int64_t mjpbr_pt_wavefront_workspace(const MjpbrScene* scenes, int ngroup, const MjpbrSettings* s, int groups_per_cu) {
   if (ngroup < 1 || ngroup > kWfMaxGroups || s->path_samples < 1 || groups_per_cu < 1 || groups_per_cu > kMaxGroupsPerCu) return -1;
-  const int64_t items = count_items(scenes, ngroup);
+  const int64_t items = count_items(scenes, ngroup, s->path_samples);
   if (items < 1) return -1;
   int64_t regions = (int64_t)device_cus() * groups_per_cu;
   if (regions > items) regions = items;
@@ -380,7 +438,8 @@ extern "C" int mjpbr_pt_wavefront_prepare(const MjpbrScene* scenes, const MjpbrO
     b += plan_mesh_cache(&scenes[g], s->has_meshes != 0, b, &G.tm);
     lds = b > lds ? b : lds;
     G.item_base = items;
-    items += (int64_t)G.tm.tiles_per_world * scenes[g].geo.nworld;
+    G.items_per_world = item_layout(G.tm, s->path_samples, G.item_cam_base);
+    items += (int64_t)G.items_per_world * scenes[g].geo.nworld;
     pixels += (int64_t)scenes[g].geo.nworld * scenes[g].cams.total_rays;
   }
   if (items >= (1ll << 31)) {
@@ -433,15 +492,21 @@ extern "C" int mjpbr_pt_wavefront_launch(void* workspace, const MjpbrWavefrontPl
   using WfFn = void (*)(const WfHeader*, int*, float*, MjpbrSettings, int, int, int64_t, uint64_t*);
   const bool count = cfg->ray_counts != nullptr;
   const bool global = plan->nstaged > 0;
+  const bool features = s.denoise != 0;
   if (cfg->waves_per_eu != 4 && cfg->waves_per_eu != 8)
     return set_error(MJPBR_EINVAL, "wavefront: waves_per_eu must be 4 or 8, got %d", cfg->waves_per_eu);
   // [waves 8][count][global][features]
   static const WfFn kFns[2][2][2][2] = {
-      {{{wf_kernel<4, false, false, false>, wf_kernel<4, false, false, true>}, {wf_kernel<4, false, true, false>, wf_kernel<4, false, true, true>}},
-       {{wf_kernel<4, true, false, false>, wf_kernel<4, true, false, true>}, {wf_kernel<4, true, true, false>, wf_kernel<4, true, true, true>}}},
-      {{{wf_kernel<8, false, false, false>, wf_kernel<8, false, false, true>}, {wf_kernel<8, false, true, false>, wf_kernel<8, false, true, true>}},
-       {{wf_kernel<8, true, false, false>, wf_kernel<8, true, false, true>}, {wf_kernel<8, true, true, false>, wf_kernel<8, true, true, true>}}}};
-  const WfFn fn = kFns[cfg->waves_per_eu == 8][count][global][s.denoise != 0];
+      {{{wf_kernel<4, false, false, false, 1>, wf_kernel<4, false, false, true, 1>}, {wf_kernel<4, false, true, false, 1>, wf_kernel<4, false, true, true, 1>}},
+       {{wf_kernel<4, true, false, false, 1>, wf_kernel<4, true, false, true, 1>}, {wf_kernel<4, true, true, false, 1>, wf_kernel<4, true, true, true, 1>}}},
+      {{{wf_kernel<8, false, false, false, 1>, wf_kernel<8, false, false, true, 1>}, {wf_kernel<8, false, true, false, 1>, wf_kernel<8, false, true, true, 1>}},
+       {{wf_kernel<8, true, false, false, 1>, wf_kernel<8, true, false, true, 1>}, {wf_kernel<8, true, true, false, 1>, wf_kernel<8, true, true, true, 1>}}}};
+  // items of four tiles (1 sample a pixel): 4 waves a SIMD whatever waves_per_eu asks (the
+  // images do not depend on it)
+  static const WfFn kFns4[2][2][2] = {
+      {{wf_kernel<4, false, false, false, 4>, wf_kernel<4, false, false, true, 4>}, {wf_kernel<4, false, true, false, 4>, wf_kernel<4, false, true, true, 4>}},
+      {{wf_kernel<4, true, false, false, 4>, wf_kernel<4, true, false, true, 4>}, {wf_kernel<4, true, true, false, 4>, wf_kernel<4, true, true, true, 4>}}};
+  const WfFn fn = tiles_per_item(s.path_samples) == 4 ? kFns4[count][global][features] : kFns[cfg->waves_per_eu == 8][count][global][features];
   hipLaunchKernelGGL(fn, dim3((unsigned)grid), dim3(256), plan->lds_bytes, st, hdr, counter, slots, s, seed, refill, (int64_t)cfg->item_limit,
                      cfg->ray_counts);
   return check_launch("wf_kernel");

@@ -214,7 +214,8 @@ def create_render_context(
 
   `path_kernel` picks the path tracer's kernel: "wavefront" (a persistent kernel that refills
   lanes as paths end), "megakernel" (one thread per pixel, the reference's scheme) or "auto"
-  (the default: the wavefront kernel where it measured faster, at two or more samples a pixel).
+  (the default: the wavefront kernel where it measured faster, everywhere but at one sample a
+  pixel on scenes of more than 65,536 rendered mesh triangles).
   Both give identical images; contexts with
   more cameras than one launch takes, and renders of hundreds of samples a pixel (sample regions
   beyond 2 GiB), use the megakernel.

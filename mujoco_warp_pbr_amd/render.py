@@ -389,18 +389,21 @@ def render_path_samples(m, d, rc: PbrRenderContext, seed: int = 0) -> None:
   _path_trace(m, d, rc, settings_struct(rc), seed, _stream_of(d.geom_xpos))
 
 
-# rc.path_kernel "auto" takes the wavefront kernel at 2+ samples a pixel (where lanes refill with
-# new paths, and a wave's new paths share few pixels) and the per-pixel kernel at 1. Measured at
-# 1024 worlds (ms, per-pixel kernel vs wavefront): the reference suite's scenes at 16 spp + a-trous
-# 97.1 vs 77.1 and 121.6 vs 108.7; the full-feature scene at 4 spp 256^2 280.2 vs 237.4, at 1 spp
-# 128^2 23.8 vs 24.4; the mesh-heavy scene at 4 spp 128^2 191.5 vs 173.9, at 1 spp 256^2 182.6 vs
-# 194.0 (its launch shape: wavefront.render_config).
+# rc.path_kernel "auto" takes the wavefront kernel (lanes refill with new paths, and a wave's new
+# paths share few pixels) except at 1 sample a pixel on scenes of more than 64k rendered mesh
+# triangles. Measured at 1024 worlds (ms, per-pixel kernel vs wavefront): the reference suite's
+# scenes at 16 spp + a-trous 97.2 vs 77.5 and 121.7 vs 109.3; the full-feature scene (4k mesh
+# triangles) at 4 spp 256^2 275.3 vs 239.0, at 1 spp 128^2 22.7 vs 21.2 (work items of four
+# tiles); the mesh-heavy scene (200k) at 4 spp 128^2 192.0 vs 174.8, at 1 spp 256^2 182.9 vs 205.3.
+AUTO_MEGAKERNEL_MESH_TRIANGLES = 1 << 16
+
+
 def _wavefront_pays(rc: PbrRenderContext) -> bool:
   if rc.path_kernel == "wavefront":
     return True
   if rc.path_kernel == "megakernel":
     return False
-  return int(rc.path_samples) >= 2
+  return int(rc.path_samples) >= 2 or int(rc.base.mesh_triangles) <= AUTO_MEGAKERNEL_MESH_TRIANGLES
 
 
 def _path_trace(m, d, rc: PbrRenderContext, s, seed: int, stream: int) -> None:

@@ -139,12 +139,15 @@ def _full_setup(nworld):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("kernel", ["wavefront", "megakernel"])
 @pytest.mark.parametrize("seed", [0, 1, 2])
-def test_gpu_per_sample_radiance_matches_oracle(cuda, seed):
-  """One path per pixel (sample 0): the GPU follows the oracle's random decisions."""
+def test_gpu_per_sample_radiance_matches_oracle(cuda, seed, kernel):
+  """One path per pixel (sample 0): the GPU follows the oracle's random decisions, through the
+  wavefront kernel (work items of four tiles) and the per-pixel kernel (its 6-wave instance of
+  scenes with light meshes)."""
   mjm = scenes.full_scene(0, res=32)
   gpu, ref, _ = _pt_both(mjm, 4, (32, 32), seed=seed, setup=_full_setup(4), envmap=scenes.procedural_sky(0, 64, 128),
-                         path_samples=1, max_depth=8, **PATH)
+                         path_samples=1, max_depth=8, path_kernel=kernel, **PATH)
   np.testing.assert_array_equal(gpu["seg"], ref["seg"])
   np.testing.assert_allclose(gpu["depth"], ref["depth"], rtol=1e-5, atol=1e-7)
   diff = np.abs(gpu["hdr"] - ref["hdr"]).max(axis=-1)
@@ -159,7 +162,7 @@ def test_gpu_per_sample_radiance_without_meshes_matches_oracle(cuda):
   scene of test_configs_gpu.py the 8-wave one) follows the oracle's random decisions too."""
   mjm = scenes.primitives_scene(3, 32)
   gpu, ref, (_, _, rc) = _pt_both(mjm, 3, (32, 32), seed=4, envmap=scenes.procedural_sky(2, 64, 128), path_samples=1, max_depth=8,
-                                  **PATH)
+                                  path_kernel="megakernel", **PATH)
   assert not rc.base.has_meshes
   np.testing.assert_array_equal(gpu["seg"], ref["seg"])
   np.testing.assert_allclose(gpu["depth"], ref["depth"], rtol=1e-5, atol=1e-7)

@@ -318,23 +318,29 @@ def _bare_scene(S, nworld, res):
 
 def test_workspace_holds_one_sample_region_a_workgroup():
   """mjpbr_pt_wavefront_workspace: a fixed header plus one region a persistent workgroup (at most
-  one a tile) of 256 pixels x samples x 3 floats (12 with the denoiser features)."""
+  one a work item) of tiles x 256 pixels x samples x 3 floats (12 with the denoiser features); a
+  work item is a run of one camera's tiles, 4 of them at 1 sample a pixel, else 1."""
   from mujoco_warp_pbr_amd import _abi
 
   lib, S = _abi.library(), _abi.structs()
-  scenes_arr = (S["MjpbrScene"] * 1)(_bare_scene(S, 3, (40, 24)))  # 3 x 2 tiles a world: 18 items
+  scenes_arr = (S["MjpbrScene"] * 1)(_bare_scene(S, 3, (40, 24)))  # 3 x 2 tiles a world
   s = S["MjpbrSettings"]()
 
   def size(spp, denoise=0, gpc=16):
     s.path_samples, s.denoise = spp, denoise
     return lib.mjpbr_pt_wavefront_workspace(ctypes.addressof(scenes_arr), 1, ctypes.addressof(s), gpc)
 
-  region = size(2) - size(1)
-  assert region == 18 * 256 * 3 * 4
-  fixed = size(1) - region
-  assert fixed > 0 and size(7) == fixed + 7 * region
-  assert size(5, denoise=1) == fixed + 4 * 5 * region
-  assert size(1, gpc=1) == size(1)  # 18 tiles: fewer regions than any grid
+  def regions(spp, tiles_per_item, floats=3):
+    items = 3 * -(-6 // tiles_per_item)  # worlds x runs of the camera's 6 tiles
+    return items * tiles_per_item * 256 * spp * floats * 4
+
+  fixed = size(4) - regions(4, 1)
+  assert fixed > 0
+  assert size(7) == fixed + regions(7, 1)
+  assert size(2) == fixed + regions(2, 1) and size(3) == fixed + regions(3, 1)
+  assert size(1) == fixed + regions(1, 4)
+  assert size(5, denoise=1) == fixed + regions(5, 1, 12)
+  assert size(1, gpc=1) == size(1)  # 6 items: fewer regions than any grid
   for bad in (0, 17):
     assert size(1, gpc=bad) == -1
   assert size(0) == -1
@@ -358,11 +364,13 @@ def test_gpu_render_beyond_the_workspace_budget_takes_the_megakernel(cuda, monke
 
 
 def test_auto_path_kernel_choice():
-  """path_kernel "auto": the wavefront kernel at 2+ samples a pixel, else the per-pixel kernel;
-  "wavefront" and "megakernel" force one. The wavefront launch shape: 8 waves a SIMD on scenes of
-  more than 65,536 rendered mesh triangles (counted once a geom), 4 otherwise."""
+  """path_kernel "auto": the wavefront kernel, except the per-pixel kernel at 1 sample a pixel on
+  scenes of more than 65,536 rendered mesh triangles (counted once a geom); "wavefront" and
+  "megakernel" force one. The wavefront launch shape: 8 waves a SIMD on such mesh-heavy scenes,
+  4 otherwise."""
   from mujoco_warp_pbr_amd import io
   from mujoco_warp_pbr_amd import wavefront
+  from mujoco_warp_pbr_amd.render import AUTO_MEGAKERNEL_MESH_TRIANGLES
   from mujoco_warp_pbr_amd.render import _wavefront_pays
 
   mjm = _scene()
@@ -377,13 +385,16 @@ def test_auto_path_kernel_choice():
   assert _wavefront_pays(rc)
   assert wavefront.render_config(rc) == wavefront.RENDER_CONFIG
   rc.path_samples = 1
-  assert not _wavefront_pays(rc)
-  rc.path_kernel = "wavefront"
   assert _wavefront_pays(rc)
-  rc.path_kernel, rc.path_samples = "megakernel", 4
+  rc.base.mesh_triangles = AUTO_MEGAKERNEL_MESH_TRIANGLES + 1
   assert not _wavefront_pays(rc)
-  rc.base.mesh_triangles = wavefront.MESH_HEAVY_TRIANGLES + 1
   assert wavefront.render_config(rc) == wavefront.MESH_HEAVY_CONFIG
+  rc.path_samples = 2
+  assert _wavefront_pays(rc)
+  rc.path_kernel = "megakernel"
+  assert not _wavefront_pays(rc)
+  rc.path_kernel, rc.path_samples = "wavefront", 1
+  assert _wavefront_pays(rc)
   assert wavefront.MESH_HEAVY_CONFIG.waves_per_eu == 8 and wavefront.RENDER_CONFIG.waves_per_eu == 4
   with pytest.raises(ValueError, match="path_kernel"):
     make(mjm, 1, device="cpu", cam_res=(16, 16), **{**PT, "path_kernel": "fast"})
