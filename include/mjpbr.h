@@ -311,7 +311,8 @@ typedef struct MjpbrAtrousParams {
 
 /* Persistent wavefront path tracer (no counterpart in the reference): tuning knobs of a launch. */
 typedef struct MjpbrWavefrontConfig {
-  int32_t waves_per_eu;      /* 4 or 8: register budget / occupancy variant of the kernel */
+  int32_t waves_per_eu;      /* 4 or 8: register budget / occupancy variant of the kernel (at 1
+                                sample a pixel, whose work items hold four tiles, always 4) */
   int32_t groups_per_cu;     /* persistent workgroups per compute unit (1..16) */
   int32_t refill_threshold;  /* a wave refills its lanes once this many paths ended (1..64) */
   int32_t item_limit;        /* > 0: only the first item_limit tiles (autotuner samples) */

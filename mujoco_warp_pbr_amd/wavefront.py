@@ -34,7 +34,7 @@ class WavefrontConfig:
   """Launch shape of the persistent kernel (see include/mjpbr.h MjpbrWavefrontConfig)."""
 
   # defaults: the autotuner's pick on 4096 worlds (64^2 / 128^2 / 256^2 mixed), 4 spp, depth 8
-  waves_per_eu: int = 4
+  waves_per_eu: int = 4  # 4 or 8; at 1 sample a pixel (work items of four tiles) the kernel runs 4
   groups_per_cu: int = 8
   refill_threshold: int = 32
   item_limit: int = 0
