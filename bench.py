@@ -9,7 +9,9 @@ resident in HBM before the timed region.
 
 Prints one JSON line: Mpixels/s, the roofline of the render kernel (algorithmic HBM bytes of
 one launch / its average duration measured with HIP events on the launch stream; `traffic` from
-the committed rocprofv3 counter runs), the CPU oracle (scalar C restatement, threads over
+the committed rocprofv3 counter runs), `valu_roofline` (the kernel's actual limit: its VALU
+instructions from the committed SQ_INSTS_VALU run over the measured launch time, against the
+GPU's VALU issue rate), the CPU oracle (scalar C restatement, threads over
 worlds) timed on a bounded sample of the same workload on this host, `parity` (the sampled
 worlds of the timed render against the oracle's images: segmentation bit-exact, depth 1e-5
 relative, sRGB8 within 1), `small_case` (16 worlds x 64^2 on the GPU and on every host core the
@@ -95,6 +97,31 @@ def algorithmic_bytes(mjm, rc, nworld: int, npix: int) -> tuple[float, float]:
 
 
 TRAFFIC_PROFILE = ROOT / "profiles" / "rt_kernel_full_scene_1024x128_traffic.md"
+
+
+RT_PROFILE = ROOT / "profiles" / "rt_kernel_full_scene_1024x128.md"
+
+
+def valu_roofline(insts, kernel_ms: float):
+  if insts is None:
+    return None
+  achieved = insts / (kernel_ms * 1e-3)
+  return {"bound": "valu", "instructions_per_launch": insts, "achieved": round(achieved, -6), "peak": VALU_PEAK_WAVE_INSTS,
+          "unit": "wave64 VALU instructions/s", "frac": round(achieved / VALU_PEAK_WAVE_INSTS, 4)}
+# VALU issue of one MI355X: 256 compute units x 4 SIMDs, a wave64 FP32 instruction every 4 cycles
+# of a 16-lane SIMD, at the 2400 MHz peak engine clock (wave instructions per second).
+VALU_PEAK_WAVE_INSTS = 256 * 4 * 2.4e9 / 4
+
+
+def measured_valu(nworld: int, res: int):
+  """VALU wave instructions per launch of the render kernel from the committed rocprofv3
+  SQ_INSTS_VALU run, for the workload it was taken on; None for any other shape."""
+  if (nworld, res) != (1024, 128) or not RT_PROFILE.exists():
+    return None
+  for line in RT_PROFILE.read_text().splitlines():
+    if line.startswith("valu_instructions_per_launch:"):
+      return float(line.split(":", 1)[1])
+  return None
 
 
 def measured_traffic(nworld: int, res: int):
@@ -696,6 +723,9 @@ def main(argv=None) -> int:
       "traffic": measured_traffic(args.nworld, args.res),
       "bytes_per_pixel": round(bpp, 3),
     },
+    # the kernel's actual limit: VALU issue (SQ_INSTS_VALU of the committed profile over the
+    # measured launch time, against one wave64 instruction every 4 cycles on every SIMD)
+    "valu_roofline": valu_roofline(measured_valu(args.nworld, args.res), kernel_ms),
     "cpu_baseline": baseline,
     "parity": parity,
     "small_case": configs0,
