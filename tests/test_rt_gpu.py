@@ -239,3 +239,23 @@ def test_many_cameras_path_traced_with_atrous_match_oracle(cuda):
   np.testing.assert_allclose(gpu["depth"], ref["depth"], rtol=1e-5, atol=1e-7)
   diff = np.abs(gpu["hdr"] - ref["hdr"]).max(axis=-1)
   assert np.mean(diff <= 1e-3) >= 0.999, f"{np.mean(diff > 1e-3):.4%} of pixels differ, max {diff.max()}"
+
+
+def test_one_lane_a_sample_with_a_depth_only_camera_matches_oracle(cuda):
+  """2x2 supersampling (one lane a sample: a pixel's four lanes) with a ragged 24x12 camera that
+  renders depth and segmentation only (one lane of the four traces it) beside a 16x16 RGB camera,
+  soft shadows on: every buffer follows the oracle."""
+  from mujoco_warp_pbr_amd import _abi
+  from mujoco_warp_pbr_amd.render import settings_struct
+  from tests.test_readback import _two_camera_scene
+
+  mjm, _ = _two_camera_scene()
+
+  def soft(m, d, rc):
+    rc.light_bulbradius = torch.full_like(rc.light_bulbradius, 0.05)
+
+  gpu, ref, (_, _, rc) = _render_both(mjm, 2, None, setup=soft, render_rgb=[True, False], render_depth=[True, True],
+                                      render_seg=[True, True], render_guides=True, samples_per_pixel=2, shadow_samples=4)
+  assert _abi.library().mjpbr_rt_kernel_features(_abi.byref(settings_struct(rc))) & 128
+  compare_rt(gpu, ref)
+  np.testing.assert_allclose(gpu["normal"], ref["normal"], atol=1e-5)
