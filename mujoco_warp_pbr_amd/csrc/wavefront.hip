@@ -126,6 +126,33 @@ MJ_DEV void item_pixel(int first_tile, int tiles_x, int q, int* px, int* py) {
   *py = ty * kTile + (blk >> 1) * 8 + (ln >> 3);
 }
 
+// A pixel's outputs from its sample sums (in sample order) over `fn` samples: the per-pixel
+// kernel's arithmetic after its sample loop (pathtrace.py:340-438, pt.hip); the display colour and
+// HDR, or with the features the noisy HDR and the features.
+template <bool kFeatures>
+MJ_DEV void write_pixel(const MjpbrOutputs& out, const MjpbrSettings& s, int w, int64_t adr, float exposure, float fn, V3 hdr,
+                        V3 albedo_sum, V3 normal_sum, V3 emission_sum, float lum_sq) {
+  hdr = hdr / fn;
+  if (!kFeatures) {
+    if (s.render_hdr) st3(out.hdr + 3 * ((int64_t)w * out.hdr_stride + adr), hdr);
+    out.rgb[(int64_t)w * out.rgb_stride + adr] = encode_display(display_transform(hdr * exposure, s.tone_mapper));
+    return;
+  }
+  const int64_t a = (int64_t)w * out.aov_stride + adr;
+  st3(out.hdr + 3 * ((int64_t)w * out.hdr_stride + adr), hdr);
+  const V3 alb = albedo_sum / fn;
+  out.aov_albedo[3 * a] = float_to_half(alb.x);
+  out.aov_albedo[3 * a + 1] = float_to_half(alb.y);
+  out.aov_albedo[3 * a + 2] = float_to_half(alb.z);
+  st3(out.aov_emission + 3 * a, emission_sum * (exposure / fn));
+  if (len_sq(normal_sum) > 0.0f) normal_sum = normalize(normal_sum);
+  out.aov_normal[3 * a] = float_to_half(normal_sum.x);
+  out.aov_normal[3 * a + 1] = float_to_half(normal_sum.y);
+  out.aov_normal[3 * a + 2] = float_to_half(normal_sum.z);
+  const float mean_lum = luminance(hdr) * exposure;
+  out.aov_variance[a] = fmaxf(lum_sq / fn - mean_lum * mean_lum, 0.0f) / fn;
+}
+
 // Pixel q of a finished item from the workgroup's sample region, where sample k of item pixel q
 // sits in slot (k << item_bits) | q (a wave's lanes read adjacent slots): the per-pixel kernel's
 // arithmetic after its sample loop (pathtrace.py:340-438, pt.hip), sums in sample order; the
@@ -155,9 +182,7 @@ MJ_DEV void resolve_pixel(const WfGroup& G, const MjpbrSettings& s, int w, int c
       for (int j = 0; j < 8; ++j) hdr += r[j];
     }
     for (; k < spp; ++k, sl += step) hdr += ld3(sl);
-    hdr = hdr / fn;
-    if (s.render_hdr) st3(out.hdr + 3 * ((int64_t)w * out.hdr_stride + adr), hdr);
-    out.rgb[(int64_t)w * out.rgb_stride + adr] = encode_display(display_transform(hdr * exposure, s.tone_mapper));
+    write_pixel<false>(out, s, w, adr, exposure, fn, hdr, hdr, hdr, hdr, 0.0f);
     return;
   }
   V3 albedo_sum = v3s(0.0f), normal_sum = v3s(0.0f), emission_sum = v3s(0.0f);
@@ -183,20 +208,7 @@ MJ_DEV void resolve_pixel(const WfGroup& G, const MjpbrSettings& s, int w, int c
     for (int j = 0; j < 4; ++j) add(v[j]);
   }
   for (; k < spp; ++k, sl += step) add(sl);
-  hdr = hdr / fn;
-  const int64_t a = (int64_t)w * out.aov_stride + adr;
-  st3(out.hdr + 3 * ((int64_t)w * out.hdr_stride + adr), hdr);
-  const V3 alb = albedo_sum / fn;
-  out.aov_albedo[3 * a] = float_to_half(alb.x);
-  out.aov_albedo[3 * a + 1] = float_to_half(alb.y);
-  out.aov_albedo[3 * a + 2] = float_to_half(alb.z);
-  st3(out.aov_emission + 3 * a, emission_sum * (exposure / fn));
-  if (len_sq(normal_sum) > 0.0f) normal_sum = normalize(normal_sum);
-  out.aov_normal[3 * a] = float_to_half(normal_sum.x);
-  out.aov_normal[3 * a + 1] = float_to_half(normal_sum.y);
-  out.aov_normal[3 * a + 2] = float_to_half(normal_sum.z);
-  const float mean_lum = luminance(hdr) * exposure;
-  out.aov_variance[a] = fmaxf(lum_sq / fn - mean_lum * mean_lum, 0.0f) / fn;
+  write_pixel<true>(out, s, w, adr, exposure, fn, hdr, albedo_sum, normal_sum, emission_sum, lum_sq);
 }
 
 // kCount: adds the rays traced to ray_counts (the same image). kGlobal: some group reads its
@@ -225,12 +237,11 @@ wf_kernel(const WfHeader* __restrict__ hdr, int* counter, float* slots, MjpbrSet
   const int sf = kFeatures ? 12 : 3;
   float* region = slots + (int64_t)blockIdx.x * region_floats(s);
   RayCount rays{0u, 0u, 0u};
-  int done_g = -1, done_w = 0, done_cidx = 0, done_first = 0, done_tiles = 0;  // the item to resolve
+  int done_g = -1, done_w = 0, done_cidx = 0, done_first = 0;  // the item to resolve
   for (;;) {
     __syncthreads();  // every wave is done with the previous item's LDS, pool and sample slots
-    if (done_g >= 0)
-      for (int t = 0; t < (kTiles == 1 ? 1 : done_tiles); ++t)
-        resolve_pixel<kFeatures>(hdr->groups[done_g], s, done_w, done_cidx, done_first, t * kTilePixels + threadIdx.x, region);
+    // (items of four tiles hold one sample a pixel, written as their paths end)
+    if (kTiles == 1 && done_g >= 0) resolve_pixel<kFeatures>(hdr->groups[done_g], s, done_w, done_cidx, done_first, threadIdx.x, region);
     if (threadIdx.x == 0) {
       s_item = atomicAdd(counter, 1);
       s_pool = 0;
@@ -270,7 +281,6 @@ wf_kernel(const WfHeader* __restrict__ hdr, int* counter, float* slots, MjpbrSet
     done_w = w;
     done_cidx = cidx;
     done_first = first_tile;
-    done_tiles = ntiles;
     // (G is the same for the whole workgroup, so both branches keep their barriers uniform)
     SceneView sv = kGlobal && G.global ? global_world(sc, w, lds, G.tm.mesh_cache_nodes, G.tm.mesh_cache_n)
                                        : stage_world(sc, w, lds, G.tm.mesh_cache_nodes, G.tm.mesh_cache_n);
@@ -329,12 +339,14 @@ wf_kernel(const WfHeader* __restrict__ hdr, int* counter, float* slots, MjpbrSet
       if (ndead == 64 && pool_done) break;
       if (alive && !path_step(c, &st, head_dir, pseed, slot >> bits, &first)) {
         alive = false;
-        float* sl = region + (int64_t)slot * sf;
-        st3(sl, st.radiance);
-        if (kFeatures) {
-          st3(sl + 3, first.albedo);
-          st3(sl + 6, first.normal);
-          st3(sl + 9, first.emission);
+        if (kTiles == 1) {
+          float* sl = region + (int64_t)slot * sf;
+          st3(sl, st.radiance);
+          if (kFeatures) {
+            st3(sl + 3, first.albedo);
+            st3(sl + 6, first.normal);
+            st3(sl + 9, first.emission);
+          }
         }
         if ((slot >> bits) == 0) {  // sample 0
           int px, py;
@@ -352,6 +364,20 @@ wf_kernel(const WfHeader* __restrict__ hdr, int* counter, float* slots, MjpbrSet
             st3(G.out.normal + 3 * a, first.normal);
           }
           if (kFeatures) G.out.aov_depth[(int64_t)w * G.out.aov_stride + G.out.rgb_adr[cidx] + pixelid] = depth;
+          if (kTiles > 1) {  // one sample a pixel: its outputs now (the resolve's sums of one sample)
+            V3 hdr = v3s(0.0f), albedo_sum = v3s(0.0f), normal_sum = v3s(0.0f), emission_sum = v3s(0.0f);
+            float lum_sq = 0.0f;
+            hdr += st.radiance;
+            if (kFeatures) {
+              albedo_sum += first.albedo;
+              normal_sum += first.normal;
+              emission_sum += first.emission;
+              const float lum = luminance(st.radiance) * exposure;
+              lum_sq += lum * lum;
+            }
+            write_pixel<kFeatures>(G.out, s, w, G.out.rgb_adr[cidx] + pixelid, exposure, 1.0f, hdr, albedo_sum, normal_sum, emission_sum,
+                                   lum_sq);
+          }
         }
       }
     }
